@@ -1,0 +1,127 @@
+"""ctypes binding of include/yarnstep.h (libyarnstep.so)."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+MAX_HANDLES = 16
+
+YC_OK, YC_ERR_NONFINITE, YC_ERR_NOT_CONVERGED, YC_ERR_HIP, YC_ERR_ARG = 0, 1, 2, 3, 4
+
+
+class YcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("yarnstep error %d: %s" % (code, msg))
+        self.code = code
+
+
+class NonFinite(YcError):
+    pass
+
+
+class NotConverged(YcError):
+    pass
+
+
+class YcDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int), ("length", ctypes.c_int),
+        ("hight", ctypes.c_double), ("L", ctypes.c_double), ("R", ctypes.c_double),
+        ("n_yarns", ctypes.c_int),
+        ("rho", ctypes.c_double * 2), ("Y", ctypes.c_double * 2), ("B", ctypes.c_double * 2),
+        ("kc", ctypes.c_double), ("kf", ctypes.c_double), ("df", ctypes.c_double),
+        ("mu", ctypes.c_double), ("S", ctypes.c_double), ("handle_k", ctypes.c_double),
+        ("pattern", ctypes.c_int), ("pose", ctypes.c_int),
+        ("G", ctypes.c_double * 3),
+        ("wind_vel", ctypes.c_double * 3), ("wind_density", ctypes.c_double), ("wind_drag", ctypes.c_double),
+        ("n_handles", ctypes.c_int),
+        ("handles", ctypes.c_int * (2 * MAX_HANDLES)),
+    ]
+
+
+class YcStepInfo(ctypes.Structure):
+    _fields_ = [("cg_iters", ctypes.c_int), ("cg_relres", ctypes.c_double), ("status", ctypes.c_int)]
+
+
+# every symbol include/yarnstep.h declares
+EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state", "yc_get_state",
+           "yc_set_ext_force", "yc_assemble", "yc_export_csr", "yc_export_vec", "yc_export_crossing",
+           "yc_export_normals", "yc_solve", "yc_update", "yc_step", "yc_run", "yc_time_spmv", "yc_spmv_bytes"]
+
+
+def lib_path():
+    return os.path.join(_HERE, "libyarnstep.so")
+
+
+_lib = None
+
+
+def load_library():
+    """Load libyarnstep.so from the package directory; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not os.path.exists(p):
+        raise YcError(YC_ERR_HIP, "libyarnstep.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(p)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    vp = ctypes.c_void_p
+    L.yc_create.argtypes = [ctypes.POINTER(YcDesc), ctypes.POINTER(vp)]
+    L.yc_destroy.argtypes = [vp]
+    L.yc_destroy.restype = None
+    L.yc_last_error.restype = ctypes.c_char_p
+    L.yc_sizes.argtypes = [vp, ip, ip, ip, ip]
+    L.yc_set_state.argtypes = [vp] + [dp] * 6
+    L.yc_get_state.argtypes = [vp] + [dp] * 6
+    L.yc_set_ext_force.argtypes = [vp, dp]
+    L.yc_assemble.argtypes = [vp, ctypes.c_double]
+    L.yc_export_csr.argtypes = [vp, ip, ip, dp, ip]
+    L.yc_export_vec.argtypes = [vp, ctypes.c_int, dp]
+    L.yc_export_crossing.argtypes = [vp, dp, dp, dp]
+    L.yc_export_normals.argtypes = [vp, dp]
+    L.yc_solve.argtypes = [vp, dp, ctypes.c_double, ctypes.c_int, ip, dp]
+    L.yc_update.argtypes = [vp, ctypes.c_double]
+    L.yc_step.argtypes = [vp, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]
+    L.yc_run.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
+    L.yc_time_spmv.argtypes = [vp, ctypes.c_int, dp]
+    L.yc_spmv_bytes.argtypes = [vp]
+    L.yc_spmv_bytes.restype = ctypes.c_double
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc == YC_OK:
+        return rc
+    msg = load_library().yc_last_error().decode()
+    if rc == YC_ERR_NONFINITE:
+        raise NonFinite(rc, msg)
+    if rc == YC_ERR_NOT_CONVERGED:
+        raise NotConverged(rc, msg)
+    raise YcError(rc, msg)
+
+
+def make_desc(spec):
+    d = YcDesc()
+    d.width, d.length = int(spec["width"]), int(spec["length"])
+    d.hight, d.L, d.R = float(spec["hight"]), float(spec["L"]), float(spec["R"])
+    yarns = spec["yarns"]
+    d.n_yarns = len(yarns)
+    for i, y in enumerate(yarns):
+        d.rho[i], d.Y[i], d.B[i] = float(y["rho"]), float(y["Y"]), float(y["B"])
+    for k in ("kc", "kf", "df", "mu", "S", "handle_k"):
+        setattr(d, k, float(spec[k]))
+    d.pattern = {"plain": 0, "twill": 1, "satin": 2}[spec["pattern"]]
+    d.pose = {"upright": 0, "flat": 1}[spec["pose"]]
+    for i in range(3):
+        d.G[i] = float(spec["G"][i])
+        d.wind_vel[i] = float(spec["wind_vel"][i])
+    d.wind_density, d.wind_drag = float(spec["wind_density"]), float(spec["wind_drag"])
+    hs = spec["handles"]
+    if len(hs) > MAX_HANDLES:
+        raise ValueError("at most %d handles" % MAX_HANDLES)
+    d.n_handles = len(hs)
+    for i, (u, v) in enumerate(hs):
+        d.handles[2 * i], d.handles[2 * i + 1] = int(u), int(v)
+    return d

@@ -1,0 +1,181 @@
+"""Host mirror of the reference step API over the C ABI.
+
+``Cloth`` owns one device context (src/Cloth.h:31-82 + src/PhysicsCloth.h:6-78
+collapsed into one object, since the HIP context holds both the state and the
+system); ``PhysicsCloth`` is a thin view that keeps the reference's method names
+(FillMV, get_SpaM-style exports); ``SimStep`` mirrors src/TrainHelper.cpp:15-97 and
+returns False on NaN/Inf as the reference does.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+_dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+_NULL = ctypes.POINTER(ctypes.c_double)()
+
+
+def _arr(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != int(np.prod(shape)):
+        raise ValueError("expected %s doubles, got %d" % (shape, a.size))
+    return a
+
+
+class Cloth:
+    """One cloth on the current HIP device (reference Cloth + PhysicsCloth)."""
+
+    def __init__(self, spec):
+        L = _abi.load_library()
+        self.spec = spec
+        self._desc = _abi.make_desc(spec)
+        h = ctypes.c_void_p()
+        _abi.check(L.yc_create(ctypes.byref(self._desc), ctypes.byref(h)))
+        self._h = h
+        N, Nc, n, ns = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _abi.check(L.yc_sizes(h, ctypes.byref(N), ctypes.byref(Nc), ctypes.byref(n), ctypes.byref(ns)))
+        self.N, self.Nc, self.n, self.nslots = N.value, Nc.value, n.value, ns.value
+        self.width, self.length = spec["width"], spec["length"]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.load_library().yc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- src/Cloth.cpp:350-438 ---
+    def get_ClothState(self):
+        N = self.N
+        s = {k: np.zeros((N, 3)) for k in ("LPos", "LPosFix", "LVel")}
+        s.update({k: np.zeros((N, 2)) for k in ("EPos", "EPosBar", "EVel")})
+        _abi.check(_abi.load_library().yc_get_state(self._h, _dp(s["LPos"]), _dp(s["LPosFix"]), _dp(s["LVel"]),
+                                                    _dp(s["EPos"]), _dp(s["EPosBar"]), _dp(s["EVel"])))
+        return s
+
+    def set_ClothState(self, s):
+        N = self.N
+        a = [_arr(s[k], (N, 3)) for k in ("LPos", "LPosFix", "LVel")] + \
+            [_arr(s[k], (N, 2)) for k in ("EPos", "EPosBar", "EVel")]
+        _abi.check(_abi.load_library().yc_set_state(self._h, *[_dp(x) for x in a]))
+
+    def set_ext_force(self, f):
+        if f is None:
+            _abi.check(_abi.load_library().yc_set_ext_force(self._h, _NULL))
+        else:
+            self._ext = _arr(f, (self.n,))
+            _abi.check(_abi.load_library().yc_set_ext_force(self._h, _dp(self._ext)))
+
+    # --- FillMV + system (src/PhysicsCloth.cpp:46-156, src/TrainHelper.cpp:19-20) ---
+    def assemble(self, h):
+        _abi.check(_abi.load_library().yc_assemble(self._h, float(h)))
+
+    def csr(self):
+        """A (as assembled) -> scipy.sparse.csr_matrix in the reference's global layout."""
+        import scipy.sparse as sp
+        L = _abi.load_library()
+        nnz = ctypes.c_int()
+        ip = ctypes.POINTER(ctypes.c_int)
+        _abi.check(L.yc_export_csr(self._h, ip(), ip(), _NULL, ctypes.byref(nnz)))
+        rp = np.zeros(self.n + 1, dtype=np.int32)
+        cl = np.zeros(nnz.value, dtype=np.int32)
+        val = np.zeros(nnz.value)
+        _abi.check(L.yc_export_csr(self._h, rp.ctypes.data_as(ip), cl.ctypes.data_as(ip), _dp(val), ctypes.byref(nnz)))
+        return sp.csr_matrix((val, cl, rp), shape=(self.n, self.n))
+
+    def vec(self, which):
+        out = np.zeros(self.n)
+        _abi.check(_abi.load_library().yc_export_vec(self._h, {"F": 0, "b": 1, "v": 2, "x": 3}[which], _dp(out)))
+        return out
+
+    def crossing(self):
+        sl, cp, fp = np.zeros(2 * self.Nc), np.zeros(self.Nc), np.zeros(2 * self.Nc)
+        _abi.check(_abi.load_library().yc_export_crossing(self._h, _dp(sl), _dp(cp), _dp(fp)))
+        return sl.reshape(-1, 2), cp, fp.reshape(-1, 2)
+
+    def normals(self):
+        out = np.zeros((self.N, 3))
+        _abi.check(_abi.load_library().yc_export_normals(self._h, _dp(out)))
+        return out
+
+    # --- EigenSolver::apply forward (src/Solvers.cpp:3-31) ---
+    def solve(self, tol=1e-8, maxit=None, x0=None, raise_on_nonconv=False):
+        if maxit is None:
+            maxit = 2 * self.n
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        x0a = _NULL if x0 is None else _dp(_arr(x0, (self.n,)))
+        rc = _abi.load_library().yc_solve(self._h, x0a, float(tol), int(maxit), ctypes.byref(it), ctypes.byref(rr))
+        if rc == _abi.YC_ERR_NOT_CONVERGED and not raise_on_nonconv:
+            return it.value, rr.value, False
+        _abi.check(rc)
+        return it.value, rr.value, True
+
+    # --- Cloth::update (src/Cloth.cpp:111-148) ---
+    def update(self, h):
+        _abi.check(_abi.load_library().yc_update(self._h, float(h)))
+
+    def step(self, h, tol=1e-8, maxit=None, x0=None):
+        """SimStep; returns (iters, relres, status)."""
+        if maxit is None:
+            maxit = 2 * self.n
+        info = _abi.YcStepInfo()
+        x0a = _NULL if x0 is None else _dp(_arr(x0, (self.n,)))
+        rc = _abi.load_library().yc_step(self._h, float(h), x0a, float(tol), int(maxit), ctypes.byref(info))
+        if rc not in (_abi.YC_OK, _abi.YC_ERR_NOT_CONVERGED, _abi.YC_ERR_NONFINITE):
+            _abi.check(rc)
+        return info.cg_iters, info.cg_relres, rc
+
+    def run(self, h, steps, tol=1e-8, maxit=None):
+        if maxit is None:
+            maxit = 2 * self.n
+        tot = ctypes.c_longlong()
+        rc = _abi.load_library().yc_run(self._h, float(h), int(steps), float(tol), int(maxit), ctypes.byref(tot))
+        if rc not in (_abi.YC_OK, _abi.YC_ERR_NOT_CONVERGED):
+            _abi.check(rc)
+        return tot.value, rc
+
+    def time_spmv(self, reps=200):
+        ms = ctypes.c_double()
+        _abi.check(_abi.load_library().yc_time_spmv(self._h, int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def spmv_bytes(self):
+        return _abi.load_library().yc_spmv_bytes(self._h)
+
+
+class PhysicsCloth:
+    """Reference-named view of the assembled system (src/PhysicsCloth.h:27-77)."""
+
+    def __init__(self, cloth):
+        self.cloth = cloth
+
+    def FillMV(self, h):
+        self.cloth.assemble(h)
+
+    def get_SpaA(self):
+        return self.cloth.csr()
+
+    def get_SpaF(self):
+        return self.cloth.vec("F")
+
+    def get_SpaVel(self):
+        return self.cloth.vec("v")
+
+    @property
+    def F_E_is_Slides(self):
+        return self.cloth.crossing()[0]
+
+
+def SimStep(cloth, sim_phy, h, ext_force=None, tol=1e-8, maxit=None):
+    """src/TrainHelper.cpp:15-97: False on NaN/Inf in A, b or the new velocity."""
+    if ext_force is not None:
+        cloth.set_ext_force(ext_force)
+    _, _, rc = cloth.step(h, tol=tol, maxit=maxit)
+    if ext_force is not None:
+        cloth.set_ext_force(None)
+    return rc != _abi.YC_ERR_NONFINITE

@@ -1,0 +1,1093 @@
+// yarnstep.hip -- MI355X (gfx950) implicit yarn-level time step behind the C ABI of
+// include/yarnstep.h.
+//
+// Device layout (FP64 throughout, the reference's dtype, src/Constants.h:9):
+//   node state      structure of arrays, planes of N = W*Lg doubles (x, v, xfix: 3
+//                   planes; uv, uvd, uvbar: 2 planes; crossing normals: 3 planes)
+//   system matrix   159 slot planes of N doubles (ys::NSLOT): the fixed 13-node
+//                   stencil of the yarn grid, columns implicit, no index arrays
+//   CG vectors      5 planes (x, y, z, u, v) over the grid padded by 2 nodes on each
+//                   side, so the stencil product needs no bounds checks; the padding
+//                   and the boundary nodes' u, v entries stay exactly zero
+// Kernels: k_assemble1 (edges, bends, crimp, faces, handles, friction; one thread
+// per node), k_assemble2 (shear, diagonal, finiteness), CG (3 kernels per
+// iteration, replayed from a hipGraph), k_update + k_normals.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/yarnstep.h"
+#include "elements.hpp"
+
+using namespace ys;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) return fail(YC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int BLK = 256;
+
+struct CGScal {
+    double rz, pAp, beta, rr, bb, thr;
+    int done, iters, maxit, zero_rhs;
+    unsigned int ticket[4];
+};
+
+// ---------------------------------------------------------------------------
+// Block reduction helpers (wave64 shuffles + LDS), deterministic order
+// ---------------------------------------------------------------------------
+__device__ double block_sum(double v) {
+    __shared__ double sh[BLK / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+    return s;  // valid in thread 0
+}
+
+// Sum of partials[0..n) in a fixed order by one block (all threads participate).
+__device__ double block_sum_partials(const double* partials, int n) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+    return block_sum(s);
+}
+
+// Last-block-done pattern: returns true in every thread of the last block to finish.
+__device__ bool last_block(unsigned int* ticket) {
+    __shared__ bool am_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = atomicAdd(ticket, 1u);
+        am_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (am_last) __threadfence();
+    return am_last;
+}
+
+__device__ __forceinline__ int ld_done(const CGScal* s) { return __atomic_load_n(&s->done, __ATOMIC_RELAXED); }
+
+// ---------------------------------------------------------------------------
+// Assembly
+// ---------------------------------------------------------------------------
+struct AsmOut {
+    double* A;          // NSLOT planes of N
+    double* F;          // 5 planes of N
+    double* b;          // padded 5-plane vector
+    double* is_slide;   // 2 planes of N
+    double* comp;       // N
+    double* fepart;     // 2 planes of N
+    double* invdiag;    // padded 5-plane vector
+    const double* ext;  // global-layout n vector or null
+    int* flag;          // [0] structural error, [1] non-finite
+    int Wp, P;          // padded width, plane stride
+};
+
+__device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
+
+__global__ void __launch_bounds__(BLK) k_assemble1(ClothDev C, AsmOut O, double h) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= C.N) return;
+    for (int s = 0; s < NSLOT; ++s) O.A[(size_t)s * C.N + k] = 0.0;
+    RowSink S;
+    S.init(&C, k, O.A, h);
+    int u = S.mu_, v = S.mv_;
+    // handle (src/PhysicsCloth.cpp:232-248)
+    if (C.flags[k] & 2) {
+        d3 xf = D3(C.xfix[k], C.xfix[C.N + k], C.xfix[2 * C.N + k]);
+        S.FL(k, (-C.handle_k) * (pos(C, k) - xf));
+        S.LL(KIND_KX, k, k, I33(-C.handle_k));
+    }
+    // edges (list order: warp rows, then weft columns; src/Cloth.cpp:190-228)
+    int ne_warp = C.Lg * (C.W - 1);
+    if (u > 0) edge_element(C, S, u - 1, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u - 1));
+    if (u < C.W - 1) edge_element(C, S, u, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u));
+    if (v > 0) edge_element(C, S, u, v - 1, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v - 1));
+    if (v < C.Lg - 1) edge_element(C, S, u, v, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v));
+    // bends centred on u-1, u, u+1 (warp) and v-1, v, v+1 (weft) (src/Cloth.cpp:230-270)
+    int nb_warp = C.Lg * (C.W - 2);
+    for (int d = -1; d <= 1; ++d) {
+        int uc = u + d;
+        if (uc >= 1 && uc <= C.W - 2) bend_element(C, S, uc, v, 0, v & 1, omp_owner(C.n_bends, v * (C.W - 2) + uc - 1));
+    }
+    for (int d = -1; d <= 1; ++d) {
+        int vc = v + d;
+        if (vc >= 1 && vc <= C.Lg - 2) bend_element(C, S, u, vc, 1, u & 1, omp_owner(C.n_bends, nb_warp + u * (C.Lg - 2) + vc - 1));
+    }
+    // faces of the up to four cells around the node (src/Cloth.cpp:272-286)
+    if (C.wind_density != 0.0 || C.wind_drag != 0.0) {
+        for (int cu = u - 1; cu <= u; ++cu)
+            for (int cv = v - 1; cv <= v; ++cv) {
+                if (cu < 0 || cv < 0 || cu > C.W - 2 || cv > C.Lg - 2) continue;
+                int tl = cu + cv * C.W, tr = tl + 1, bl = tl + C.W, br = bl + 1;
+                int f[2][3];
+                if (((cu + cv) & 1) == 0) { f[0][0] = tl; f[0][1] = tr; f[0][2] = br; f[1][0] = tl; f[1][1] = br; f[1][2] = bl; }
+                else { f[0][0] = tl; f[0][1] = tr; f[0][2] = bl; f[1][0] = tr; f[1][1] = br; f[1][2] = bl; }
+                for (int q = 0; q < 2; ++q)
+                    if (f[q][0] == k || f[q][1] == k || f[q][2] == k) face_element(C, S, f[q][0], f[q][1], f[q][2]);
+            }
+    }
+    // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
+    double comp = 0.0, sl[2] = {0.0, 0.0};
+    if (S.inter) comp = friction(C, S, sl);
+    // outputs
+    for (int c = 0; c < 5; ++c) O.F[c * C.N + k] = S.F[c];
+    O.comp[k] = comp;
+    O.is_slide[k] = sl[0]; O.is_slide[C.N + k] = sl[1];
+    O.fepart[k] = S.FEpart[0]; O.fepart[C.N + k] = S.FEpart[1];
+    // b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:20, 63)
+    int pk = padidx(O.Wp, u, v);
+    int e = S.inter ? crossing(C, u, v) : 0;
+    for (int c = 0; c < 5; ++c) {
+        double bv = 0.0;
+        if (c < 3 || S.inter) {
+            double ext = 0.0;
+            if (O.ext) ext = c < 3 ? O.ext[3 * k + c] : O.ext[3 * C.N + 2 * e + (c - 3)];
+            bv = h * (S.F[c] + ext - S.KvTv[c]) + S.Mv[c];
+        }
+        O.b[c * O.P + pk] = bv;
+    }
+    if (S.bad) atomicOr(&O.flag[0], 1);
+}
+
+// Shear segments touching the node (4 own, 2 each as warp neighbour of the left and
+// right crossings, 2 each as weft neighbour of the top and bottom crossings), then the
+// Jacobi diagonal and the NaN/Inf screen of the node's rows.
+template <class Sink>
+__device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp);
+
+__global__ void __launch_bounds__(BLK) k_assemble2(ClothDev C, AsmOut O, double h) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= C.N) return;
+    RowSink S;
+    S.init(&C, k, O.A, h);
+    int u = S.mu_, v = S.mv_;
+    auto K = [&](int uu, int vv) { return uu + vv * C.W; };
+    // shear_segs order per crossing: (left,bot), (left,top), (right,bot), (right,top)
+    for (int cu = u - 1; cu <= u + 1; ++cu)
+        for (int cv = v - 1; cv <= v + 1; ++cv) {
+            if (!interior(C, cu, cv)) continue;
+            if (cu != u && cv != v) continue;
+            int kc = K(cu, cv);
+            int nb1[2] = {K(cu - 1, cv), K(cu + 1, cv)};
+            int nb3[2] = {K(cu, cv + 1), K(cu, cv - 1)};
+            for (int a = 0; a < 2; ++a)
+                for (int bq = 0; bq < 2; ++bq) {
+                    int k1 = nb1[a], k3 = nb3[bq];
+                    if (kc != k && k1 != k && k3 != k) continue;
+                    shear_element(C, S, kc, k1, k3, O.comp);
+                }
+        }
+    int pk = padidx(O.Wp, u, v);
+    for (int c = 0; c < 3; ++c) {
+        O.F[c * C.N + k] += S.F[c];
+        O.b[c * O.P + pk] += h * S.F[c];
+    }
+    // Jacobi diagonal (the reference solver's DiagonalPreconditioner: 1/a_ii, or 1 if zero)
+    bool bad = false;
+    for (int c = 0; c < 5; ++c) {
+        double d = 0.0;
+        if (c < 3) d = O.A[(size_t)(c * 45 + c) * C.N + k];
+        else if (S.inter) d = O.A[(size_t)((c == 3 ? SLOT_EU : SLOT_EV) + 10) * C.N + k];
+        double inv = 0.0;
+        if (c < 3 || S.inter) inv = d != 0.0 ? 1.0 / d : 1.0;
+        O.invdiag[c * O.P + pk] = inv;
+        if (!isfinite(O.b[c * O.P + pk])) bad = true;
+    }
+    for (int s = 0; s < NSLOT; ++s)
+        if (!isfinite(O.A[(size_t)s * C.N + k])) bad = true;
+    if (bad) atomicOr(&O.flag[1], 1);
+    if (S.bad) atomicOr(&O.flag[0], 1);
+}
+
+}  // namespace
+
+#include "shear.hpp"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Stencil SpMV: 4 threads per node (rows x, y, z and the two E rows), node-fastest
+// ---------------------------------------------------------------------------
+struct SpmvArgs {
+    const double* A;
+    const double* x;   // padded 5-plane
+    double* y;         // padded 5-plane
+    int W, Lg, N, Wp, P;
+};
+
+__device__ __forceinline__ double spmv_row(const SpmvArgs& a, int k, int g) {
+    int u = k % a.W, v = k / a.W;
+    int p0 = padidx(a.Wp, u, v);
+    const double* A = a.A + k;
+    const double* x = a.x;
+    size_t N = a.N;
+    const int Wp = a.Wp, P = a.P;
+    double acc = 0.0;
+    if (g < 3) {
+        const int off[13] = {0, -1, 1, -Wp, Wp, -2, 2, -2 * Wp, 2 * Wp, -1 - Wp, 1 - Wp, -1 + Wp, 1 + Wp};
+        const double* Ar = A + (size_t)(g * 45) * N;
+#pragma unroll
+        for (int o = 0; o < 13; ++o) {
+            int q = p0 + off[o];
+            acc += Ar[(size_t)(o * 3 + 0) * N] * x[q];
+            acc += Ar[(size_t)(o * 3 + 1) * N] * x[P + q];
+            acc += Ar[(size_t)(o * 3 + 2) * N] * x[2 * P + q];
+        }
+        acc += Ar[(size_t)(SLOT_LE + 0) * N] * x[3 * P + p0 - 1];
+        acc += Ar[(size_t)(SLOT_LE + 1) * N] * x[3 * P + p0];
+        acc += Ar[(size_t)(SLOT_LE + 2) * N] * x[3 * P + p0 + 1];
+        acc += Ar[(size_t)(SLOT_LE + 3) * N] * x[4 * P + p0 - Wp];
+        acc += Ar[(size_t)(SLOT_LE + 4) * N] * x[4 * P + p0];
+        acc += Ar[(size_t)(SLOT_LE + 5) * N] * x[4 * P + p0 + Wp];
+    } else {
+        // E_u row then E_v row of the same node
+        double au = 0.0, av = 0.0;
+        const double* Au = A + (size_t)SLOT_EU * N;
+        const double* Av = A + (size_t)SLOT_EV * N;
+#pragma unroll
+        for (int d = -1; d <= 1; ++d) {
+            int qu = p0 + d, qv = p0 + d * Wp;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                au += Au[(size_t)((d + 1) * 3 + cc) * N] * x[cc * P + qu];
+                av += Av[(size_t)((d + 1) * 3 + cc) * N] * x[cc * P + qv];
+            }
+            au += Au[(size_t)(9 + d + 1) * N] * x[3 * P + qu];
+            av += Av[(size_t)(9 + d + 1) * N] * x[4 * P + qv];
+        }
+        return g == 3 ? au : av;  // g == 3 computes both, see caller
+    }
+    return acc;
+}
+
+// y = A x over the 4N row groups; optionally partial sums of x . y per block
+__global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, CGScal* sc, int check_done) {
+    if (check_done && ld_done(sc)) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double dotp = 0.0;
+    if (t < 4 * a.N) {
+        int g = t / a.N, k = t - g * a.N;
+        int u = k % a.W, v = k / a.W;
+        int p0 = padidx(a.Wp, u, v);
+        if (g < 3) {
+            double r = spmv_row(a, k, g);
+            a.y[g * a.P + p0] = r;
+            dotp = r * a.x[g * a.P + p0];
+        } else {
+            // both E rows in one thread (12 + 12 slots)
+            double au = 0.0, av = 0.0;
+            const double* Au = a.A + k + (size_t)SLOT_EU * a.N;
+            const double* Av = a.A + k + (size_t)SLOT_EV * a.N;
+            size_t N = a.N;
+#pragma unroll
+            for (int d = -1; d <= 1; ++d) {
+                int qu = p0 + d, qv = p0 + d * a.Wp;
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    au += Au[(size_t)((d + 1) * 3 + cc) * N] * a.x[cc * a.P + qu];
+                    av += Av[(size_t)((d + 1) * 3 + cc) * N] * a.x[cc * a.P + qv];
+                }
+                au += Au[(size_t)(9 + d + 1) * N] * a.x[3 * a.P + qu];
+                av += Av[(size_t)(9 + d + 1) * N] * a.x[4 * a.P + qv];
+            }
+            a.y[3 * a.P + p0] = au;
+            a.y[4 * a.P + p0] = av;
+            dotp = au * a.x[3 * a.P + p0] + av * a.x[4 * a.P + p0];
+        }
+    }
+    if (partials) {
+        double s = block_sum(dotp);
+        if (threadIdx.x == 0) partials[blockIdx.x] = s;
+        if (last_block(&sc->ticket[0])) {
+            double tot = block_sum_partials(partials, gridDim.x);
+            if (threadIdx.x == 0) { sc->pAp = tot; sc->ticket[0] = 0; }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CG vector kernels (Jacobi-PCG in the reference solver's order, src/Solvers.cpp:22-24)
+// ---------------------------------------------------------------------------
+struct VecArgs {
+    double* x; double* r; double* p; double* y; const double* b; const double* invd;
+    int W, Lg, N, Wp, P;
+};
+
+// thread -> (plane, node) over the 5N real entries; returns the padded index
+__device__ __forceinline__ int vec_index(const VecArgs& a, int t, int& c) {
+    c = t / a.N;
+    int k = t - c * a.N;
+    int u = k % a.W, v = k / a.W;
+    return c * a.P + padidx(a.Wp, u, v);
+}
+
+// init: r = b - y (y = A x0 when has_x0, else x = 0, r = b); p = D^-1 r
+__global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, double* partials, CGScal* sc, int has_x0, double tol, int maxit) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0, rz = 0.0, bb = 0.0;
+    if (t < 5 * a.N) {
+        int c;
+        int i = vec_index(a, t, c);
+        double bi = a.b[i];
+        double ri = has_x0 ? bi - a.y[i] : bi;
+        if (!has_x0) a.x[i] = 0.0;
+        double zi = a.invd[i] * ri;
+        a.r[i] = ri;
+        a.p[i] = zi;
+        rr = ri * ri; rz = ri * zi; bb = bi * bi;
+    }
+    double s1 = block_sum(rr);
+    double s2 = block_sum(rz);
+    double s3 = block_sum(bb);
+    if (threadIdx.x == 0) { partials[3 * blockIdx.x] = s1; partials[3 * blockIdx.x + 1] = s2; partials[3 * blockIdx.x + 2] = s3; }
+    if (last_block(&sc->ticket[1])) {
+        double a1 = 0, a2 = 0, a3 = 0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) { a1 += partials[3 * i]; a2 += partials[3 * i + 1]; a3 += partials[3 * i + 2]; }
+        a1 = block_sum(a1); a2 = block_sum(a2); a3 = block_sum(a3);
+        if (threadIdx.x == 0) {
+            sc->rr = a1; sc->rz = a2; sc->bb = a3;
+            double thr = tol * tol * a3;
+            sc->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+            sc->iters = 0; sc->maxit = maxit; sc->zero_rhs = a3 == 0.0;
+            sc->done = (a3 == 0.0 || a1 < sc->thr || maxit <= 0) ? 1 : 0;
+            sc->ticket[1] = 0;
+        }
+    }
+}
+
+// alpha = rz / pAp; x += alpha p; r -= alpha y; then |r|^2, r.z and the loop decision
+__global__ void __launch_bounds__(BLK) k_cg_xr(VecArgs a, double* partials, CGScal* sc) {
+    if (ld_done(sc)) return;
+    double alpha = sc->rz / sc->pAp;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0, rz = 0.0;
+    if (t < 5 * a.N) {
+        int c;
+        int i = vec_index(a, t, c);
+        a.x[i] += alpha * a.p[i];
+        double ri = a.r[i] - alpha * a.y[i];
+        a.r[i] = ri;
+        rr = ri * ri;
+        rz = ri * (a.invd[i] * ri);
+    }
+    double s1 = block_sum(rr);
+    double s2 = block_sum(rz);
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = s1; partials[2 * blockIdx.x + 1] = s2; }
+    if (last_block(&sc->ticket[2])) {
+        double a1 = 0, a2 = 0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) { a1 += partials[2 * i]; a2 += partials[2 * i + 1]; }
+        a1 = block_sum(a1); a2 = block_sum(a2);
+        if (threadIdx.x == 0) {
+            sc->rr = a1;
+            if (a1 < sc->thr) {
+                sc->done = 1;
+            } else {
+                sc->beta = a2 / sc->rz;
+                sc->rz = a2;
+                sc->iters += 1;
+                if (sc->iters >= sc->maxit) sc->done = 1;
+            }
+            sc->ticket[2] = 0;
+        }
+    }
+}
+
+// p = D^-1 r + beta p
+__global__ void __launch_bounds__(BLK) k_cg_p(VecArgs a, CGScal* sc) {
+    if (ld_done(sc)) return;
+    double beta = sc->beta;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 5 * a.N) {
+        int c;
+        int i = vec_index(a, t, c);
+        a.p[i] = a.invd[i] * a.r[i] + beta * a.p[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Update (src/Cloth.cpp:111-148) and geometry refresh (src/Cloth.cpp:307-348)
+// ---------------------------------------------------------------------------
+struct StateDev {
+    double* x; double* v; double* xfix; double* uv; double* uvd; double* uvbar; double* nrm;
+    const unsigned char* flags;
+    int W, Lg, N;
+};
+
+__global__ void __launch_bounds__(BLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.N) return;
+    int u = k % s.W, v = k / s.W;
+    int pk = padidx(Wp, u, v);
+    bool fixed = (s.flags[k] & 2) != 0;
+    for (int c = 0; c < 3; ++c) {
+        double nv = sol[c * P + pk];
+        s.v[c * s.N + k] = nv;
+        double nx = s.x[c * s.N + k] + nv * h;
+        s.x[c * s.N + k] = nx;
+        if (!fixed) s.xfix[c * s.N + k] = nx;
+    }
+    if (u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1) {
+        for (int c = 0; c < 2; ++c) {
+            double nv = sol[(3 + c) * P + pk];
+            s.uvd[c * s.N + k] = nv;
+            s.uv[c * s.N + k] = s.uv[c * s.N + k] + nv * h;
+            s.uvbar[c * s.N + k] = s.uvbar[c * s.N + k] + is_slide[c * s.N + k] * nv * h;
+        }
+    }
+}
+
+// Cloth::NodeNorm (src/Cloth.cpp:440-463): smallest-eigenvalue eigenvector of the 3x3
+// scatter matrix of the 5-point cross (cyclic Jacobi), aligned with
+// (x_right - x_left) x (x_bot - x_top), flipped by the weave pattern.
+__global__ void __launch_bounds__(BLK) k_normals(StateDev s) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.N) return;
+    int u = k % s.W, v = k / s.W;
+    if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
+        for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = 0.0;
+        return;
+    }
+    int ids[5] = {k, k - 1, k + 1, k - s.W, k + s.W};
+    double p[5][3];
+    for (int i = 0; i < 5; ++i) for (int c = 0; c < 3; ++c) p[i][c] = s.x[c * s.N + ids[i]];
+    double m[3] = {0, 0, 0};
+    for (int i = 0; i < 5; ++i) for (int c = 0; c < 3; ++c) m[c] += p[i][c];
+    for (int c = 0; c < 3; ++c) m[c] = m[c] * (1.0 / 5.0);
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int i = 0; i < 5; ++i) {
+        double d[3] = {p[i][0] - m[0], p[i][1] - m[1], p[i][2] - m[2]};
+        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) A[a][b] += d[a] * d[b];
+    }
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        double tr = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off == 0.0 || off <= 1e-34 * tr) break;
+        for (int pp = 0; pp < 2; ++pp)
+            for (int q = pp + 1; q < 3; ++q) {
+                if (A[pp][q] == 0.0) continue;
+                double th = (A[q][q] - A[pp][pp]) / (2.0 * A[pp][q]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int r = 0; r < 3; ++r) { double a0 = A[r][pp], a1 = A[r][q]; A[r][pp] = cs * a0 - sn * a1; A[r][q] = sn * a0 + cs * a1; }
+                for (int r = 0; r < 3; ++r) { double a0 = A[pp][r], a1 = A[q][r]; A[pp][r] = cs * a0 - sn * a1; A[q][r] = sn * a0 + cs * a1; }
+                for (int r = 0; r < 3; ++r) { double a0 = V[r][pp], a1 = V[r][q]; V[r][pp] = cs * a0 - sn * a1; V[r][q] = sn * a0 + cs * a1; }
+            }
+    }
+    int im = 0;
+    for (int i = 1; i < 3; ++i) if (A[i][i] < A[im][im]) im = i;
+    double n[3] = {V[0][im], V[1][im], V[2][im]};
+    double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int c = 0; c < 3; ++c) n[c] = n[c] / nn;
+    double a[3], b[3];
+    for (int c = 0; c < 3; ++c) { a[c] = p[2][c] - p[1][c]; b[c] = p[4][c] - p[3][c]; }
+    double cr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    double cn = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    double nn2 = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    double cosv = (cr[0] * n[0] + cr[1] * n[1] + cr[2] * n[2]) / (cn * nn2);
+    double sg = cosv < 0 ? -1.0 : 1.0;
+    if (s.flags[k] & 1) sg = -sg;
+    for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = sg * n[c];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct yc_ctx {
+    yc_desc d;
+    int device;
+    int W, Lg, N, Nc, n, Wp, Hp, P;
+    hipStream_t stream;
+    // device buffers
+    double *x, *v, *xfix, *uv, *uvd, *uvbar, *nrm;
+    unsigned char* flags;
+    double *A, *F, *is_slide, *comp, *fepart;
+    double *b, *invd, *cx, *cr, *cp, *cy;  // padded 5-plane vectors
+    double* ext;
+    double* partials;
+    CGScal* sc;
+    int* flag;
+    CGScal* h_sc;  // pinned
+    int* h_flag;   // pinned
+    ClothDev cd;
+    bool has_ext;
+    bool assembled, solved;
+    hipGraphExec_t graph;
+    int graph_chunk;
+    int ks_cg_blocks;
+};
+
+namespace {
+
+ClothDev make_dev(const yc_ctx* c) {
+    ClothDev C;
+    const yc_desc& d = c->d;
+    C.W = c->W; C.Lg = c->Lg; C.N = c->N; C.Nc = c->Nc;
+    C.n_edges = c->Lg * (c->W - 1) + c->W * (c->Lg - 1);
+    C.n_bends = c->Lg * (c->W - 2) + c->W * (c->Lg - 2);
+    C.L = d.L; C.R = d.R;
+    for (int y = 0; y < 2; ++y) {
+        int yy = d.n_yarns == 1 ? 0 : y;
+        C.rho[y] = d.rho[yy];
+        C.ks[y] = d.Y[yy] * PI_REF * std::pow(d.R, 2);   // src/Yarn.h:14-15
+        C.kb[y] = d.B[yy] * PI_REF * std::pow(d.R, 2);
+    }
+    C.kc = d.kc; C.kf = d.kf; C.df = d.df; C.mu = d.mu; C.S = d.S; C.handle_k = d.handle_k;
+    for (int i = 0; i < 3; ++i) { C.G[i] = d.G[i]; C.wind_vel[i] = d.wind_vel[i]; }
+    C.wind_density = d.wind_density; C.wind_drag = d.wind_drag;
+    C.jam_thrd = 2.0 * std::asin(d.R / d.L);   // src/Cloth.cpp:29
+    C.x = c->x; C.v = c->v; C.xfix = c->xfix; C.uv = c->uv; C.uvd = c->uvd; C.uvbar = c->uvbar;
+    C.nrm = c->nrm; C.flags = c->flags;
+    return C;
+}
+
+AsmOut make_out(yc_ctx* c) {
+    AsmOut O;
+    O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
+    O.invdiag = c->invd; O.ext = c->has_ext ? c->ext : nullptr; O.flag = c->flag; O.Wp = c->Wp; O.P = c->P;
+    return O;
+}
+
+StateDev make_state(yc_ctx* c) {
+    StateDev s;
+    s.x = c->x; s.v = c->v; s.xfix = c->xfix; s.uv = c->uv; s.uvd = c->uvd; s.uvbar = c->uvbar; s.nrm = c->nrm;
+    s.flags = c->flags; s.W = c->W; s.Lg = c->Lg; s.N = c->N;
+    return s;
+}
+
+VecArgs make_vec(yc_ctx* c) {
+    VecArgs a;
+    a.x = c->cx; a.r = c->cr; a.p = c->cp; a.y = c->cy; a.b = c->b; a.invd = c->invd;
+    a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P;
+    return a;
+}
+
+SpmvArgs make_spmv(yc_ctx* c, const double* in, double* out) {
+    SpmvArgs a;
+    a.A = c->A; a.x = in; a.y = out; a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P;
+    return a;
+}
+
+int grid_for(long long threads) { return (int)((threads + BLK - 1) / BLK); }
+
+// flip_norm by weave pattern (src/Cloth.cpp:66-92)
+bool pattern_flip(int pattern, int i, int j) {
+    if (pattern == 0) return (i + j) % 2 != 0;
+    if (pattern == 1) { int rv = j % 4; int ru = (i + rv) % 4; return ru < 2; }
+    int rv = (j - 1) % 4; int q = rv / 2; int r = rv % 2; int iu = q * 3 + r * 2; int ru = (i - 1 + iu) % 4;
+    return ru == 0;
+}
+
+int refresh_geometry(yc_ctx* c) {
+    hipLaunchKernelGGL(k_normals, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, make_state(c));
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+int cg_launch_iteration(yc_ctx* c) {
+    int nb_spmv = grid_for(4LL * c->N);
+    int nb_vec = grid_for(5LL * c->N);
+    hipLaunchKernelGGL(k_spmv, dim3(nb_spmv), dim3(BLK), 0, c->stream, make_spmv(c, c->cp, c->cy), c->partials, c->sc, 1);
+    hipLaunchKernelGGL(k_cg_xr, dim3(nb_vec), dim3(BLK), 0, c->stream, make_vec(c), c->partials, c->sc);
+    hipLaunchKernelGGL(k_cg_p, dim3(nb_vec), dim3(BLK), 0, c->stream, make_vec(c), c->sc);
+    return YC_OK;
+}
+
+int build_graph(yc_ctx* c) {
+    if (c->graph) return YC_OK;
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < c->graph_chunk; ++i) cg_launch_iteration(c);
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return YC_OK;
+}
+
+void to_padded(const yc_ctx* c, const double* glob, std::vector<double>& pad) {
+    pad.assign((size_t)5 * c->P, 0.0);
+    for (int v = 0; v < c->Lg; ++v)
+        for (int u = 0; u < c->W; ++u) {
+            int k = u + v * c->W;
+            int pk = (u + 2) + (v + 2) * c->Wp;
+            for (int a = 0; a < 3; ++a) pad[(size_t)a * c->P + pk] = glob[3 * k + a];
+            if (u > 0 && u < c->W - 1 && v > 0 && v < c->Lg - 1) {
+                int e = (u - 1) + (v - 1) * (c->W - 2);
+                for (int a = 0; a < 2; ++a) pad[(size_t)(3 + a) * c->P + pk] = glob[3 * c->N + 2 * e + a];
+            }
+        }
+}
+
+void from_padded(const yc_ctx* c, const std::vector<double>& pad, double* glob) {
+    for (int v = 0; v < c->Lg; ++v)
+        for (int u = 0; u < c->W; ++u) {
+            int k = u + v * c->W;
+            int pk = (u + 2) + (v + 2) * c->Wp;
+            for (int a = 0; a < 3; ++a) glob[3 * k + a] = pad[(size_t)a * c->P + pk];
+            if (u > 0 && u < c->W - 1 && v > 0 && v < c->Lg - 1) {
+                int e = (u - 1) + (v - 1) * (c->W - 2);
+                for (int a = 0; a < 2; ++a) glob[3 * c->N + 2 * e + a] = pad[(size_t)(3 + a) * c->P + pk];
+            }
+        }
+}
+
+int do_assemble(yc_ctx* c, double h) {
+    HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+    c->cd = make_dev(c);
+    hipLaunchKernelGGL(k_assemble1, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, c->cd, make_out(c), h);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_assemble2, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, c->cd, make_out(c), h);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->assembled = true;
+    if (c->h_flag[0]) return fail(YC_ERR_HIP, "assembly: stencil slot out of range");
+    if (c->h_flag[1]) return fail(YC_ERR_NONFINITE, "NaN/Inf in A or b");
+    return YC_OK;
+}
+
+int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* iters, double* relres) {
+    if (!c->assembled) return fail(YC_ERR_ARG, "yc_solve before yc_assemble");
+    int has_x0 = 0;
+    if (x0_global) {
+        std::vector<double> pad;
+        to_padded(c, x0_global, pad);
+        HIPCHK(hipMemcpyAsync(c->cx, pad.data(), pad.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_spmv, dim3(grid_for(4LL * c->N)), dim3(BLK), 0, c->stream, make_spmv(c, c->cx, c->cy), (double*)nullptr, c->sc, 0);
+        has_x0 = 1;
+    }
+    hipLaunchKernelGGL(k_cg_init, dim3(grid_for(5LL * c->N)), dim3(BLK), 0, c->stream, make_vec(c), c->partials, c->sc, has_x0, tol, maxit);
+    HIPCHK(hipGetLastError());
+    int rc = build_graph(c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long long launched = 0;
+    while (!c->h_sc->done && launched < (long long)maxit + c->graph_chunk) {
+        HIPCHK(hipGraphLaunch(c->graph, c->stream));
+        launched += c->graph_chunk;
+        HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (c->h_sc->zero_rhs) HIPCHK(hipMemsetAsync(c->cx, 0, (size_t)5 * c->P * sizeof(double), c->stream));
+    c->solved = true;
+    double rel = c->h_sc->bb > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bb) : 0.0;
+    if (iters) *iters = c->h_sc->iters;
+    if (relres) *relres = rel;
+    if (!std::isfinite(rel)) return fail(YC_ERR_NONFINITE, "NaN/Inf in the solution");
+    if (c->h_sc->rr >= c->h_sc->thr && !c->h_sc->zero_rhs) return fail(YC_ERR_NOT_CONVERGED, "CG stopped at maxit above tolerance");
+    return YC_OK;
+}
+
+int do_update(yc_ctx* c, double h) {
+    if (!c->solved) return fail(YC_ERR_ARG, "yc_update before yc_solve");
+    hipLaunchKernelGGL(k_update, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, make_state(c), (const double*)c->cx,
+                       (const double*)c->is_slide, c->Wp, c->P, h);
+    HIPCHK(hipGetLastError());
+    int rc = refresh_geometry(c);
+    c->assembled = false;
+    c->solved = false;
+    return rc;
+}
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    HIPCHK(hipMalloc((void**)p, count * sizeof(T) + 64));
+    HIPCHK(hipMemset(*p, 0, count * sizeof(T) + 64));
+    return YC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* yc_last_error(void) { return g_err.c_str(); }
+
+int yc_create(const yc_desc* desc, yc_ctx** out) {
+    g_err.clear();
+    if (!desc || !out) return fail(YC_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (desc->width < 3 || desc->length < 3) return fail(YC_ERR_ARG, "cloth must be at least 3x3 crossings");
+    if (desc->n_yarns < 1 || desc->n_yarns > 2) return fail(YC_ERR_ARG, "n_yarns must be 1 or 2");
+    if (desc->n_handles < 0 || desc->n_handles > YC_MAX_HANDLES) return fail(YC_ERR_ARG, "too many handles");
+    if ((long long)desc->width * desc->length > (1LL << 26)) return fail(YC_ERR_ARG, "cloth too large");
+    yc_ctx* c = new yc_ctx();
+    std::memset((void*)c, 0, sizeof(yc_ctx));
+    c->d = *desc;
+    c->W = desc->width; c->Lg = desc->length; c->N = c->W * c->Lg; c->Nc = (c->W - 2) * (c->Lg - 2);
+    c->n = 3 * c->N + 2 * c->Nc;
+    c->Wp = c->W + 4; c->Hp = c->Lg + 4;
+    c->P = ((c->Wp * c->Hp + 63) / 64) * 64;
+    c->graph_chunk = 16;
+    int rc = YC_OK;
+    auto chk = [&](int r) { if (r && !rc) rc = r; };
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return fail(YC_ERR_HIP, "no HIP device"); }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return fail(YC_ERR_HIP, "stream"); }
+    size_t N = c->N;
+    chk(dalloc(&c->x, 3 * N)); chk(dalloc(&c->v, 3 * N)); chk(dalloc(&c->xfix, 3 * N));
+    chk(dalloc(&c->uv, 2 * N)); chk(dalloc(&c->uvd, 2 * N)); chk(dalloc(&c->uvbar, 2 * N)); chk(dalloc(&c->nrm, 3 * N));
+    chk(dalloc(&c->flags, N));
+    chk(dalloc(&c->A, (size_t)NSLOT * N)); chk(dalloc(&c->F, 5 * N));
+    chk(dalloc(&c->is_slide, 2 * N)); chk(dalloc(&c->comp, N)); chk(dalloc(&c->fepart, 2 * N));
+    size_t PV = (size_t)5 * c->P;
+    chk(dalloc(&c->b, PV)); chk(dalloc(&c->invd, PV)); chk(dalloc(&c->cx, PV)); chk(dalloc(&c->cr, PV));
+    chk(dalloc(&c->cp, PV)); chk(dalloc(&c->cy, PV));
+    chk(dalloc(&c->ext, (size_t)c->n));
+    chk(dalloc(&c->partials, (size_t)3 * grid_for(5LL * c->N) + 64));
+    chk(dalloc(&c->sc, 1));
+    chk(dalloc(&c->flag, 4));
+    if (!rc && hipHostMalloc((void**)&c->h_sc, sizeof(CGScal)) != hipSuccess) rc = fail(YC_ERR_HIP, "pinned alloc");
+    if (!rc && hipHostMalloc((void**)&c->h_flag, 4 * sizeof(int)) != hipSuccess) rc = fail(YC_ERR_HIP, "pinned alloc");
+    if (rc) { yc_destroy(c); return rc; }
+    // constructor rest state (src/Cloth.cpp:52-99) and flags
+    std::vector<double> x(3 * N), uv(2 * N);
+    std::vector<unsigned char> fl(N, 0);
+    for (int j = 0; j < c->Lg; ++j)
+        for (int i = 0; i < c->W; ++i) {
+            int k = i + j * c->W;
+            double px = i * desc->L, py = j * desc->L, pz = desc->hight;
+            if (desc->pose == 0) { px = i * desc->L; py = 0.0; pz = desc->hight - j * desc->L; }
+            x[k] = px; x[N + k] = py; x[2 * N + k] = pz;
+            uv[k] = i * desc->L; uv[N + k] = j * desc->L;
+            bool interior_ = i > 0 && i < c->W - 1 && j > 0 && j < c->Lg - 1;
+            if (interior_ && pattern_flip(desc->pattern, i, j)) fl[k] |= 1;
+        }
+    for (int hnd = 0; hnd < desc->n_handles; ++hnd) {
+        int hu = desc->handles[2 * hnd], hv = desc->handles[2 * hnd + 1];
+        if (hu < 0 || hu >= c->W || hv < 0 || hv >= c->Lg) { yc_destroy(c); return fail(YC_ERR_ARG, "handle outside the cloth"); }
+        fl[hu + hv * c->W] |= 2;
+    }
+    bool ok = hipMemcpy(c->x, x.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->xfix, x.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->uv, uv.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->uvbar, uv.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->flags, fl.data(), N, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) { yc_destroy(c); return fail(YC_ERR_HIP, "upload"); }
+    c->cd = make_dev(c);
+    rc = refresh_geometry(c);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(YC_ERR_HIP, "sync");
+    if (rc) { yc_destroy(c); return rc; }
+    *out = c;
+    return YC_OK;
+}
+
+void yc_destroy(yc_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
+                    c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->partials, c->sc, c->flag};
+    for (void* p : bufs) if (p) (void)hipFree(p);
+    if (c->h_sc) (void)hipHostFree(c->h_sc);
+    if (c->h_flag) (void)hipHostFree(c->h_flag);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int yc_sizes(const yc_ctx* c, int* N, int* Nc, int* n, int* nslots) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (N) *N = c->N;
+    if (Nc) *Nc = c->Nc;
+    if (n) *n = c->n;
+    if (nslots) *nslots = NSLOT;
+    return YC_OK;
+}
+
+int yc_set_state(yc_ctx* c, const double* lpos, const double* lposfix, const double* lvel,
+                 const double* epos, const double* eposbar, const double* evel) {
+    if (!c || !lpos || !lposfix || !lvel || !epos || !eposbar || !evel) return fail(YC_ERR_ARG, "null argument");
+    size_t N = c->N;
+    std::vector<double> a(3 * N), b2(3 * N), cv(3 * N), d(2 * N), e(2 * N), f(2 * N);
+    for (int v = 0; v < c->Lg; ++v)
+        for (int u = 0; u < c->W; ++u) {
+            int k = u + v * c->W, s = u + v * c->Lg;   // state index u + v*length (src/Cloth.cpp:426)
+            for (int q = 0; q < 3; ++q) { a[q * N + k] = lpos[3 * s + q]; b2[q * N + k] = lposfix[3 * s + q]; cv[q * N + k] = lvel[3 * s + q]; }
+            for (int q = 0; q < 2; ++q) { d[q * N + k] = epos[2 * s + q]; e[q * N + k] = eposbar[2 * s + q]; f[q * N + k] = evel[2 * s + q]; }
+        }
+    HIPCHK(hipMemcpyAsync(c->x, a.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->xfix, b2.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->v, cv.data(), 3 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->uv, d.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->uvbar, e.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->uvd, f.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = refresh_geometry(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->assembled = c->solved = false;
+    return YC_OK;
+}
+
+int yc_get_state(yc_ctx* c, double* lpos, double* lposfix, double* lvel, double* epos, double* eposbar, double* evel) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    size_t N = c->N;
+    std::vector<double> a(3 * N), b2(3 * N), cv(3 * N), d(2 * N), e(2 * N), f(2 * N);
+    HIPCHK(hipMemcpyAsync(a.data(), c->x, 3 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(b2.data(), c->xfix, 3 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(cv.data(), c->v, 3 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(d.data(), c->uv, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(e.data(), c->uvbar, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(f.data(), c->uvd, 2 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int v = 0; v < c->Lg; ++v)
+        for (int u = 0; u < c->W; ++u) {
+            int k = u + v * c->W, s = u + v * c->Lg;
+            for (int q = 0; q < 3; ++q) {
+                if (lpos) lpos[3 * s + q] = a[q * N + k];
+                if (lposfix) lposfix[3 * s + q] = b2[q * N + k];
+                if (lvel) lvel[3 * s + q] = cv[q * N + k];
+            }
+            for (int q = 0; q < 2; ++q) {
+                if (epos) epos[2 * s + q] = d[q * N + k];
+                if (eposbar) eposbar[2 * s + q] = e[q * N + k];
+                if (evel) evel[2 * s + q] = f[q * N + k];
+            }
+        }
+    return YC_OK;
+}
+
+int yc_set_ext_force(yc_ctx* c, const double* ext) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (!ext) { c->has_ext = false; return YC_OK; }
+    HIPCHK(hipMemcpyAsync(c->ext, ext, (size_t)c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_ext = true;
+    return YC_OK;
+}
+
+int yc_assemble(yc_ctx* c, double h) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    g_err.clear();
+    return do_assemble(c, h);
+}
+
+int yc_export_csr(yc_ctx* c, int* rowptr, int* col, double* val, int* nnz) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (!c->assembled) return fail(YC_ERR_ARG, "export before assemble");
+    size_t N = c->N;
+    std::vector<double> A((size_t)NSLOT * N);
+    HIPCHK(hipMemcpyAsync(A.data(), c->A, A.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // rows in global order; every structurally possible slot inside the grid
+    std::vector<std::vector<std::pair<int, double>>> rows(c->n);
+    const int offu[13] = {0, -1, 1, 0, 0, -2, 2, 0, 0, -1, 1, -1, 1};
+    const int offv[13] = {0, 0, 0, -1, 1, 0, 0, -2, 2, -1, -1, 1, 1};
+    auto inter = [&](int u, int v) { return u > 0 && u < c->W - 1 && v > 0 && v < c->Lg - 1; };
+    auto gL = [&](int u, int v, int a) { return 3 * (u + v * c->W) + a; };
+    auto gE = [&](int u, int v, int a) { return 3 * c->N + 2 * ((u - 1) + (v - 1) * (c->W - 2)) + a; };
+    auto inside = [&](int u, int v) { return u >= 0 && u < c->W && v >= 0 && v < c->Lg; };
+    for (int v = 0; v < c->Lg; ++v)
+        for (int u = 0; u < c->W; ++u) {
+            int k = u + v * c->W;
+            for (int rc = 0; rc < 3; ++rc) {
+                int row = gL(u, v, rc);
+                for (int o = 0; o < 13; ++o) {
+                    int uu = u + offu[o], vv = v + offv[o];
+                    if (!inside(uu, vv)) continue;
+                    for (int cc = 0; cc < 3; ++cc) rows[row].push_back({gL(uu, vv, cc), A[(size_t)(rc * 45 + o * 3 + cc) * N + k]});
+                }
+                for (int d = -1; d <= 1; ++d) {
+                    if (inside(u + d, v) && inter(u + d, v)) rows[row].push_back({gE(u + d, v, 0), A[(size_t)(rc * 45 + SLOT_LE + d + 1) * N + k]});
+                    if (inside(u, v + d) && inter(u, v + d)) rows[row].push_back({gE(u, v + d, 1), A[(size_t)(rc * 45 + SLOT_LE + 3 + d + 1) * N + k]});
+                }
+            }
+            if (!inter(u, v)) continue;
+            for (int a = 0; a < 2; ++a) {
+                int row = gE(u, v, a);
+                int base = a == 0 ? SLOT_EU : SLOT_EV;
+                for (int d = -1; d <= 1; ++d) {
+                    int uu = u + (a == 0 ? d : 0), vv = v + (a == 1 ? d : 0);
+                    for (int cc = 0; cc < 3; ++cc) rows[row].push_back({gL(uu, vv, cc), A[(size_t)(base + (d + 1) * 3 + cc) * N + k]});
+                    if (inter(uu, vv)) rows[row].push_back({gE(uu, vv, a), A[(size_t)(base + 9 + d + 1) * N + k]});
+                }
+            }
+        }
+    int total = 0;
+    for (auto& r : rows) { std::sort(r.begin(), r.end()); total += (int)r.size(); }
+    if (nnz) *nnz = total;
+    if (rowptr) {
+        rowptr[0] = 0;
+        for (int i = 0; i < c->n; ++i) rowptr[i + 1] = rowptr[i] + (int)rows[i].size();
+    }
+    if (col || val) {
+        int j = 0;
+        for (auto& r : rows)
+            for (auto& e : r) { if (col) col[j] = e.first; if (val) val[j] = e.second; ++j; }
+    }
+    return YC_OK;
+}
+
+int yc_export_vec(yc_ctx* c, int which, double* out) {
+    if (!c || !out) return fail(YC_ERR_ARG, "null argument");
+    size_t N = c->N;
+    if (which == 0) {
+        std::vector<double> F(5 * N);
+        HIPCHK(hipMemcpyAsync(F.data(), c->F, F.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int v = 0; v < c->Lg; ++v)
+            for (int u = 0; u < c->W; ++u) {
+                int k = u + v * c->W;
+                for (int a = 0; a < 3; ++a) out[3 * k + a] = F[a * N + k];
+                if (u > 0 && u < c->W - 1 && v > 0 && v < c->Lg - 1) {
+                    int e = (u - 1) + (v - 1) * (c->W - 2);
+                    for (int a = 0; a < 2; ++a) out[3 * N + 2 * e + a] = F[(3 + a) * N + k];
+                }
+            }
+        return YC_OK;
+    }
+    if (which == 2) {
+        std::vector<double> v3(3 * N), v2(2 * N);
+        HIPCHK(hipMemcpyAsync(v3.data(), c->v, v3.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(v2.data(), c->uvd, v2.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int v = 0; v < c->Lg; ++v)
+            for (int u = 0; u < c->W; ++u) {
+                int k = u + v * c->W;
+                for (int a = 0; a < 3; ++a) out[3 * k + a] = v3[a * N + k];
+                if (u > 0 && u < c->W - 1 && v > 0 && v < c->Lg - 1) {
+                    int e = (u - 1) + (v - 1) * (c->W - 2);
+                    for (int a = 0; a < 2; ++a) out[3 * N + 2 * e + a] = v2[a * N + k];
+                }
+            }
+        return YC_OK;
+    }
+    if (which == 1 || which == 3) {
+        std::vector<double> pad((size_t)5 * c->P);
+        HIPCHK(hipMemcpyAsync(pad.data(), which == 1 ? c->b : c->cx, pad.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        from_padded(c, pad, out);
+        return YC_OK;
+    }
+    return fail(YC_ERR_ARG, "unknown vector");
+}
+
+int yc_export_crossing(yc_ctx* c, double* is_slide, double* comp, double* fe_partial) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    size_t N = c->N;
+    std::vector<double> sl(2 * N), cp(N), fp(2 * N);
+    HIPCHK(hipMemcpyAsync(sl.data(), c->is_slide, sl.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(cp.data(), c->comp, cp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(fp.data(), c->fepart, fp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int v = 1; v < c->Lg - 1; ++v)
+        for (int u = 1; u < c->W - 1; ++u) {
+            int k = u + v * c->W, e = (u - 1) + (v - 1) * (c->W - 2);
+            for (int a = 0; a < 2; ++a) {
+                if (is_slide) is_slide[2 * e + a] = sl[a * N + k];
+                if (fe_partial) fe_partial[2 * e + a] = fp[a * N + k];
+            }
+            if (comp) comp[e] = cp[k];
+        }
+    return YC_OK;
+}
+
+int yc_export_normals(yc_ctx* c, double* out) {
+    if (!c || !out) return fail(YC_ERR_ARG, "null argument");
+    size_t N = c->N;
+    std::vector<double> nr(3 * N);
+    HIPCHK(hipMemcpyAsync(nr.data(), c->nrm, nr.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < N; ++k) for (int a = 0; a < 3; ++a) out[3 * k + a] = nr[a * N + k];
+    return YC_OK;
+}
+
+int yc_solve(yc_ctx* c, const double* x0, double tol, int maxit, int* iters, double* relres) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    g_err.clear();
+    return do_solve(c, x0, tol, maxit, iters, relres);
+}
+
+int yc_update(yc_ctx* c, double h) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    int rc = do_update(c, h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return YC_OK;
+}
+
+int yc_step(yc_ctx* c, double h, const double* x0, double tol, int maxit, yc_step_info* info) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    g_err.clear();
+    int it = 0;
+    double rr = 0;
+    int rc = do_assemble(c, h);
+    if (!rc) rc = do_solve(c, x0, tol, maxit, &it, &rr);
+    if (info) { info->cg_iters = it; info->cg_relres = rr; info->status = rc; }
+    if (rc == YC_ERR_NONFINITE || rc == YC_ERR_HIP || rc == YC_ERR_ARG) return rc;
+    int rc2 = do_update(c, h);
+    if (rc2) return rc2;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* total_iters) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    long long tot = 0;
+    int worst = YC_OK;
+    for (int s = 0; s < steps; ++s) {
+        yc_step_info info;
+        int rc = yc_step(c, h, nullptr, tol, maxit, &info);
+        tot += info.cg_iters;
+        if (rc == YC_ERR_NOT_CONVERGED) worst = rc;
+        else if (rc) { if (total_iters) *total_iters = tot; return rc; }
+    }
+    if (total_iters) *total_iters = tot;
+    return worst;
+}
+
+int yc_time_spmv(yc_ctx* c, int reps, double* avg_ms) {
+    if (!c || reps <= 0 || !avg_ms) return fail(YC_ERR_ARG, "bad argument");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    SpmvArgs a = make_spmv(c, c->cp, c->cy);
+    hipLaunchKernelGGL(k_spmv, dim3(grid_for(4LL * c->N)), dim3(BLK), 0, c->stream, a, (double*)nullptr, c->sc, 0);
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i)
+        hipLaunchKernelGGL(k_spmv, dim3(grid_for(4LL * c->N)), dim3(BLK), 0, c->stream, a, (double*)nullptr, c->sc, 0);
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return YC_OK;
+}
+
+double yc_spmv_bytes(const yc_ctx* c) {
+    if (!c) return 0.0;
+    // structural nonzeros of A (LL 13-point stencil + LE/EL/EE along the yarns)
+    double W = c->W;
+    double nnz = 9.0 * (13.0 * W * W - 20.0 * W - 4.0) + 14.0 * (W - 2.0) * (3.0 * W - 8.0);
+    return 8.0 * nnz + 16.0 * c->n;
+}
+
+}  // extern "C"

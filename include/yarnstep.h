@@ -1,0 +1,131 @@
+/*
+ * yarnstep.h -- C ABI of the MI355X (gfx950) implicit yarn-level time step.
+ *
+ * One opaque context holds one cloth: its structure-of-arrays node state in HBM,
+ * the 159-slot stencil system matrix, the right-hand side and the CG workspaces.
+ * All entry points return an int status:
+ *   YC_OK (0), YC_ERR_NONFINITE (1: NaN/Inf in A, b or the solution -- the cases
+ *   where the reference's SimStep returns false, src/TrainHelper.cpp:22-48),
+ *   YC_ERR_NOT_CONVERGED (2: CG stopped at maxit above tol), YC_ERR_HIP (3),
+ *   YC_ERR_ARG (4: bad argument).  No C++ exception crosses this boundary.
+ * Host arrays passed in are borrowed for the duration of the call.  A context is
+ * bound to the device that was current at yc_create and uses one stream; it is not
+ * thread-safe, separate contexts are independent.
+ *
+ * Index conventions are the reference's:
+ *   node k = u + v*width, crossing e = (u-1) + (v-1)*(width-2),
+ *   global vector [L: 3k+{x,y,z}] ++ [E: 3N + 2e+{u,v}]      (src/MatrixMp.cpp:120-122)
+ *   state arrays use index u + v*length as get/set_ClothState do (src/Cloth.cpp:362,426).
+ */
+#ifndef YARNSTEP_H
+#define YARNSTEP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YC_OK 0
+#define YC_ERR_NONFINITE 1
+#define YC_ERR_NOT_CONVERGED 2
+#define YC_ERR_HIP 3
+#define YC_ERR_ARG 4
+
+#define YC_MAX_HANDLES 16
+
+/* Cloth description: the arguments of the reference's Cloth constructor
+ * (src/Cloth.h:55-60, src/Cloth.cpp:3-50), its Yarn list (src/Yarn.h:6-17), the
+ * Environment's gravity and wind (src/Environment.h:6-22, src/Wind.h:6-35) and
+ * the handle list of Cloth::set_handles (src/Cloth.cpp:386-394). */
+typedef struct {
+    int width, length;
+    double hight, L, R;
+    int n_yarns;                 /* 1 or 2 (warp row / weft column i uses yarn i%2) */
+    double rho[2], Y[2], B[2];
+    double kc, kf, df, mu, S, handle_k;
+    int pattern;                 /* 0 plain, 1 twill, 2 satin */
+    int pose;                    /* 0 upright, 1 flat */
+    double G[3];
+    double wind_vel[3], wind_density, wind_drag;
+    int n_handles;
+    int handles[2 * YC_MAX_HANDLES];
+} yc_desc;
+
+typedef struct yc_ctx yc_ctx;
+
+typedef struct {
+    int cg_iters;        /* iterations of the last solve (reference semantics: completed iterations) */
+    double cg_relres;    /* |b - A x| / |b| estimate from the recursive residual */
+    int status;
+} yc_step_info;
+
+/* Replaces: Cloth::Cloth + PhysicsCloth::PhysicsCloth + InitMV
+ * (src/Cloth.cpp:3-50, src/PhysicsCloth.h:22-23, src/PhysicsCloth.cpp:3-44). */
+int yc_create(const yc_desc* desc, yc_ctx** out);
+/* Replaces: the destructors of Cloth / PhysicsCloth. */
+void yc_destroy(yc_ctx* ctx);
+/* Last error message of this thread (empty string when none). */
+const char* yc_last_error(void);
+
+/* Sizes: N nodes, Nc crossings, n = 3N + 2Nc DOFs, nnz slots of the stencil matrix. */
+int yc_sizes(const yc_ctx* ctx, int* N, int* Nc, int* n, int* nslots);
+
+/* Replaces: Cloth::set_ClothState (src/Cloth.cpp:421-438), followed by the
+ * geometry refresh of UpdateClothUnits (src/Cloth.cpp:307-348).  Host arrays:
+ * lpos/lposfix/lvel N*3 doubles, epos/eposbar/evel N*2 doubles. */
+int yc_set_state(yc_ctx* ctx, const double* lpos, const double* lposfix, const double* lvel,
+                 const double* epos, const double* eposbar, const double* evel);
+/* Replaces: Cloth::get_ClothState (src/Cloth.cpp:350-371).  Any pointer may be NULL. */
+int yc_get_state(yc_ctx* ctx, double* lpos, double* lposfix, double* lvel,
+                 double* epos, double* eposbar, double* evel);
+
+/* External generalized force added to b (the ext_force of
+ * SimStep(cloth, phys, h, ext_force), src/TrainHelper.cpp:56-63).  n doubles in the
+ * global layout, copied to the device; NULL clears it. */
+int yc_set_ext_force(yc_ctx* ctx, const double* ext_force);
+
+/* Replaces: PhysicsCloth::FillMV (src/PhysicsCloth.cpp:46-156) and the system
+ * formation A = M - h^2 K_x - h K_v, b = h (F + f_ext - K_v^T v) + M v
+ * (src/TrainHelper.cpp:19-20, 61-63).  Returns YC_ERR_NONFINITE on NaN/Inf in A or b. */
+int yc_assemble(yc_ctx* ctx, double h);
+
+/* Replaces: get_SpaM/get_SpaFPos/get_SpaFVel-derived A (src/PhysicsCloth.h:72-77,
+ * src/MatrixMp.cpp:51-110) for inspection: the structural pattern of A as CSR of the
+ * global n x n matrix.  rowptr n+1 ints, col/val nnz entries (nnz from yc_export_csr
+ * with col == val == NULL). */
+int yc_export_csr(yc_ctx* ctx, int* rowptr, int* col, double* val, int* nnz);
+/* which: 0 F (get_SpaF), 1 b, 2 v (get_SpaVel), 3 last solution x.  out: n doubles. */
+int yc_export_vec(yc_ctx* ctx, int which, double* out);
+/* Per crossing: is_slide (Nc*2, the F_E_is_Slides[0] of src/PhysicsCloth.cpp:2633),
+ * compression (Nc), friction input F_E partial (Nc*2).  Any pointer may be NULL. */
+int yc_export_crossing(yc_ctx* ctx, double* is_slide, double* comp, double* fe_partial);
+/* Crossing normals N*3 (zero on the boundary), as Cloth::NodeNorm (src/Cloth.cpp:440-463). */
+int yc_export_normals(yc_ctx* ctx, double* nrm);
+
+/* Replaces: EigenSolver::apply forward (src/Solvers.cpp:3-31): Jacobi-preconditioned
+ * CG on the full A from x0 (n doubles, NULL = zero guess) until |r|^2 < tol^2 |b|^2 or
+ * maxit iterations.  Returns YC_ERR_NOT_CONVERGED when stopped at maxit. */
+int yc_solve(yc_ctx* ctx, const double* x0, double tol, int maxit, int* iters, double* relres);
+
+/* Replaces: Cloth::update(new_vel, F_E_is_Slides[0], h) (src/Cloth.cpp:111-148) with
+ * the last solution as new_vel, including the geometry refresh. */
+int yc_update(yc_ctx* ctx, double h);
+
+/* Replaces: SimStep (src/TrainHelper.cpp:15-97): assemble, solve, update.  x0 as in
+ * yc_solve (NULL = zero guess).  info may be NULL. */
+int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_step_info* info);
+
+/* Runs `steps` SimSteps back to back on the device with zero-guess CG; returns the
+ * total CG iterations.  Host synchronises once per CG chunk only. */
+int yc_run(yc_ctx* ctx, double h, int steps, double tol, int maxit, long long* total_iters);
+
+/* Times `reps` launches of the stencil SpMV kernel on the context's stream with HIP
+ * events; avg_ms receives the mean duration of one launch. */
+int yc_time_spmv(yc_ctx* ctx, int reps, double* avg_ms);
+
+/* Algorithmic HBM bytes of one SpMV launch (stencil values + input and output vectors). */
+double yc_spmv_bytes(const yc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,89 @@
+/*
+ * yarn_oracle.h -- C API of the CPU oracle for the implicit yarn-level step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain FP64 CPU restatement of
+ * the reference's step (src/PhysicsCloth.cpp, src/Cloth.cpp, src/MatrixMp.cpp,
+ * src/TrainHelper.cpp, src/Solvers.cpp).  It is the checker the HIP product is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  Nothing in the product links it.
+ *
+ * Layouts follow the reference:
+ *   node index        k = u + v*width                      (src/Cloth.cpp:201)
+ *   crossing index    e = (u-1) + (v-1)*(width-2)           (src/Cloth.cpp:221)
+ *   global vector     [L: 3k+{x,y,z}] ++ [E: 3N + 2e+{u,v}] (src/MatrixMp.cpp:120)
+ *   state arrays      index u + v*length, as get/set_ClothState do (src/Cloth.cpp:362,426)
+ */
+#ifndef YARN_ORACLE_H
+#define YARN_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YO_MAX_HANDLES 16
+
+typedef struct {
+    int width, length;
+    double hight, L, R;
+    int n_yarns;                 /* 1 (homogeneous) or 2 (warp row / weft column i uses yarn i%2) */
+    double rho[2], Y[2], B[2];   /* per yarn: density, Young's modulus, bending modulus */
+    double kc, kf, df, mu, S, handle_k;
+    int pattern;                 /* 0 plain, 1 twill, 2 satin (src/Cloth.cpp:70-88) */
+    int pose;                    /* 0 upright, 1 flat (src/Cloth.cpp:58-59) */
+    double G[3];
+    double wind_vel[3], wind_density, wind_drag;
+    int n_handles;
+    int handles[2 * YO_MAX_HANDLES];   /* (u, v) pairs */
+} yo_params;
+
+typedef struct yo_cloth yo_cloth;
+
+yo_cloth* yo_create(const yo_params* p);
+void yo_destroy(yo_cloth* c);
+
+/* n = 3N + 2Nc, N = width*length, Nc = (width-2)*(length-2) */
+int yo_ndof(const yo_cloth* c);
+
+/* get/set_ClothState (src/Cloth.cpp:350-438).  Arrays hold width*length entries
+ * of 3 (LPos, LPosFix, LVel) or 2 (EPos, EPosBar, EVel) doubles. */
+void yo_get_state(const yo_cloth* c, double* lpos, double* lposfix, double* lvel,
+                  double* epos, double* eposbar, double* evel);
+void yo_set_state(yo_cloth* c, const double* lpos, const double* lposfix, const double* lvel,
+                  const double* epos, const double* eposbar, const double* evel);
+
+/* FillMV + system formation (src/PhysicsCloth.cpp:46-156, src/TrainHelper.cpp:19-20,61-63).
+ * ext_force may be NULL (n doubles otherwise).  Returns 0, or 1 on NaN/Inf in A or b. */
+int yo_assemble(yo_cloth* c, double h, const double* ext_force);
+
+/* Results of the last yo_assemble. */
+int yo_nnz(const yo_cloth* c);
+/* which: 0 = A, 1 = M, 2 = K_x (F_*_Pos), 3 = K_v (F_*_Vel); CSR of the global n x n matrix */
+void yo_get_csr(const yo_cloth* c, int which, int* rowptr, int* col, double* val);
+void yo_get_dense(const yo_cloth* c, int which, double* dense /* n*n row-major */);
+/* which: 0 = F (get_SpaF), 1 = b, 2 = v (get_SpaVel) */
+void yo_get_vec(const yo_cloth* c, int which, double* out);
+/* per crossing: is_slide (2), comp (1), thread-partial friction input F_E (2) */
+void yo_get_crossing(const yo_cloth* c, double* is_slide, double* comp, double* fe_partial);
+/* per node normal (3 doubles, zero on the boundary) */
+void yo_get_normals(const yo_cloth* c, double* n);
+
+/* Jacobi-preconditioned CG with the reference solver's stopping rule
+ * (squared residual below tol^2*|b|^2, at most maxit iterations, x0 or zero guess). */
+int yo_solve(yo_cloth* c, double tol, int maxit, const double* x0, double* x,
+             int* iters, double* relres);
+
+/* Cloth::update (src/Cloth.cpp:111-148) with the is_slide of the last assembly. */
+void yo_update(yo_cloth* c, const double* new_vel, double h);
+
+/* SimStep (src/TrainHelper.cpp:15-97): assemble, solve, update.  Returns 0 ok,
+ * 1 NaN/Inf in A/b, 2 NaN/Inf in the solution. */
+int yo_step(yo_cloth* c, double h, const double* ext_force, double tol, int maxit,
+            int* iters, double* relres);
+
+/* Static-schedule owner thread used for the friction input (8 threads). */
+int yo_owner(int n_iter, int i);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+"""GPU parity: the HIP step against the CPU oracle on the same seeded inputs.
+
+Tolerances: per-step F, A and b within 1e-10 relative (Frobenius / 2-norm over the
+whole object); trajectories within 1e-6 relative over 100 steps.  Solves use a
+1e-12 relative residual on both sides so the comparison measures the step, not the
+solver tolerance.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleCloth
+from tests.golden_cases import cases, excited_state
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-10
+TOL_TRAJ = 1e-6
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(ys, spec, state):
+    g = ys.Cloth(spec)
+    g.set_ClothState(state)
+    r = OracleCloth(spec)
+    r.set_state(state)
+    return g, r
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_assembly_matches_oracle(ys, cs, idx):
+    name, spec = cases(cs)[idx]
+    say("case %s W=%d" % (name, spec["width"]))
+    state = excited_state(cs, spec, 100 + idx)
+    g, r = _pair(ys, spec, state)
+    h = spec["h"]
+    g.assemble(h)
+    r.assemble(h)
+    np.testing.assert_allclose(g.normals(), r.normals(), atol=1e-12)
+    A_g, A_r = g.csr().toarray(), r.csr(0).toarray()
+    assert A_g.shape == A_r.shape
+    # the oracle's stored entries lie inside the stencil pattern the GPU exports
+    pat = g.csr()
+    pat.data[:] = 1.0
+    assert np.all(pat.toarray()[A_r != 0] == 1.0)
+    eA = np.linalg.norm(A_g - A_r) / np.linalg.norm(A_r)
+    eF = rel(g.vec("F"), r.vec("F"))
+    eb = rel(g.vec("b"), r.vec("b"))
+    sl_g, cp_g, fp_g = g.crossing()
+    sl_r, cp_r, fp_r = r.crossing()
+    say("  |dA|=%.2e |dF|=%.2e |db|=%.2e" % (eA, eF, eb))
+    assert eA < TOL_STEP and eF < TOL_STEP and eb < TOL_STEP
+    assert rel(cp_g, cp_r) < TOL_STEP
+    assert rel(fp_g, fp_r) < TOL_STEP
+    np.testing.assert_allclose(sl_g, sl_r, rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["homo5", "heter64_twill"])
+def test_workload_first_step(ys, cs, name):
+    say("workload %s" % name)
+    spec, state, ext, _ = cs.workload(name)
+    g, r = _pair(ys, spec, state)
+    h = spec["h"]
+    g.assemble(h)
+    r.assemble(h, ext)
+    eA = np.linalg.norm((g.csr() - r.csr(0)).toarray()) / np.linalg.norm(r.csr(0).toarray())
+    eb = rel(g.vec("b"), r.vec("b"))
+    say("  |dA|=%.2e |db|=%.2e" % (eA, eb))
+    assert eA < TOL_STEP and eb < TOL_STEP
+
+
+def test_solve_matches_oracle(ys, cs):
+    spec = cases(cs)[1][1]
+    state = excited_state(cs, spec, 7)
+    g, r = _pair(ys, spec, state)
+    h = spec["h"]
+    g.assemble(h)
+    r.assemble(h)
+    it_g, rr_g, ok = g.solve(tol=1e-12)
+    x_r, it_r, rr_r, st = r.solve(tol=1e-12, maxit=20000)
+    say("  cg iters gpu %d oracle %d relres %.2e %.2e" % (it_g, it_r, rr_g, rr_r))
+    assert ok and rr_g < 1e-12
+    assert rel(g.vec("x"), x_r) < 1e-9
+
+
+def test_trajectory_100_steps(ys, cs):
+    spec, state, ext, steps = cs.workload("homo5")
+    g, r = _pair(ys, spec, state)
+    h = spec["h"]
+    worst = 0.0
+    for s in range(steps):
+        it, rr, rc = g.step(h, tol=1e-12)
+        st, it_r, rr_r = r.step(h, ext, tol=1e-12, maxit=20000)
+        assert rc == 0 and st == 0
+        if s % 10 == 9 or s == steps - 1:
+            sg, sr = g.get_ClothState(), r.get_state()
+            e = max(rel(sg[k], sr[k]) for k in ("LPos", "LVel", "EPos", "EVel", "EPosBar"))
+            worst = max(worst, e)
+            say("  step %d rel %.2e" % (s + 1, e))
+    assert worst < TOL_TRAJ
