@@ -80,8 +80,8 @@ def test_workload_first_step(ys, cs, name):
 
 
 def test_solve_matches_oracle(ys, cs):
-    spec = cases(cs)[1][1]
-    state = excited_state(cs, spec, 7)
+    spec = cases(cs)[2][1]
+    state = cs.initial_state(spec)
     g, r = _pair(ys, spec, state)
     h = spec["h"]
     g.assemble(h)
