@@ -99,7 +99,7 @@ def test_trajectory_100_steps(ys, cs):
     h = spec["h"]
     worst = 0.0
     for s in range(steps):
-        it, rr, rc = g.step(h, tol=1e-12)
+        it, rr, rc = g.step(h, tol=1e-12, maxit=20000)
         st, it_r, rr_r = r.step(h, ext, tol=1e-12, maxit=20000)
         assert rc == 0 and st == 0
         if s % 10 == 9 or s == steps - 1:
