@@ -45,7 +45,10 @@ class YcStepInfo(ctypes.Structure):
 # every symbol include/yarnstep.h declares
 EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state", "yc_get_state",
            "yc_set_ext_force", "yc_assemble", "yc_export_csr", "yc_export_vec", "yc_export_crossing",
-           "yc_export_normals", "yc_solve", "yc_update", "yc_step", "yc_run", "yc_time_spmv", "yc_spmv_bytes"]
+           "yc_export_normals", "yc_solve", "yc_update", "yc_step", "yc_run", "yc_time_spmv", "yc_spmv_bytes",
+           "yc_mlp_param_count", "yc_mlp_create", "yc_mlp_destroy", "yc_mlp_set_params", "yc_mlp_get_params",
+           "yc_mlp_set_norm", "yc_mlp_set_data", "yc_mlp_forward", "yc_mlp_grad", "yc_mlp_train", "yc_features",
+           "yc_targets", "yc_step_warm", "yc_run_warm"]
 
 
 def lib_path():
@@ -87,6 +90,23 @@ def load_library():
     L.yc_time_spmv.argtypes = [vp, ctypes.c_int, dp]
     L.yc_spmv_bytes.argtypes = [vp]
     L.yc_spmv_bytes.restype = ctypes.c_double
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.yc_mlp_param_count.argtypes = []
+    L.yc_mlp_create.argtypes = [ctypes.POINTER(vp)]
+    L.yc_mlp_destroy.argtypes = [vp]
+    L.yc_mlp_destroy.restype = None
+    L.yc_mlp_set_params.argtypes = [vp, fp]
+    L.yc_mlp_get_params.argtypes = [vp, fp]
+    L.yc_mlp_set_norm.argtypes = [vp, fp, fp, fp]
+    L.yc_mlp_set_data.argtypes = [vp, fp, fp, fp, ctypes.c_int]
+    L.yc_mlp_forward.argtypes = [vp, fp, ctypes.c_int, fp]
+    L.yc_mlp_grad.argtypes = [vp, ip, ctypes.c_int, fp, fp]
+    L.yc_mlp_train.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp]
+    L.yc_features.argtypes = [vp, fp]
+    L.yc_targets.argtypes = [vp, fp]
+    L.yc_step_warm.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]
+    L.yc_run_warm.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_longlong)]
     _lib = L
     return L
 

@@ -139,6 +139,44 @@ class Cloth:
             _abi.check(rc)
         return tot.value, rc
 
+    # --- learned warm start ---
+    def features(self):
+        out = np.zeros((self.N, 32), dtype=np.float32)
+        _abi.check(_abi.load_library().yc_features(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def targets(self):
+        out = np.zeros((self.N, 5), dtype=np.float32)
+        _abi.check(_abi.load_library().yc_targets(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def node_mask(self):
+        m = np.ones((self.N, 5), dtype=np.float32)
+        W, Lg = self.width, self.length
+        for v in range(Lg):
+            for u in range(W):
+                if not (0 < u < W - 1 and 0 < v < Lg - 1):
+                    m[u + v * W, 3:] = 0.0
+        return m
+
+    def step_warm(self, mlp, h, tol=1e-8, maxit=None):
+        if maxit is None:
+            maxit = 2 * self.n
+        info = _abi.YcStepInfo()
+        rc = _abi.load_library().yc_step_warm(self._h, mlp._h, float(h), float(tol), int(maxit), ctypes.byref(info))
+        if rc not in (_abi.YC_OK, _abi.YC_ERR_NOT_CONVERGED, _abi.YC_ERR_NONFINITE):
+            _abi.check(rc)
+        return info.cg_iters, info.cg_relres, rc
+
+    def run_warm(self, mlp, h, steps, tol=1e-8, maxit=None):
+        if maxit is None:
+            maxit = 2 * self.n
+        tot = ctypes.c_longlong()
+        rc = _abi.load_library().yc_run_warm(self._h, mlp._h, float(h), int(steps), float(tol), int(maxit), ctypes.byref(tot))
+        if rc not in (_abi.YC_OK, _abi.YC_ERR_NOT_CONVERGED):
+            _abi.check(rc)
+        return tot.value, rc
+
     def time_spmv(self, reps=200):
         ms = ctypes.c_double()
         _abi.check(_abi.load_library().yc_time_spmv(self._h, int(reps), ctypes.byref(ms)))

@@ -49,15 +49,25 @@ def homo_spec(W, pattern="plain"):
 def heter_spec(W, pattern, seed):
     """Two-yarn blend (SimHeter-style): warp row / weft column i uses yarn i%2
     (src/Cloth.cpp:199-202).  rho, Y and B of each yarn are drawn log-uniform within
-    +-1/2 decade of the cotton / polyester values; S and mu are drawn once per cloth
-    (they are cloth-level in the reference, src/Cloth.h:34)."""
+    +-0.3 decade of the cotton / polyester values, redrawn while Y/rho exceeds
+    cotton's (the reference's step becomes unstable for stiffer-per-mass yarns); S
+    and mu are drawn once per cloth (they are cloth-level in the reference,
+    src/Cloth.h:34)."""
     rng = np.random.default_rng(seed)
     spec = base_spec(W, pattern=pattern)
+    cap = COTTON["Y"] / COTTON["rho"]
 
-    def lu(center):
-        return float(center * 10.0 ** rng.uniform(-0.5, 0.5))
+    def lu(center, dec=0.3):
+        return float(center * 10.0 ** rng.uniform(-dec, dec))
 
-    spec["yarns"] = [{k: lu(y[k]) for k in ("rho", "Y", "B")} for y in (COTTON, POLYESTER)]
+    yarns = []
+    for base in (COTTON, POLYESTER):
+        while True:
+            y = {k: lu(base[k]) for k in ("rho", "Y", "B")}
+            if y["Y"] / y["rho"] <= cap:
+                break
+        yarns.append(y)
+    spec["yarns"] = yarns
     spec["S"] = lu(1000.0)
     spec["mu"] = lu(0.5)
     return spec

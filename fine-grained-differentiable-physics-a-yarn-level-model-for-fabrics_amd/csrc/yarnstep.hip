@@ -43,11 +43,27 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr int BLK = 256;
+// per-node kernels (assembly, update, normals) are heavy per thread and have only one
+// thread per node: small blocks spread them over all 256 CUs
+constexpr int NBLK = 64;
 
 struct CGScal {
-    double rz, pAp, beta, rr, bb, thr;
-    int done, iters, maxit, zero_rhs;
-    unsigned int ticket[4];
+    double rr, bb, thr;
+    int done, maxit, zero_rhs;
+    int itc[2];   // completed iterations, written by parity so readers never race the writer
+};
+
+// Partial sums of the CG reductions, one slot per block of the producing kernel.  A
+// consumer kernel reduces them in every block (fixed order, identical result in all
+// blocks), so no device-scope fence or atomic is needed inside a kernel: the kernel
+// boundary makes the partials visible.  r.z alternates between two buffers by
+// iteration parity so that the old and the new value are both readable.
+struct CGPart {
+    double* pAp;     // per SpMV block
+    double* rr;      // per vector block
+    double* rz[2];   // per vector block, by parity
+    double* bb;      // per vector block (init)
+    int nS, nV;      // number of SpMV / vector blocks
 };
 
 // ---------------------------------------------------------------------------
@@ -73,21 +89,14 @@ __device__ double block_sum_partials(const double* partials, int n) {
     return block_sum(s);
 }
 
-// Last-block-done pattern: returns true in every thread of the last block to finish.
-__device__ bool last_block(unsigned int* ticket) {
-    __shared__ bool am_last;
-    __threadfence();
+// Block-uniform read of the stop flag (thread 0 reads, the block shares it), so that a
+// block never diverges around the barriers of the reductions.
+__device__ __forceinline__ int ld_done(const CGScal* s) {
+    __shared__ int d;
+    if (threadIdx.x == 0) d = __atomic_load_n(&s->done, __ATOMIC_RELAXED);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int t = atomicAdd(ticket, 1u);
-        am_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (am_last) __threadfence();
-    return am_last;
+    return d;
 }
-
-__device__ __forceinline__ int ld_done(const CGScal* s) { return __atomic_load_n(&s->done, __ATOMIC_RELAXED); }
 
 // ---------------------------------------------------------------------------
 // Assembly
@@ -107,7 +116,7 @@ struct AsmOut {
 
 __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
 
-__global__ void __launch_bounds__(BLK) k_assemble1(ClothDev C, AsmOut O, double h) {
+__global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double h) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= C.N) return;
     for (int s = 0; s < NSLOT; ++s) O.A[(size_t)s * C.N + k] = 0.0;
@@ -178,7 +187,7 @@ __global__ void __launch_bounds__(BLK) k_assemble1(ClothDev C, AsmOut O, double 
 template <class Sink>
 __device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp);
 
-__global__ void __launch_bounds__(BLK) k_assemble2(ClothDev C, AsmOut O, double h) {
+__global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double h) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= C.N) return;
     RowSink S;
@@ -225,6 +234,7 @@ __global__ void __launch_bounds__(BLK) k_assemble2(ClothDev C, AsmOut O, double 
 }  // namespace
 
 #include "shear.hpp"
+#include "mlp.hpp"
 
 namespace {
 
@@ -284,7 +294,7 @@ __device__ __forceinline__ double spmv_row(const SpmvArgs& a, int k, int g) {
 }
 
 // y = A x over the 4N row groups; optionally partial sums of x . y per block
-__global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, CGScal* sc, int check_done) {
+__global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, const CGScal* sc, int check_done) {
     if (check_done && ld_done(sc)) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     double dotp = 0.0;
@@ -321,10 +331,6 @@ __global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, CGSc
     if (partials) {
         double s = block_sum(dotp);
         if (threadIdx.x == 0) partials[blockIdx.x] = s;
-        if (last_block(&sc->ticket[0])) {
-            double tot = block_sum_partials(partials, gridDim.x);
-            if (threadIdx.x == 0) { sc->pAp = tot; sc->ticket[0] = 0; }
-        }
     }
 }
 
@@ -332,7 +338,7 @@ __global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, CGSc
 // CG vector kernels (Jacobi-PCG in the reference solver's order, src/Solvers.cpp:22-24)
 // ---------------------------------------------------------------------------
 struct VecArgs {
-    double* x; double* r; double* p; double* y; const double* b; const double* invd;
+    double* x; double* r; double* pb[2]; double* y; const double* b; const double* invd;
     int W, Lg, N, Wp, P;
 };
 
@@ -344,8 +350,9 @@ __device__ __forceinline__ int vec_index(const VecArgs& a, int t, int& c) {
     return c * a.P + padidx(a.Wp, u, v);
 }
 
-// init: r = b - y (y = A x0 when has_x0, else x = 0, r = b); p = D^-1 r
-__global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, double* partials, CGScal* sc, int has_x0, double tol, int maxit) {
+// init: r = b - y (y = A x0 when has_x0, else x = 0, r = b); p = D^-1 r; partials of
+// r.r, r.z (into parity buffer 1, read by iteration 0) and b.b
+__global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, CGPart P, int has_x0) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     double rr = 0.0, rz = 0.0, bb = 0.0;
     if (t < 5 * a.N) {
@@ -356,74 +363,80 @@ __global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, double* partials, CG
         if (!has_x0) a.x[i] = 0.0;
         double zi = a.invd[i] * ri;
         a.r[i] = ri;
-        a.p[i] = zi;
+        a.pb[0][i] = zi;
         rr = ri * ri; rz = ri * zi; bb = bi * bi;
     }
     double s1 = block_sum(rr);
     double s2 = block_sum(rz);
     double s3 = block_sum(bb);
-    if (threadIdx.x == 0) { partials[3 * blockIdx.x] = s1; partials[3 * blockIdx.x + 1] = s2; partials[3 * blockIdx.x + 2] = s3; }
-    if (last_block(&sc->ticket[1])) {
-        double a1 = 0, a2 = 0, a3 = 0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) { a1 += partials[3 * i]; a2 += partials[3 * i + 1]; a3 += partials[3 * i + 2]; }
-        a1 = block_sum(a1); a2 = block_sum(a2); a3 = block_sum(a3);
-        if (threadIdx.x == 0) {
-            sc->rr = a1; sc->rz = a2; sc->bb = a3;
-            double thr = tol * tol * a3;
-            sc->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
-            sc->iters = 0; sc->maxit = maxit; sc->zero_rhs = a3 == 0.0;
-            sc->done = (a3 == 0.0 || a1 < sc->thr || maxit <= 0) ? 1 : 0;
-            sc->ticket[1] = 0;
-        }
+    if (threadIdx.x == 0) { P.rr[blockIdx.x] = s1; P.rz[1][blockIdx.x] = s2; P.bb[blockIdx.x] = s3; }
+}
+
+// one block: stopping threshold and the initial decision (src/Solvers.cpp:22-24 semantics)
+__global__ void __launch_bounds__(BLK) k_cg_init2(CGPart P, CGScal* sc, double tol, int maxit) {
+    double rr = block_sum_partials(P.rr, P.nV);
+    double bb = block_sum_partials(P.bb, P.nV);
+    if (threadIdx.x == 0) {
+        sc->rr = rr; sc->bb = bb;
+        double thr = tol * tol * bb;
+        sc->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+        sc->itc[0] = 0; sc->itc[1] = 0; sc->maxit = maxit; sc->zero_rhs = bb == 0.0;
+        sc->done = (bb == 0.0 || rr < sc->thr || maxit <= 0) ? 1 : 0;
     }
 }
 
-// alpha = rz / pAp; x += alpha p; r -= alpha y; then |r|^2, r.z and the loop decision
-__global__ void __launch_bounds__(BLK) k_cg_xr(VecArgs a, double* partials, CGScal* sc) {
+__device__ __forceinline__ double bcast(double v) {
+    __shared__ double s;
+    if (threadIdx.x == 0) s = v;
+    __syncthreads();
+    double r = s;
+    __syncthreads();
+    return r;
+}
+
+// alpha = r.z / p.Ap; x += alpha p; r -= alpha y; partials of |r|^2 and r.z
+__global__ void __launch_bounds__(BLK) k_cg_xr(VecArgs a, CGPart P, const CGScal* sc, int q) {
     if (ld_done(sc)) return;
-    double alpha = sc->rz / sc->pAp;
+    double pAp = bcast(block_sum_partials(P.pAp, P.nS));
+    double rz = bcast(block_sum_partials(P.rz[1 - q], P.nV));
+    double alpha = rz / pAp;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double rr = 0.0, rz = 0.0;
+    double rr = 0.0, rzn = 0.0;
     if (t < 5 * a.N) {
         int c;
         int i = vec_index(a, t, c);
-        a.x[i] += alpha * a.p[i];
+        a.x[i] += alpha * a.pb[0][i];
         double ri = a.r[i] - alpha * a.y[i];
         a.r[i] = ri;
         rr = ri * ri;
-        rz = ri * (a.invd[i] * ri);
+        rzn = ri * (a.invd[i] * ri);
     }
     double s1 = block_sum(rr);
-    double s2 = block_sum(rz);
-    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = s1; partials[2 * blockIdx.x + 1] = s2; }
-    if (last_block(&sc->ticket[2])) {
-        double a1 = 0, a2 = 0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) { a1 += partials[2 * i]; a2 += partials[2 * i + 1]; }
-        a1 = block_sum(a1); a2 = block_sum(a2);
-        if (threadIdx.x == 0) {
-            sc->rr = a1;
-            if (a1 < sc->thr) {
-                sc->done = 1;
-            } else {
-                sc->beta = a2 / sc->rz;
-                sc->rz = a2;
-                sc->iters += 1;
-                if (sc->iters >= sc->maxit) sc->done = 1;
-            }
-            sc->ticket[2] = 0;
-        }
-    }
+    double s2 = block_sum(rzn);
+    if (threadIdx.x == 0) { P.rr[blockIdx.x] = s1; P.rz[q][blockIdx.x] = s2; }
 }
 
-// p = D^-1 r + beta p
-__global__ void __launch_bounds__(BLK) k_cg_p(VecArgs a, CGScal* sc) {
+// Stopping test on the |r|^2 of k_cg_xr, beta, p = D^-1 r + beta p.  Block 0 keeps the
+// iteration count in the parity slot q (readers use slot 1-q, so no block ever reads
+// a slot being written in the same launch).
+__global__ void __launch_bounds__(BLK) k_cg_p(VecArgs a, CGPart Pt, CGScal* sc, int q) {
     if (ld_done(sc)) return;
-    double beta = sc->beta;
+    double rr = bcast(block_sum_partials(Pt.rr, Pt.nV));
+    double rzn = bcast(block_sum_partials(Pt.rz[q], Pt.nV));
+    double rzo = bcast(block_sum_partials(Pt.rz[1 - q], Pt.nV));
+    bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (rr < sc->thr) {
+        if (lead) { sc->rr = rr; sc->done = 1; }
+        return;
+    }
+    int itn = sc->itc[1 - q] + 1;
+    if (lead) { sc->rr = rr; sc->itc[q] = itn; if (itn >= sc->maxit) sc->done = 1; }
+    double beta = rzn / rzo;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 5 * a.N) {
         int c;
         int i = vec_index(a, t, c);
-        a.p[i] = a.invd[i] * a.r[i] + beta * a.p[i];
+        a.pb[0][i] = a.invd[i] * a.r[i] + beta * a.pb[0][i];
     }
 }
 
@@ -436,7 +449,7 @@ struct StateDev {
     int W, Lg, N;
 };
 
-__global__ void __launch_bounds__(BLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h) {
+__global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
@@ -462,7 +475,7 @@ __global__ void __launch_bounds__(BLK) k_update(StateDev s, const double* sol, c
 // Cloth::NodeNorm (src/Cloth.cpp:440-463): smallest-eigenvalue eigenvector of the 3x3
 // scatter matrix of the 5-point cross (cyclic Jacobi), aligned with
 // (x_right - x_left) x (x_bot - x_top), flipped by the weave pattern.
-__global__ void __launch_bounds__(BLK) k_normals(StateDev s) {
+__global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
@@ -529,7 +542,9 @@ struct yc_ctx {
     double *A, *F, *is_slide, *comp, *fepart;
     double *b, *invd, *cx, *cr, *cp, *cy;  // padded 5-plane vectors
     double* ext;
+    double *vpad, *dprev, *Av;   // padded v_old, previous dv, A v_old (learned warm start)
     double* partials;
+    CGPart part;
     CGScal* sc;
     int* flag;
     CGScal* h_sc;  // pinned
@@ -539,6 +554,7 @@ struct yc_ctx {
     bool assembled, solved;
     hipGraphExec_t graph;
     int graph_chunk;
+    int last_iters;
     int ks_cg_blocks;
 };
 
@@ -582,7 +598,7 @@ StateDev make_state(yc_ctx* c) {
 
 VecArgs make_vec(yc_ctx* c) {
     VecArgs a;
-    a.x = c->cx; a.r = c->cr; a.p = c->cp; a.y = c->cy; a.b = c->b; a.invd = c->invd;
+    a.x = c->cx; a.r = c->cr; a.pb[0] = c->cp; a.pb[1] = c->cp; a.y = c->cy; a.b = c->b; a.invd = c->invd;
     a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P;
     return a;
 }
@@ -604,17 +620,18 @@ bool pattern_flip(int pattern, int i, int j) {
 }
 
 int refresh_geometry(yc_ctx* c) {
-    hipLaunchKernelGGL(k_normals, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, make_state(c));
+    hipLaunchKernelGGL(k_normals, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c));
     HIPCHK(hipGetLastError());
     return YC_OK;
 }
 
-int cg_launch_iteration(yc_ctx* c) {
-    int nb_spmv = grid_for(4LL * c->N);
-    int nb_vec = grid_for(5LL * c->N);
-    hipLaunchKernelGGL(k_spmv, dim3(nb_spmv), dim3(BLK), 0, c->stream, make_spmv(c, c->cp, c->cy), c->partials, c->sc, 1);
-    hipLaunchKernelGGL(k_cg_xr, dim3(nb_vec), dim3(BLK), 0, c->stream, make_vec(c), c->partials, c->sc);
-    hipLaunchKernelGGL(k_cg_p, dim3(nb_vec), dim3(BLK), 0, c->stream, make_vec(c), c->sc);
+// CG iteration with parity q: y = A p and p.y partials; x, r update with |r|^2 and r.z
+// partials (r.z into slot q); stopping test and p update
+int cg_launch_iteration(yc_ctx* c, int q) {
+    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->cp, c->cy), c->part.pAp,
+                       (const CGScal*)c->sc, 1);
+    hipLaunchKernelGGL(k_cg_xr, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, (const CGScal*)c->sc, q);
+    hipLaunchKernelGGL(k_cg_p, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, c->sc, q);
     return YC_OK;
 }
 
@@ -622,7 +639,7 @@ int build_graph(yc_ctx* c) {
     if (c->graph) return YC_OK;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < c->graph_chunk; ++i) cg_launch_iteration(c);
+    for (int i = 0; i < c->graph_chunk; ++i) cg_launch_iteration(c, i & 1);   // chunk is even
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
@@ -659,10 +676,12 @@ void from_padded(const yc_ctx* c, const std::vector<double>& pad, double* glob) 
 int do_assemble(yc_ctx* c, double h) {
     HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
-    hipLaunchKernelGGL(k_assemble1, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble1, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_assemble2, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble2, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
+                       c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
     HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->assembled = true;
@@ -671,33 +690,43 @@ int do_assemble(yc_ctx* c, double h) {
     return YC_OK;
 }
 
-int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* iters, double* relres) {
+int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* iters, double* relres, bool x0_in_cx = false) {
     if (!c->assembled) return fail(YC_ERR_ARG, "yc_solve before yc_assemble");
     int has_x0 = 0;
-    if (x0_global) {
+    if (x0_in_cx) {
+        hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->cx, c->cy), (double*)nullptr,
+                           (const CGScal*)c->sc, 0);
+        has_x0 = 1;
+    } else if (x0_global) {
         std::vector<double> pad;
         to_padded(c, x0_global, pad);
         HIPCHK(hipMemcpyAsync(c->cx, pad.data(), pad.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_spmv, dim3(grid_for(4LL * c->N)), dim3(BLK), 0, c->stream, make_spmv(c, c->cx, c->cy), (double*)nullptr, c->sc, 0);
+        hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->cx, c->cy), (double*)nullptr,
+                           (const CGScal*)c->sc, 0);
         has_x0 = 1;
     }
-    hipLaunchKernelGGL(k_cg_init, dim3(grid_for(5LL * c->N)), dim3(BLK), 0, c->stream, make_vec(c), c->partials, c->sc, has_x0, tol, maxit);
+    hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, has_x0);
+    hipLaunchKernelGGL(k_cg_init2, dim3(1), dim3(BLK), 0, c->stream, c->part, c->sc, tol, maxit);
     HIPCHK(hipGetLastError());
     int rc = build_graph(c);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     long long launched = 0;
+    // launch ahead as many chunks as the previous solve needed, then check once per chunk
+    int ahead = std::max(1, (int)(0.9 * c->last_iters) / c->graph_chunk);
     while (!c->h_sc->done && launched < (long long)maxit + c->graph_chunk) {
-        HIPCHK(hipGraphLaunch(c->graph, c->stream));
-        launched += c->graph_chunk;
+        int n = launched == 0 ? ahead : 1;
+        for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+        launched += (long long)n * c->graph_chunk;
         HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     if (c->h_sc->zero_rhs) HIPCHK(hipMemsetAsync(c->cx, 0, (size_t)5 * c->P * sizeof(double), c->stream));
     c->solved = true;
     double rel = c->h_sc->bb > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bb) : 0.0;
-    if (iters) *iters = c->h_sc->iters;
+    c->last_iters = std::max(c->h_sc->itc[0], c->h_sc->itc[1]);
+    if (iters) *iters = c->last_iters;
     if (relres) *relres = rel;
     if (!std::isfinite(rel)) return fail(YC_ERR_NONFINITE, "NaN/Inf in the solution");
     if (c->h_sc->rr >= c->h_sc->thr && !c->h_sc->zero_rhs) return fail(YC_ERR_NOT_CONVERGED, "CG stopped at maxit above tolerance");
@@ -706,7 +735,9 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
 
 int do_update(yc_ctx* c, double h) {
     if (!c->solved) return fail(YC_ERR_ARG, "yc_update before yc_solve");
-    hipLaunchKernelGGL(k_update, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, make_state(c), (const double*)c->cx,
+    hipLaunchKernelGGL(k_dprev, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->cx, (const double*)c->vpad,
+                       5 * c->P, c->dprev);
+    hipLaunchKernelGGL(k_update, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), (const double*)c->cx,
                        (const double*)c->is_slide, c->Wp, c->P, h);
     HIPCHK(hipGetLastError());
     int rc = refresh_geometry(c);
@@ -746,7 +777,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     c->n = 3 * c->N + 2 * c->Nc;
     c->Wp = c->W + 4; c->Hp = c->Lg + 4;
     c->P = ((c->Wp * c->Hp + 63) / 64) * 64;
-    c->graph_chunk = 16;
+    c->graph_chunk = 32;
     int rc = YC_OK;
     auto chk = [&](int r) { if (r && !rc) rc = r; };
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return fail(YC_ERR_HIP, "no HIP device"); }
@@ -761,7 +792,17 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     chk(dalloc(&c->b, PV)); chk(dalloc(&c->invd, PV)); chk(dalloc(&c->cx, PV)); chk(dalloc(&c->cr, PV));
     chk(dalloc(&c->cp, PV)); chk(dalloc(&c->cy, PV));
     chk(dalloc(&c->ext, (size_t)c->n));
-    chk(dalloc(&c->partials, (size_t)3 * grid_for(5LL * c->N) + 64));
+    chk(dalloc(&c->vpad, PV)); chk(dalloc(&c->dprev, PV)); chk(dalloc(&c->Av, PV));
+    c->part.nS = grid_for(4LL * c->N);
+    c->part.nV = grid_for(5LL * c->N);
+    chk(dalloc(&c->partials, (size_t)c->part.nS + 4 * (size_t)c->part.nV + 64));
+    if (!rc) {
+        c->part.pAp = c->partials;
+        c->part.rr = c->part.pAp + c->part.nS;
+        c->part.rz[0] = c->part.rr + c->part.nV;
+        c->part.rz[1] = c->part.rz[0] + c->part.nV;
+        c->part.bb = c->part.rz[1] + c->part.nV;
+    }
     chk(dalloc(&c->sc, 1));
     chk(dalloc(&c->flag, 4));
     if (!rc && hipHostMalloc((void**)&c->h_sc, sizeof(CGScal)) != hipSuccess) rc = fail(YC_ERR_HIP, "pinned alloc");
@@ -804,7 +845,7 @@ void yc_destroy(yc_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
-                    c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->partials, c->sc, c->flag};
+                    c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
@@ -1067,11 +1108,12 @@ int yc_time_spmv(yc_ctx* c, int reps, double* avg_ms) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    // the CG loop's stencil product y = A p (partials of p.y included, as in the loop)
     SpmvArgs a = make_spmv(c, c->cp, c->cy);
-    hipLaunchKernelGGL(k_spmv, dim3(grid_for(4LL * c->N)), dim3(BLK), 0, c->stream, a, (double*)nullptr, c->sc, 0);
+    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, a, c->part.pAp, (const CGScal*)c->sc, 0);
     HIPCHK(hipEventRecord(e0, c->stream));
     for (int i = 0; i < reps; ++i)
-        hipLaunchKernelGGL(k_spmv, dim3(grid_for(4LL * c->N)), dim3(BLK), 0, c->stream, a, (double*)nullptr, c->sc, 0);
+        hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, a, c->part.pAp, (const CGScal*)c->sc, 0);
     HIPCHK(hipEventRecord(e1, c->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -1087,7 +1129,300 @@ double yc_spmv_bytes(const yc_ctx* c) {
     // structural nonzeros of A (LL 13-point stencil + LE/EL/EE along the yarns)
     double W = c->W;
     double nnz = 9.0 * (13.0 * W * W - 20.0 * W - 4.0) + 14.0 * (W - 2.0) * (3.0 * W - 8.0);
+    // values once, p read and A p written once per DOF
     return 8.0 * nnz + 16.0 * c->n;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned warm start
+// ---------------------------------------------------------------------------
+namespace {
+
+int mlp_refresh_bf16(yc_mlp* m) {
+    hipLaunchKernelGGL(k_mlp_to_bf16, dim3((MLP_H * MLP_H + 255) / 256), dim3(256), 0, m->stream, (const float*)m->param,
+                       m->W1, m->W2, m->W3, m->W4);
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+MlpW mlp_w(const yc_mlp* m) {
+    MlpW w;
+    w.W1 = m->W1; w.W2 = m->W2; w.W3 = m->W3; w.W4 = m->W4; w.b = m->param;
+    return w;
+}
+
+int mlp_reserve_batch(yc_mlp* m, int rows) {
+    if (rows <= m->cap_batch) return YC_OK;
+    void* bufs[] = {m->idx, m->sX0, m->sZ, m->sH, m->Y, m->part};
+    for (void* p : bufs) if (p) (void)hipFree(p);
+    int nblk = (rows + MLP_TILE - 1) / MLP_TILE;
+    HIPCHK(hipMalloc((void**)&m->idx, (size_t)rows * sizeof(int)));
+    HIPCHK(hipMalloc((void**)&m->sX0, (size_t)rows * MLP_IN * 2));
+    HIPCHK(hipMalloc((void**)&m->sZ, (size_t)3 * rows * MLP_H * 4));
+    HIPCHK(hipMalloc((void**)&m->sH, (size_t)3 * rows * MLP_H * 2));
+    HIPCHK(hipMalloc((void**)&m->Y, (size_t)rows * MLP_OUT * 4));
+    HIPCHK(hipMalloc((void**)&m->part, (size_t)nblk * (MLP_NPARAM + 1) * 4));
+    m->cap_batch = rows;
+    return YC_OK;
+}
+
+// forward + backward (+ Adam when lr > 0) on rows idx_dev[0..rows) of the dataset
+int mlp_batch(yc_mlp* m, const int* idx_dev, int rows, float lr) {
+    int nblk = (rows + MLP_TILE - 1) / MLP_TILE;
+    MlpFwdArgs f;
+    f.X = m->X; f.idx = idx_dev; f.rows = rows; f.in_mean = m->in_mean; f.in_istd = m->in_istd; f.w = mlp_w(m);
+    f.Y = m->Y; f.sX0 = m->sX0; f.sZ = m->sZ; f.sH = m->sH;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk), dim3(256), 0, m->stream, f);
+    MlpBwdArgs b;
+    b.rows = rows; b.sX0 = m->sX0; b.sZ = m->sZ; b.sH = m->sH; b.Y = m->Y; b.T = m->T; b.M = m->M; b.idx = idx_dev;
+    b.w = mlp_w(m); b.part = m->part; b.inv_count = 1.0f / (float)(rows * MLP_OUT);
+    hipLaunchKernelGGL(k_mlp_bwd, dim3(nblk), dim3(256), 0, m->stream, b);
+    float b1t = 1.0f, b2t = 1.0f;
+    if (lr > 0.0f) {
+        m->adam_t += 1;
+        b1t = std::pow(0.9f, (float)m->adam_t);
+        b2t = std::pow(0.999f, (float)m->adam_t);
+    }
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((MLP_NPARAM + 1 + 255) / 256), dim3(256), 0, m->stream, (const float*)m->part, nblk,
+                       m->grad, m->param, m->m1, m->m2, lr, b1t, b2t);
+    HIPCHK(hipGetLastError());
+    if (lr > 0.0f) return mlp_refresh_bf16(m);
+    return YC_OK;
+}
+
+int warm_guess(yc_ctx* c, yc_mlp* m) {
+    if (m->capN < c->N) {
+        if (m->F) (void)hipFree(m->F);
+        if (m->Yi) (void)hipFree(m->Yi);
+        HIPCHK(hipMalloc((void**)&m->F, (size_t)c->N * MLP_IN * 4));
+        HIPCHK(hipMalloc((void**)&m->Yi, (size_t)c->N * MLP_OUT * 4));
+        m->capN = c->N;
+    }
+    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->vpad, c->Av), (double*)nullptr,
+                       (const CGScal*)c->sc, 0);
+    FeatArgs fa;
+    fa.x = c->x; fa.v = c->v; fa.uvd = c->uvd; fa.uv = c->uv; fa.uvbar = c->uvbar; fa.flags = c->flags;
+    fa.b = c->b; fa.invd = c->invd; fa.Av = c->Av; fa.dprev = c->dprev;
+    fa.W = c->W; fa.Lg = c->Lg; fa.N = c->N; fa.Wp = c->Wp; fa.P = c->P; fa.L = c->d.L; fa.F = m->F;
+    hipLaunchKernelGGL(k_features, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, fa);
+    MlpFwdArgs f;
+    f.X = m->F; f.idx = nullptr; f.rows = c->N; f.in_mean = m->in_mean; f.in_istd = m->in_istd; f.w = mlp_w(m);
+    f.Y = m->Yi; f.sX0 = nullptr; f.sZ = nullptr; f.sH = nullptr;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3((c->N + MLP_TILE - 1) / MLP_TILE), dim3(256), 0, c->stream, f);
+    hipLaunchKernelGGL(k_warm_x0, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->vpad, (const float*)m->Yi,
+                       (const float*)m->out_std, c->W, c->Lg, c->N, c->Wp, c->P, c->cx);
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yc_mlp_param_count(void) { return MLP_NPARAM; }
+
+int yc_mlp_create(yc_mlp** out) {
+    if (!out) return fail(YC_ERR_ARG, "null argument");
+    yc_mlp* m = new yc_mlp();
+    std::memset((void*)m, 0, sizeof(yc_mlp));
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) { delete m; return fail(YC_ERR_HIP, "stream"); }
+    int rc = YC_OK;
+    auto chk = [&](int r) { if (r && !rc) rc = r; };
+    chk(dalloc(&m->param, MLP_NPARAM + 1)); chk(dalloc(&m->m1, MLP_NPARAM + 1)); chk(dalloc(&m->m2, MLP_NPARAM + 1));
+    chk(dalloc(&m->grad, MLP_NPARAM + 1));
+    chk(dalloc(&m->W1, MLP_H * MLP_IN)); chk(dalloc(&m->W2, MLP_H * MLP_H)); chk(dalloc(&m->W3, MLP_H * MLP_H));
+    chk(dalloc(&m->W4, MLP_OUTP * MLP_H));
+    chk(dalloc(&m->in_mean, MLP_IN)); chk(dalloc(&m->in_istd, MLP_IN)); chk(dalloc(&m->out_std, MLP_OUT));
+    if (rc) { yc_mlp_destroy(m); return rc; }
+    std::vector<float> one(MLP_IN, 1.0f);
+    if (hipMemcpy(m->in_istd, one.data(), MLP_IN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->out_std, one.data(), MLP_OUT * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        yc_mlp_destroy(m);
+        return fail(YC_ERR_HIP, "upload");
+    }
+    *out = m;
+    return YC_OK;
+}
+
+void yc_mlp_destroy(yc_mlp* m) {
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    void* bufs[] = {m->param, m->m1, m->m2, m->grad, m->W1, m->W2, m->W3, m->W4, m->in_mean, m->in_istd, m->out_std,
+                    m->X, m->T, m->M, m->idx, m->sX0, m->sZ, m->sH, m->Y, m->part, m->F, m->Yi};
+    for (void* p : bufs) if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int yc_mlp_set_params(yc_mlp* m, const float* p) {
+    if (!m || !p) return fail(YC_ERR_ARG, "null argument");
+    HIPCHK(hipMemcpyAsync(m->param, p, MLP_NPARAM * 4, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemsetAsync(m->m1, 0, (MLP_NPARAM + 1) * 4, m->stream));
+    HIPCHK(hipMemsetAsync(m->m2, 0, (MLP_NPARAM + 1) * 4, m->stream));
+    m->adam_t = 0;
+    int rc = mlp_refresh_bf16(m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return YC_OK;
+}
+
+int yc_mlp_get_params(yc_mlp* m, float* p) {
+    if (!m || !p) return fail(YC_ERR_ARG, "null argument");
+    HIPCHK(hipMemcpyAsync(p, m->param, MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return YC_OK;
+}
+
+int yc_mlp_set_norm(yc_mlp* m, const float* in_mean, const float* in_std, const float* out_std) {
+    if (!m || !in_mean || !in_std || !out_std) return fail(YC_ERR_ARG, "null argument");
+    float istd[MLP_IN];
+    for (int i = 0; i < MLP_IN; ++i) istd[i] = in_std[i] > 0.0f ? 1.0f / in_std[i] : 0.0f;
+    HIPCHK(hipMemcpyAsync(m->in_mean, in_mean, MLP_IN * 4, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->in_istd, istd, MLP_IN * 4, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->out_std, out_std, MLP_OUT * 4, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return YC_OK;
+}
+
+int yc_mlp_set_data(yc_mlp* m, const float* X, const float* T, const float* M, int rows) {
+    if (!m || !X || !T || !M || rows <= 0) return fail(YC_ERR_ARG, "bad argument");
+    if (m->X) (void)hipFree(m->X);
+    if (m->T) (void)hipFree(m->T);
+    if (m->M) (void)hipFree(m->M);
+    m->X = m->T = m->M = nullptr;
+    HIPCHK(hipMalloc((void**)&m->X, (size_t)rows * MLP_IN * 4));
+    HIPCHK(hipMalloc((void**)&m->T, (size_t)rows * MLP_OUT * 4));
+    HIPCHK(hipMalloc((void**)&m->M, (size_t)rows * MLP_OUT * 4));
+    HIPCHK(hipMemcpyAsync(m->X, X, (size_t)rows * MLP_IN * 4, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->T, T, (size_t)rows * MLP_OUT * 4, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipMemcpyAsync(m->M, M, (size_t)rows * MLP_OUT * 4, hipMemcpyHostToDevice, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    m->nrows = rows;
+    return YC_OK;
+}
+
+int yc_mlp_forward(yc_mlp* m, const float* X, int rows, float* Y) {
+    if (!m || !X || !Y || rows <= 0) return fail(YC_ERR_ARG, "bad argument");
+    float *dX = nullptr, *dY = nullptr;
+    HIPCHK(hipMalloc((void**)&dX, (size_t)rows * MLP_IN * 4));
+    HIPCHK(hipMalloc((void**)&dY, (size_t)rows * MLP_OUT * 4));
+    HIPCHK(hipMemcpyAsync(dX, X, (size_t)rows * MLP_IN * 4, hipMemcpyHostToDevice, m->stream));
+    MlpFwdArgs f;
+    f.X = dX; f.idx = nullptr; f.rows = rows; f.in_mean = m->in_mean; f.in_istd = m->in_istd; f.w = mlp_w(m);
+    f.Y = dY; f.sX0 = nullptr; f.sZ = nullptr; f.sH = nullptr;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3((rows + MLP_TILE - 1) / MLP_TILE), dim3(256), 0, m->stream, f);
+    HIPCHK(hipMemcpyAsync(Y, dY, (size_t)rows * MLP_OUT * 4, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    return YC_OK;
+}
+
+int yc_mlp_grad(yc_mlp* m, const int* idx, int rows, float* grad, float* loss) {
+    if (!m || !idx || rows <= 0 || !m->X) return fail(YC_ERR_ARG, "bad argument (dataset set?)");
+    for (int i = 0; i < rows; ++i) if (idx[i] < 0 || idx[i] >= m->nrows) return fail(YC_ERR_ARG, "row index out of range");
+    int rc = mlp_reserve_batch(m, rows);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(m->idx, idx, (size_t)rows * 4, hipMemcpyHostToDevice, m->stream));
+    rc = mlp_batch(m, m->idx, rows, 0.0f);
+    if (rc) return rc;
+    std::vector<float> g(MLP_NPARAM + 1);
+    HIPCHK(hipMemcpyAsync(g.data(), m->grad, (MLP_NPARAM + 1) * 4, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (grad) std::memcpy(grad, g.data(), MLP_NPARAM * 4);
+    if (loss) *loss = g[MLP_NPARAM];
+    return YC_OK;
+}
+
+int yc_mlp_train(yc_mlp* m, const int* idx, int rows, int batch, float lr, float* mean_loss) {
+    if (!m || !idx || rows <= 0 || batch <= 0 || !m->X) return fail(YC_ERR_ARG, "bad argument (dataset set?)");
+    for (int i = 0; i < rows; ++i) if (idx[i] < 0 || idx[i] >= m->nrows) return fail(YC_ERR_ARG, "row index out of range");
+    int rc = mlp_reserve_batch(m, std::max(batch, 1));
+    if (rc) return rc;
+    int* didx = nullptr;
+    HIPCHK(hipMalloc((void**)&didx, (size_t)rows * 4));
+    HIPCHK(hipMemcpyAsync(didx, idx, (size_t)rows * 4, hipMemcpyHostToDevice, m->stream));
+    double lsum = 0.0;
+    int nb = 0;
+    for (int s = 0; s + batch <= rows; s += batch) {
+        rc = mlp_batch(m, didx + s, batch, lr);
+        if (rc) break;
+        float l = 0.0f;
+        HIPCHK(hipMemcpyAsync(&l, m->grad + MLP_NPARAM, 4, hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+        lsum += l;
+        ++nb;
+    }
+    (void)hipFree(didx);
+    if (mean_loss) *mean_loss = nb ? (float)(lsum / nb) : 0.0f;
+    return rc;
+}
+
+int yc_features(yc_ctx* c, float* out) {
+    if (!c || !out) return fail(YC_ERR_ARG, "null argument");
+    if (!c->assembled) return fail(YC_ERR_ARG, "features before assemble");
+    float* F = nullptr;
+    HIPCHK(hipMalloc((void**)&F, (size_t)c->N * MLP_IN * 4));
+    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->vpad, c->Av), (double*)nullptr,
+                       (const CGScal*)c->sc, 0);
+    FeatArgs fa;
+    fa.x = c->x; fa.v = c->v; fa.uvd = c->uvd; fa.uv = c->uv; fa.uvbar = c->uvbar; fa.flags = c->flags;
+    fa.b = c->b; fa.invd = c->invd; fa.Av = c->Av; fa.dprev = c->dprev;
+    fa.W = c->W; fa.Lg = c->Lg; fa.N = c->N; fa.Wp = c->Wp; fa.P = c->P; fa.L = c->d.L; fa.F = F;
+    hipLaunchKernelGGL(k_features, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, fa);
+    HIPCHK(hipMemcpyAsync(out, F, (size_t)c->N * MLP_IN * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(F);
+    return YC_OK;
+}
+
+int yc_targets(yc_ctx* c, float* out) {
+    if (!c || !out) return fail(YC_ERR_ARG, "null argument");
+    if (!c->solved) return fail(YC_ERR_ARG, "targets before solve");
+    size_t PV = (size_t)5 * c->P;
+    std::vector<double> x(PV), v(PV);
+    HIPCHK(hipMemcpyAsync(x.data(), c->cx, PV * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(v.data(), c->vpad, PV * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int vv = 0; vv < c->Lg; ++vv)
+        for (int u = 0; u < c->W; ++u) {
+            int k = u + vv * c->W, pk = (u + 2) + (vv + 2) * c->Wp;
+            for (int q = 0; q < 5; ++q) out[(size_t)k * 5 + q] = (float)(x[(size_t)q * c->P + pk] - v[(size_t)q * c->P + pk]);
+        }
+    return YC_OK;
+}
+
+int yc_step_warm(yc_ctx* c, yc_mlp* m, double h, double tol, int maxit, yc_step_info* info) {
+    if (!c || !m) return fail(YC_ERR_ARG, "null argument");
+    g_err.clear();
+    int it = 0;
+    double rr = 0;
+    int rc = do_assemble(c, h);
+    if (!rc) rc = warm_guess(c, m);
+    if (!rc) rc = do_solve(c, nullptr, tol, maxit, &it, &rr, true);
+    if (info) { info->cg_iters = it; info->cg_relres = rr; info->status = rc; }
+    if (rc == YC_ERR_NONFINITE || rc == YC_ERR_HIP || rc == YC_ERR_ARG) return rc;
+    int rc2 = do_update(c, h);
+    if (rc2) return rc2;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+int yc_run_warm(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters) {
+    if (!c || !m) return fail(YC_ERR_ARG, "null argument");
+    long long tot = 0;
+    int worst = YC_OK;
+    for (int s = 0; s < steps; ++s) {
+        yc_step_info info;
+        int rc = yc_step_warm(c, m, h, tol, maxit, &info);
+        tot += info.cg_iters;
+        if (rc == YC_ERR_NOT_CONVERGED) worst = rc;
+        else if (rc) { if (total_iters) *total_iters = tot; return rc; }
+    }
+    if (total_iters) *total_iters = tot;
+    return worst;
 }
 
 }  // extern "C"

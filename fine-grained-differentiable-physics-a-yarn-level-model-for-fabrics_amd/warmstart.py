@@ -1,0 +1,168 @@
+"""Learned CG warm start: dataset generation from the simulator's own solves, GPU
+training with the library's BF16-MFMA kernels, and the network wrapper.
+
+The network (32-128-128-128-5, SiLU) maps per-node features to the velocity change
+of the implicit step; CG starts from v + dv.  Plain zero-guess CG (Cloth.step)
+remains the fallback and the judge: the warm-started solve stops at the same
+relative residual.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from . import cloth_spec as cs
+
+FEATURE_NAMES = (["v_x", "v_y", "v_z", "uvd_u", "uvd_v"] + ["jacobi_dv_%d" % i for i in range(5)] +
+                 ["dv_prev_%d" % i for i in range(5)] + ["slide_u", "slide_v"] +
+                 ["nb_%s_%s" % (n, c) for n in ("left", "right", "top", "bottom") for c in "xyz"] +
+                 ["interior", "handle", "flip"])
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
+
+_fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+_ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))  # noqa: E731
+
+
+def init_params(seed):
+    """Seeded initial parameters (uniform +-1/sqrt(fan_in), zero biases)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fin, fout in ((32, 128), (128, 128), (128, 128), (128, 5)):
+        lim = 1.0 / np.sqrt(fin)
+        parts.append(rng.uniform(-lim, lim, (fout, fin)).astype(np.float32).ravel())
+        parts.append(np.zeros(fout, np.float32))
+    return np.concatenate(parts)
+
+
+class WarmStartMLP:
+    def __init__(self, params=None, seed=0):
+        L = _abi.load_library()
+        h = ctypes.c_void_p()
+        _abi.check(L.yc_mlp_create(ctypes.byref(h)))
+        self._h = h
+        self.nparam = L.yc_mlp_param_count()
+        self.in_mean = np.zeros(32, np.float32)
+        self.in_std = np.ones(32, np.float32)
+        self.out_std = np.ones(5, np.float32)
+        self.set_params(init_params(seed) if params is None else params)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.load_library().yc_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        _abi.check(_abi.load_library().yc_mlp_set_params(self._h, _fp(p)))
+
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        _abi.check(_abi.load_library().yc_mlp_get_params(self._h, _fp(p)))
+        return p
+
+    def set_norm(self, in_mean, in_std, out_std):
+        self.in_mean = np.ascontiguousarray(in_mean, dtype=np.float32)
+        self.in_std = np.ascontiguousarray(in_std, dtype=np.float32)
+        self.out_std = np.ascontiguousarray(out_std, dtype=np.float32)
+        _abi.check(_abi.load_library().yc_mlp_set_norm(self._h, _fp(self.in_mean), _fp(self.in_std), _fp(self.out_std)))
+
+    def set_data(self, X, T, M):
+        self._X = np.ascontiguousarray(X, dtype=np.float32)
+        self._T = np.ascontiguousarray(T, dtype=np.float32)
+        self._M = np.ascontiguousarray(M, dtype=np.float32)
+        _abi.check(_abi.load_library().yc_mlp_set_data(self._h, _fp(self._X), _fp(self._T), _fp(self._M), len(self._X)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), 5), np.float32)
+        _abi.check(_abi.load_library().yc_mlp_forward(self._h, _fp(X), len(X), _fp(Y)))
+        return Y
+
+    def grad(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        g = np.zeros(self.nparam, np.float32)
+        loss = ctypes.c_float()
+        _abi.check(_abi.load_library().yc_mlp_grad(self._h, _ip(idx), len(idx), _fp(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train_epoch(self, idx, batch, lr):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = ctypes.c_float()
+        _abi.check(_abi.load_library().yc_mlp_train(self._h, _ip(idx), len(idx), int(batch), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def save(self, path):
+        np.savez(path, params=self.params(), in_mean=self.in_mean, in_std=self.in_std, out_std=self.out_std)
+
+    @classmethod
+    def load(cls, path):
+        d = np.load(path)
+        m = cls(params=d["params"])
+        m.set_norm(d["in_mean"], d["in_std"], d["out_std"])
+        return m
+
+
+def generate(cloth_specs, steps, skip=5, tol=1e-10, node_stride=1):
+    """Run the simulator and collect (features, dv, mask) for every node of every step
+    after the first `skip` steps: the (state, converged dv) pairs of the simulator's own
+    solves.  node_stride subsamples nodes."""
+    from .cloth import Cloth
+    Xs, Ts, Ms = [], [], []
+    for spec in cloth_specs:
+        c = Cloth(spec)
+        c.set_ClothState(cs.initial_state(spec))
+        mask = c.node_mask()
+        h = spec["h"]
+        for s in range(steps):
+            c.assemble(h)
+            X = c.features()
+            it, rr, ok = c.solve(tol=tol, maxit=20000)
+            if not ok:
+                raise RuntimeError("data generation: CG did not converge")
+            T = c.targets()
+            c.update(h)
+            if s >= skip:
+                Xs.append(X[::node_stride]); Ts.append(T[::node_stride]); Ms.append(mask[::node_stride])
+        c.close()
+    return np.concatenate(Xs), np.concatenate(Ts), np.concatenate(Ms)
+
+
+def training_cloths(n=6, seed=1000):
+    """Seeded cloths of 32x32 to 96x96 crossings, all three weaves, seeded wind."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        W = int(rng.integers(32, 97))
+        spec = cs.heter_spec(W, ("plain", "twill", "satin")[i % 3], seed + i)
+        spec["wind_vel"] = [0.0, float(rng.uniform(2.0, 8.0)), 0.0]
+        out.append(spec)
+    return out
+
+
+def fit(X, T, M, epochs=12, batch=4096, lr=2e-3, seed=0, log=None):
+    """Standardise, train on the GPU, return the network (with its normalisation)."""
+    in_mean = X.mean(0)
+    in_std = X.std(0)
+    in_std[in_std == 0] = 1.0
+    out_std = np.sqrt((T ** 2 * M).sum(0) / np.maximum(M.sum(0), 1.0)).astype(np.float32)
+    out_std[out_std == 0] = 1.0
+    net = WarmStartMLP(seed=seed)
+    net.set_norm(in_mean, in_std, out_std)
+    net.set_data(X, T / out_std, M)
+    rng = np.random.default_rng(seed)
+    losses = []
+    for ep in range(epochs):
+        lr_e = lr * 0.5 * (1.0 + np.cos(np.pi * ep / epochs))
+        loss = net.train_epoch(rng.permutation(len(X)).astype(np.int32), batch, lr_e)
+        losses.append(loss)
+        if log:
+            log("epoch %d loss %.4e" % (ep, loss))
+    return net, losses

@@ -118,12 +118,52 @@ int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_s
  * total CG iterations.  Host synchronises once per CG chunk only. */
 int yc_run(yc_ctx* ctx, double h, int steps, double tol, int maxit, long long* total_iters);
 
-/* Times `reps` launches of the stencil SpMV kernel on the context's stream with HIP
- * events; avg_ms receives the mean duration of one launch. */
+/* Times `reps` launches of the CG loop's stencil product y = A p on the context's
+ * stream with HIP events; avg_ms receives the mean duration of one launch.  Call after
+ * yc_assemble; it overwrites the CG workspace. */
 int yc_time_spmv(yc_ctx* ctx, int reps, double* avg_ms);
 
-/* Algorithmic HBM bytes of one SpMV launch (stencil values + input and output vectors). */
+/* Algorithmic HBM bytes of one product launch: 8 bytes per structural nonzero of A
+ * plus 16 bytes per DOF (p read, A p written). */
 double yc_spmv_bytes(const yc_ctx* ctx);
+
+/* ---------------------------------------------------------------------------
+ * Learned CG warm start.  No counterpart in the reference (whose only network,
+ * src/ForceNet.h, predicts control forces): a 32-128-128-128-5 MLP with BF16 weights
+ * and FP32 accumulation on MFMA predicts the velocity change of the implicit step
+ * per node; CG then starts from v + dv and refines to the same tolerance as from a
+ * zero guess (yc_step stays the zero-guess fallback and judge).
+ * Parameters: one flat FP32 array of yc_mlp_param_count() values in the order
+ * W1[128x32] b1[128] W2[128x128] b2[128] W3[128x128] b3[128] W4[5x128] b4[5]
+ * (row-major [out][in]).  Inputs are standardised with (x - in_mean) / in_std; the
+ * outputs are in units of out_std.
+ * ------------------------------------------------------------------------- */
+typedef struct yc_mlp yc_mlp;
+
+int yc_mlp_param_count(void);
+int yc_mlp_create(yc_mlp** out);
+void yc_mlp_destroy(yc_mlp* m);
+int yc_mlp_set_params(yc_mlp* m, const float* params);   /* also resets the Adam state */
+int yc_mlp_get_params(yc_mlp* m, float* params);
+int yc_mlp_set_norm(yc_mlp* m, const float* in_mean /*32*/, const float* in_std /*32*/, const float* out_std /*5*/);
+/* Training set on the device: X rows x 32 raw features, T rows x 5 targets in units
+ * of out_std, M rows x 5 masks (0 for DOFs a node does not have). */
+int yc_mlp_set_data(yc_mlp* m, const float* X, const float* T, const float* M, int rows);
+/* Inference on host features: Y rows x 5 (units of out_std). */
+int yc_mlp_forward(yc_mlp* m, const float* X, int rows, float* Y);
+/* Loss (mean of the masked squared error) and its gradient on dataset rows idx[0..rows). */
+int yc_mlp_grad(yc_mlp* m, const int* idx, int rows, float* grad, float* loss);
+/* Adam over idx[0..rows) in batches of `batch` rows (remainder dropped); mean batch loss out. */
+int yc_mlp_train(yc_mlp* m, const int* idx, int rows, int batch, float lr, float* mean_loss);
+
+/* Network features of every node (N x 32 floats) after yc_assemble, and the velocity
+ * change of the last solve (N x 5) after yc_solve: the (input, target) pairs the
+ * simulator produces for training. */
+int yc_features(yc_ctx* ctx, float* out);
+int yc_targets(yc_ctx* ctx, float* out);
+/* SimStep with the network's warm start (x0 = v + dv on the device). */
+int yc_step_warm(yc_ctx* ctx, yc_mlp* m, double h, double tol, int maxit, yc_step_info* info);
+int yc_run_warm(yc_ctx* ctx, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters);
 
 #ifdef __cplusplus
 }
