@@ -1,9 +1,11 @@
-"""Benchmark: implicit yarn-level steps/s on one MI355X.
+"""Benchmark: implicit yarn-level steps/s on one MI355X, learned warm start vs plain CG.
 
 Workload: the 128x128 two-yarn twill cloth with seeded wind (cloth_spec.workload
-"twill128"), h = 1 ms, Jacobi-PCG to a 1e-8 relative residual from a zero guess,
-the reference's step (assemble, solve, update) per step, FP64.  Inputs live in HBM
-before the timed region.  Prints one JSON line.
+"twill128"), h = 1 ms, FP64, the reference's step (assemble, solve, update) per step.
+Two runs from the same initial state: plain zero-guess Jacobi-PCG, and CG started
+from the learned warm start; both stop at the same 1e-8 relative residual.  `value`
+is the warm-started steps/s; the plain rate and the ratio are in `config`.  Inputs
+live in HBM before the timed region.  Prints one JSON line.
 """
 import argparse
 import json
@@ -15,6 +17,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_NODE = 2 * (32 * 128 + 128 * 128 * 2 + 128 * 5)   # 75,008
 
 
 def cpu_baseline(cs, name, nsteps):
@@ -47,24 +51,45 @@ def main():
         sys.exit(2)
     import __graft_entry__
     ys = __graft_entry__.load_package()
+    import yarnstep.warmstart as ws
     cs = ys.cloth_spec
     spec, state, ext, _ = cs.workload(args.workload)
     h = spec["h"]
-    cloth = ys.Cloth(spec)
-    cloth.set_ClothState(state)
-    cloth.set_ext_force(ext)
     tol = 1e-8
-    it_w, rc = cloth.run(h, args.warmup, tol=tol)
-    t0 = time.perf_counter()
-    iters, rc = cloth.run(h, args.steps, tol=tol)
-    dt = time.perf_counter() - t0
-    if rc != 0:
-        print("warning: a CG solve stopped at maxit above tolerance", file=sys.stderr)
-    steps_per_s = args.steps / dt
-    # dominant kernel: stencil SpMV, timed with HIP events on the context stream
+    import os
+    if os.path.exists(ws.WEIGHTS):
+        net = ws.WarmStartMLP.load(ws.WEIGHTS)
+    else:   # train on the GPU before the timed region
+        X, T, M = ws.generate(ws.training_cloths(6), steps=40, skip=5)
+        net, _ = ws.fit(X, T, M, epochs=16)
+
+    def timed(warm):
+        cloth = ys.Cloth(spec)
+        cloth.set_ClothState(state)
+        cloth.set_ext_force(ext)
+        run = (lambda n: cloth.run_warm(net, h, n, tol=tol)) if warm else (lambda n: cloth.run(h, n, tol=tol))
+        run(args.warmup)
+        t0 = time.perf_counter()
+        iters, rc = run(args.steps)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            print("warning: a CG solve stopped at maxit above tolerance", file=sys.stderr)
+        return cloth, dt, iters
+
+    cloth, dt_plain, it_plain = timed(False)
+    # dominant kernel: stencil SpMV of the CG loop, timed with HIP events on the context stream
     spmv_ms = cloth.time_spmv(200)
     spmv_bytes = cloth.spmv_bytes()
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    cloth.close()
+    cloth, dt_warm, it_warm = timed(True)
+    cloth.close()
+    mlp_ms = net.time_forward(spec["width"] * spec["length"], 100)
+    mlp_tflops = MLP_FLOP_PER_NODE * spec["width"] * spec["length"] / (mlp_ms * 1e-3) / 1e12
+    steps_per_s = args.steps / dt_warm
+    plain_steps_per_s = args.steps / dt_plain
+    dt = dt_warm
+    iters = it_warm
     out = {
         "metric": "implicit steps/s at 128x128 crossings: warm-start vs plain CG, same residual",
         "value": steps_per_s,
@@ -78,9 +103,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "%s: %dx%d crossings, %s weave, %d yarns, h=%g, plain zero-guess Jacobi-PCG to %g" % (
+        "config": {"workload": "%s: %dx%d crossings, %s weave, %d yarns, seeded wind, h=%g, Jacobi-PCG to %g relative residual" % (
             args.workload, spec["width"], spec["length"], spec["pattern"], len(spec["yarns"]), h, tol),
-            "solver": "plain-cg", "cg_iters_per_step": iters / args.steps},
+            "solver": "learned warm start (32-128-128-128-5 BF16 MLP) + Jacobi-PCG",
+            "cg_iters_per_step": it_warm / args.steps,
+            "plain_cg_steps_per_s": plain_steps_per_s, "plain_cg_ms_per_step": 1e3 * dt_plain / args.steps,
+            "plain_cg_iters_per_step": it_plain / args.steps,
+            "warm_over_plain": steps_per_s / plain_steps_per_s,
+            "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": mlp_tflops / BF16_PEAK_TFLOPS, "avg_launch_ms": mlp_ms, "rows": spec["width"] * spec["length"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_spmv", "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms},

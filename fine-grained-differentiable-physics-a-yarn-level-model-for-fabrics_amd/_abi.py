@@ -48,7 +48,7 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_export_normals", "yc_solve", "yc_update", "yc_step", "yc_run", "yc_time_spmv", "yc_spmv_bytes",
            "yc_mlp_param_count", "yc_mlp_create", "yc_mlp_destroy", "yc_mlp_set_params", "yc_mlp_get_params",
            "yc_mlp_set_norm", "yc_mlp_set_data", "yc_mlp_forward", "yc_mlp_grad", "yc_mlp_train", "yc_features",
-           "yc_targets", "yc_step_warm", "yc_run_warm"]
+           "yc_targets", "yc_step_warm", "yc_run_warm", "yc_mlp_time_forward"]
 
 
 def lib_path():
@@ -102,6 +102,7 @@ def load_library():
     L.yc_mlp_forward.argtypes = [vp, fp, ctypes.c_int, fp]
     L.yc_mlp_grad.argtypes = [vp, ip, ctypes.c_int, fp, fp]
     L.yc_mlp_train.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp]
+    L.yc_mlp_time_forward.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.yc_features.argtypes = [vp, fp]
     L.yc_targets.argtypes = [vp, fp]
     L.yc_step_warm.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]

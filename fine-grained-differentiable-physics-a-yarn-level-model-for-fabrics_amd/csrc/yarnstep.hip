@@ -1360,6 +1360,34 @@ int yc_mlp_train(yc_mlp* m, const int* idx, int rows, int batch, float lr, float
     return rc;
 }
 
+int yc_mlp_time_forward(yc_mlp* m, int rows, int reps, double* avg_ms) {
+    if (!m || rows <= 0 || reps <= 0 || !avg_ms) return fail(YC_ERR_ARG, "bad argument");
+    float *dX = nullptr, *dY = nullptr;
+    HIPCHK(hipMalloc((void**)&dX, (size_t)rows * MLP_IN * 4));
+    HIPCHK(hipMalloc((void**)&dY, (size_t)rows * MLP_OUT * 4));
+    HIPCHK(hipMemsetAsync(dX, 0, (size_t)rows * MLP_IN * 4, m->stream));
+    MlpFwdArgs f;
+    f.X = dX; f.idx = nullptr; f.rows = rows; f.in_mean = m->in_mean; f.in_istd = m->in_istd; f.w = mlp_w(m);
+    f.Y = dY; f.sX0 = nullptr; f.sZ = nullptr; f.sH = nullptr;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int nblk = (rows + MLP_TILE - 1) / MLP_TILE;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk), dim3(256), 0, m->stream, f);
+    HIPCHK(hipEventRecord(e0, m->stream));
+    for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk), dim3(256), 0, m->stream, f);
+    HIPCHK(hipEventRecord(e1, m->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(dX);
+    (void)hipFree(dY);
+    *avg_ms = ms / reps;
+    return YC_OK;
+}
+
 int yc_features(yc_ctx* c, float* out) {
     if (!c || !out) return fail(YC_ERR_ARG, "null argument");
     if (!c->assembled) return fail(YC_ERR_ARG, "features before assemble");

@@ -99,6 +99,11 @@ class WarmStartMLP:
         _abi.check(_abi.load_library().yc_mlp_train(self._h, _ip(idx), len(idx), int(batch), float(lr), ctypes.byref(loss)))
         return loss.value
 
+    def time_forward(self, rows, reps=100):
+        ms = ctypes.c_double()
+        _abi.check(_abi.load_library().yc_mlp_time_forward(self._h, int(rows), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def save(self, path):
         np.savez(path, params=self.params(), in_mean=self.in_mean, in_std=self.in_std, out_std=self.out_std)
 

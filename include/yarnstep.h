@@ -156,6 +156,10 @@ int yc_mlp_grad(yc_mlp* m, const int* idx, int rows, float* grad, float* loss);
 /* Adam over idx[0..rows) in batches of `batch` rows (remainder dropped); mean batch loss out. */
 int yc_mlp_train(yc_mlp* m, const int* idx, int rows, int batch, float lr, float* mean_loss);
 
+/* Times `reps` launches of the network's forward kernel on `rows` rows with HIP events
+ * on the network's stream; avg_ms receives the mean duration of one launch. */
+int yc_mlp_time_forward(yc_mlp* m, int rows, int reps, double* avg_ms);
+
 /* Network features of every node (N x 32 floats) after yc_assemble, and the velocity
  * change of the last solve (N x 5) after yc_solve: the (input, target) pairs the
  * simulator produces for training. */
