@@ -37,6 +37,18 @@ def cpu_baseline(cs, name, nsteps):
             "sample": "steps 2..%d of the %s workload, oracle single-thread FP64 CG to 1e-8" % (nsteps + 1, name)}
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE runs, scaled by the copy calibration), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_pmc.md" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,19 +57,32 @@ def main():
     ap.add_argument("--workload", default="twill128")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="time only the zero-guess CG path (profiling runs)")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="first run 5 launches of a 256 MiB -> 256 MiB FP64 negation, the known-size kernel "
+                         "that scales FETCH_SIZE / WRITE_SIZE counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures a single MI355X (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import __graft_entry__
     ys = __graft_entry__.load_package()
+    if args.copy_calibrate:
+        import torch
+        x = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        del x, y
     import yarnstep.warmstart as ws
     cs = ys.cloth_spec
     spec, state, ext, _ = cs.workload(args.workload)
     h = spec["h"]
     tol = 1e-8
-    import os
-    if os.path.exists(ws.WEIGHTS):
+    if args.plain_only:
+        net = None
+    elif os.path.exists(ws.WEIGHTS):
         net = ws.WarmStartMLP.load(ws.WEIGHTS)
     else:   # train on the GPU before the timed region
         X, T, M = ws.generate(ws.training_cloths(6), steps=40, skip=5)
@@ -82,10 +107,13 @@ def main():
     spmv_bytes = cloth.spmv_bytes()
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     cloth.close()
-    cloth, dt_warm, it_warm = timed(True)
-    cloth.close()
-    mlp_ms = net.time_forward(spec["width"] * spec["length"], 100)
-    mlp_tflops = MLP_FLOP_PER_NODE * spec["width"] * spec["length"] / (mlp_ms * 1e-3) / 1e12
+    if net is not None:
+        cloth, dt_warm, it_warm = timed(True)
+        cloth.close()
+        mlp_ms = net.time_forward(spec["width"] * spec["length"], 100)
+    else:
+        dt_warm, it_warm, mlp_ms = dt_plain, it_plain, None
+    mlp_tflops = MLP_FLOP_PER_NODE * spec["width"] * spec["length"] / (mlp_ms * 1e-3) / 1e12 if net else None
     steps_per_s = args.steps / dt_warm
     plain_steps_per_s = args.steps / dt_plain
     dt = dt_warm
@@ -111,9 +139,10 @@ def main():
             "plain_cg_iters_per_step": it_plain / args.steps,
             "warm_over_plain": steps_per_s / plain_steps_per_s,
             "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": mlp_tflops / BF16_PEAK_TFLOPS, "avg_launch_ms": mlp_ms, "rows": spec["width"] * spec["length"]}},
+                    "frac": mlp_tflops / BF16_PEAK_TFLOPS, "avg_launch_ms": mlp_ms,
+                    "rows": spec["width"] * spec["length"]} if net else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_spmv", args.workload),
                      "kernel": "k_spmv", "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms},
     }
     if not args.no_cpu:

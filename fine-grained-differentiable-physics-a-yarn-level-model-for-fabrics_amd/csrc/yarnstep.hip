@@ -43,6 +43,30 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr int BLK = 256;
+
+// Mean device time per launch of `reps` back-to-back launches issued by `launch` on
+// `stream` (after one untimed launch), from one event pair around the whole run.  An
+// event pair around each single launch would add the event packets' own gaps (about
+// 2 us per launch on MI355X) to a kernel of a few microseconds.
+template <class F>
+int time_launches(hipStream_t stream, int reps, double* avg_ms, F launch) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(YC_ERR_HIP, "hipEventCreate"); }
+    launch();
+    hipError_t err = hipEventRecord(e0, stream);
+    for (int i = 0; i < reps && err == hipSuccess; ++i) launch();
+    if (err == hipSuccess) err = hipGetLastError();
+    if (err == hipSuccess) err = hipEventRecord(e1, stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return fail(YC_ERR_HIP, std::string("timing: ") + hipGetErrorString(err));
+    *avg_ms = (double)ms / reps;
+    return YC_OK;
+}
 // per-node kernels (assembly, update, normals) are heavy per thread and have only one
 // thread per node: small blocks spread them over all 256 CUs
 constexpr int NBLK = 64;
@@ -1105,23 +1129,12 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
 
 int yc_time_spmv(yc_ctx* c, int reps, double* avg_ms) {
     if (!c || reps <= 0 || !avg_ms) return fail(YC_ERR_ARG, "bad argument");
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
     // the CG loop's stencil product y = A p (partials of p.y included, as in the loop)
     SpmvArgs a = make_spmv(c, c->cp, c->cy);
-    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, a, c->part.pAp, (const CGScal*)c->sc, 0);
-    HIPCHK(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < reps; ++i)
+    int st = time_launches(c->stream, reps, avg_ms, [&] {
         hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, a, c->part.pAp, (const CGScal*)c->sc, 0);
-    HIPCHK(hipEventRecord(e1, c->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *avg_ms = ms / reps;
-    return YC_OK;
+    });
+    return st;
 }
 
 double yc_spmv_bytes(const yc_ctx* c) {
@@ -1369,23 +1382,13 @@ int yc_mlp_time_forward(yc_mlp* m, int rows, int reps, double* avg_ms) {
     MlpFwdArgs f;
     f.X = dX; f.idx = nullptr; f.rows = rows; f.in_mean = m->in_mean; f.in_istd = m->in_istd; f.w = mlp_w(m);
     f.Y = dY; f.sX0 = nullptr; f.sZ = nullptr; f.sH = nullptr;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
     int nblk = (rows + MLP_TILE - 1) / MLP_TILE;
-    hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk), dim3(256), 0, m->stream, f);
-    HIPCHK(hipEventRecord(e0, m->stream));
-    for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk), dim3(256), 0, m->stream, f);
-    HIPCHK(hipEventRecord(e1, m->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    int st = time_launches(m->stream, reps, avg_ms, [&] {
+        hipLaunchKernelGGL(k_mlp_fwd, dim3(nblk), dim3(256), 0, m->stream, f);
+    });
     (void)hipFree(dX);
     (void)hipFree(dY);
-    *avg_ms = ms / reps;
-    return YC_OK;
+    return st;
 }
 
 int yc_features(yc_ctx* c, float* out) {
