@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="time only the zero-guess CG path (profiling runs)")
+    ap.add_argument("--cg", default="fused", choices=["fused", "reference"],
+                    help="CG form of the timed runs (the reference form is also timed once, zero guess)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run 5 launches of a 256 MiB -> 256 MiB FP64 negation, the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counter runs")
@@ -88,8 +90,9 @@ def main():
         X, T, M = ws.generate(ws.training_cloths(6), steps=40, skip=5)
         net, _ = ws.fit(X, T, M, epochs=16)
 
-    def timed(warm):
+    def timed(warm, variant=None):
         cloth = ys.Cloth(spec)
+        cloth.set_cg_variant(variant or args.cg)
         cloth.set_ClothState(state)
         cloth.set_ext_force(ext)
         run = (lambda n: cloth.run_warm(net, h, n, tol=tol)) if warm else (lambda n: cloth.run(h, n, tol=tol))
@@ -101,6 +104,11 @@ def main():
             print("warning: a CG solve stopped at maxit above tolerance", file=sys.stderr)
         return cloth, dt, iters
 
+    ref_form = None
+    if args.cg != "reference" and not args.plain_only:
+        c_ref, dt_ref, it_ref = timed(False, "reference")
+        c_ref.close()
+        ref_form = {"steps_per_s": args.steps / dt_ref, "cg_iters_per_step": it_ref / args.steps}
     cloth, dt_plain, it_plain = timed(False)
     # dominant kernel: stencil SpMV of the CG loop, timed with HIP events on the context stream
     spmv_ms = cloth.time_spmv(200)
@@ -133,11 +141,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "%s: %dx%d crossings, %s weave, %d yarns, seeded wind, h=%g, Jacobi-PCG to %g relative residual" % (
             args.workload, spec["width"], spec["length"], spec["pattern"], len(spec["yarns"]), h, tol),
-            "solver": "learned warm start (32-128-128-128-5 BF16 MLP) + Jacobi-PCG",
+            "solver": "learned warm start (32-128-128-128-5 BF16 MLP) + Jacobi-PCG (%s form)" % args.cg,
             "cg_iters_per_step": it_warm / args.steps,
             "plain_cg_steps_per_s": plain_steps_per_s, "plain_cg_ms_per_step": 1e3 * dt_plain / args.steps,
             "plain_cg_iters_per_step": it_plain / args.steps,
             "warm_over_plain": steps_per_s / plain_steps_per_s,
+            "plain_cg_reference_form": ref_form,
             "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": mlp_tflops / BF16_PEAK_TFLOPS, "avg_launch_ms": mlp_ms,
                     "rows": spec["width"] * spec["length"]} if net else None},

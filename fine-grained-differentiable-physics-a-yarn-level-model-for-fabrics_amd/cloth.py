@@ -177,6 +177,13 @@ class Cloth:
             _abi.check(rc)
         return tot.value, rc
 
+    # CG variants: "reference" (the reference solver's three-kernel order) or "fused"
+    # (the same recurrences with one kernel and one reduction per iteration)
+    CG_VARIANTS = {"reference": 0, "fused": 1}
+
+    def set_cg_variant(self, name):
+        _abi.check(_abi.load_library().yc_set_cg_variant(self._h, self.CG_VARIANTS[name]))
+
     def time_spmv(self, reps=200):
         ms = ctypes.c_double()
         _abi.check(_abi.load_library().yc_time_spmv(self._h, int(reps), ctypes.byref(ms)))

@@ -24,8 +24,20 @@ constexpr int SLOT_LE = 39;
 constexpr int SLOT_EU = 135;
 constexpr int SLOT_EV = 147;
 
+// Layout of the system matrix in HBM: the grid is cut into tiles of 8 x 8 nodes and a
+// tile stores its slots one after another as planes of 64 doubles (node lane
+// (v % 8) * 8 + u % 8).  A wavefront that works on one tile reads a slot as 512
+// contiguous bytes.  Slot s of node (u, v) lives at a_base(ntx, u, v) + 64 s, ntx
+// being the number of tiles along u.
+constexpr int TILE = 8;
+constexpr int TILE_N = TILE * TILE;
+__host__ __device__ inline size_t a_base(int ntx, int u, int v) {
+    return (size_t)((v / TILE) * ntx + (u / TILE)) * NSLOT * TILE_N + (size_t)((v % TILE) * TILE + (u % TILE));
+}
+
 struct ClothDev {
     int W, Lg, N, Nc;
+    int ntx;              // tiles of the system matrix along u
     int n_edges, n_bends;
     double L, R;
     double rho[2], ks[2], kb[2];
@@ -77,7 +89,7 @@ struct RowSink {
     const ClothDev* C;
     int me, mu_, mv_;
     bool inter;
-    double* A;          // slot planes, N apart
+    double* A;          // slot 0 of node "me"; slots TILE_N apart
     double coef[3];     // +1, -h^2, -h
     double F[5], Mv[5], KvTv[5];
     double KxEE[2], KvEE[2], FEpart[2];
@@ -88,7 +100,7 @@ struct RowSink {
 
     YS_DEV void init(const ClothDev* c, int k, double* a, double h) {
         C = c; me = k; mu_ = k % c->W; mv_ = k / c->W; inter = interior(*c, mu_, mv_);
-        A = a; coef[0] = 1.0; coef[1] = -(h * h); coef[2] = -h;
+        A = a + a_base(c->ntx, mu_, mv_); coef[0] = 1.0; coef[1] = -(h * h); coef[2] = -h;
         for (int i = 0; i < 5; ++i) { F[i] = 0; Mv[i] = 0; KvTv[i] = 0; }
         KxEE[0] = KxEE[1] = KvEE[0] = KvEE[1] = FEpart[0] = FEpart[1] = 0;
         SBC_L = D3(0, 0, 0); SBC_LE[0] = SBC_L; SBC_LE[1] = SBC_L;
@@ -133,7 +145,7 @@ struct RowSink {
     }
     YS_DEV void addA(int rc, int kcol, int cc, double val) {
         int s = slot(rc, kcol, cc);
-        if (s >= 0) A[(size_t)s * C->N + me] += val;
+        if (s >= 0) A[(size_t)s * TILE_N] += val;
     }
     YS_DEV double velL(int k, int c) { return C->v[c * C->N + k]; }
     YS_DEV double velE(int k, int c) { return C->uvd[c * C->N + k]; }

@@ -75,6 +75,10 @@ struct CGScal {
     double rr, bb, thr;
     int done, maxit, zero_rhs;
     int itc[2];   // completed iterations, written by parity so readers never race the writer
+    // single-reduction CG: alpha and gamma = r.u of the last iteration, by parity, and
+    // the iteration count reported at the stop
+    double alpha[2], gamma[2];
+    int fin_iters;
 };
 
 // Partial sums of the CG reductions, one slot per block of the producing kernel.  A
@@ -106,6 +110,28 @@ __device__ double block_sum(double v) {
     return s;  // valid in thread 0
 }
 
+// Several block sums in one pass (fixed order); results valid in thread 0.
+template <int M>
+__device__ void block_sum_n(double (&v)[M]) {
+    __shared__ double sh[M][BLK / 64];
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+        for (int off = 32; off > 0; off >>= 1) v[m] += __shfl_down(v[m], off, 64);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int m = 0; m < M; ++m) sh[m][wid] = v[m];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            double t = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[m][i];
+            v[m] = t;
+        }
+}
+
 // Sum of partials[0..n) in a fixed order by one block (all threads participate).
 __device__ double block_sum_partials(const double* partials, int n) {
     double s = 0.0;
@@ -126,7 +152,7 @@ __device__ __forceinline__ int ld_done(const CGScal* s) {
 // Assembly
 // ---------------------------------------------------------------------------
 struct AsmOut {
-    double* A;          // NSLOT planes of N
+    double* A;          // tiles of NSLOT planes of TILE_N (a_base)
     double* F;          // 5 planes of N
     double* b;          // padded 5-plane vector
     double* is_slide;   // 2 planes of N
@@ -143,10 +169,10 @@ __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (
 __global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double h) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= C.N) return;
-    for (int s = 0; s < NSLOT; ++s) O.A[(size_t)s * C.N + k] = 0.0;
     RowSink S;
     S.init(&C, k, O.A, h);
     int u = S.mu_, v = S.mv_;
+    for (int s = 0; s < NSLOT; ++s) S.A[(size_t)s * TILE_N] = 0.0;
     // handle (src/PhysicsCloth.cpp:232-248)
     if (C.flags[k] & 2) {
         d3 xf = D3(C.xfix[k], C.xfix[C.N + k], C.xfix[2 * C.N + k]);
@@ -242,15 +268,15 @@ __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double
     bool bad = false;
     for (int c = 0; c < 5; ++c) {
         double d = 0.0;
-        if (c < 3) d = O.A[(size_t)(c * 45 + c) * C.N + k];
-        else if (S.inter) d = O.A[(size_t)((c == 3 ? SLOT_EU : SLOT_EV) + 10) * C.N + k];
+        if (c < 3) d = S.A[(size_t)(c * 45 + c) * TILE_N];
+        else if (S.inter) d = S.A[(size_t)((c == 3 ? SLOT_EU : SLOT_EV) + 10) * TILE_N];
         double inv = 0.0;
         if (c < 3 || S.inter) inv = d != 0.0 ? 1.0 / d : 1.0;
         O.invdiag[c * O.P + pk] = inv;
         if (!isfinite(O.b[c * O.P + pk])) bad = true;
     }
     for (int s = 0; s < NSLOT; ++s)
-        if (!isfinite(O.A[(size_t)s * C.N + k])) bad = true;
+        if (!isfinite(S.A[(size_t)s * TILE_N])) bad = true;
     if (bad) atomicOr(&O.flag[1], 1);
     if (S.bad) atomicOr(&O.flag[0], 1);
 }
@@ -263,93 +289,84 @@ __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double
 namespace {
 
 // ---------------------------------------------------------------------------
-// Stencil SpMV: 4 threads per node (rows x, y, z and the two E rows), node-fastest
+// Stencil SpMV.  One block per 8 x 8 tile, 4 wavefronts: wavefront g < 3 computes the
+// L row g of the tile's 64 nodes, wavefront 3 both E rows.  The matrix slots of a tile
+// are contiguous planes (a_base), so every slot load of a wavefront is 512 contiguous
+// bytes.
 // ---------------------------------------------------------------------------
 struct SpmvArgs {
     const double* A;
     const double* x;   // padded 5-plane
     double* y;         // padded 5-plane
-    int W, Lg, N, Wp, P;
+    int W, Lg, N, Wp, P, ntx;
 };
 
-__device__ __forceinline__ double spmv_row(const SpmvArgs& a, int k, int g) {
-    int u = k % a.W, v = k / a.W;
-    int p0 = padidx(a.Wp, u, v);
-    const double* A = a.A + k;
-    const double* x = a.x;
-    size_t N = a.N;
-    const int Wp = a.Wp, P = a.P;
-    double acc = 0.0;
+// Rows of one node of a tile: L row g (g < 3) into *y0, or both E rows (g == 3) into
+// *y0, *y1.  X(c, du, dv) returns entry c of the input vector at the node offset by
+// (du, dv).  S(i) returns slot i of the rows: slot g*45 + i for an L row, slot
+// SLOT_EU + i (i < 24: the E_u row, then the E_v row) for the E rows.
+template <class S, class X>
+__device__ __forceinline__ void stencil_rows(int g, S slot, X xin, double* y0, double* y1) {
     if (g < 3) {
-        const int off[13] = {0, -1, 1, -Wp, Wp, -2, 2, -2 * Wp, 2 * Wp, -1 - Wp, 1 - Wp, -1 + Wp, 1 + Wp};
-        const double* Ar = A + (size_t)(g * 45) * N;
+        const int odu[13] = {0, -1, 1, 0, 0, -2, 2, 0, 0, -1, 1, -1, 1};
+        const int odv[13] = {0, 0, 0, -1, 1, 0, 0, -2, 2, -1, -1, 1, 1};
+        double acc = 0.0;
 #pragma unroll
         for (int o = 0; o < 13; ++o) {
-            int q = p0 + off[o];
-            acc += Ar[(size_t)(o * 3 + 0) * N] * x[q];
-            acc += Ar[(size_t)(o * 3 + 1) * N] * x[P + q];
-            acc += Ar[(size_t)(o * 3 + 2) * N] * x[2 * P + q];
+            acc += slot(o * 3 + 0) * xin(0, odu[o], odv[o]);
+            acc += slot(o * 3 + 1) * xin(1, odu[o], odv[o]);
+            acc += slot(o * 3 + 2) * xin(2, odu[o], odv[o]);
         }
-        acc += Ar[(size_t)(SLOT_LE + 0) * N] * x[3 * P + p0 - 1];
-        acc += Ar[(size_t)(SLOT_LE + 1) * N] * x[3 * P + p0];
-        acc += Ar[(size_t)(SLOT_LE + 2) * N] * x[3 * P + p0 + 1];
-        acc += Ar[(size_t)(SLOT_LE + 3) * N] * x[4 * P + p0 - Wp];
-        acc += Ar[(size_t)(SLOT_LE + 4) * N] * x[4 * P + p0];
-        acc += Ar[(size_t)(SLOT_LE + 5) * N] * x[4 * P + p0 + Wp];
+        acc += slot(SLOT_LE + 0) * xin(3, -1, 0);
+        acc += slot(SLOT_LE + 1) * xin(3, 0, 0);
+        acc += slot(SLOT_LE + 2) * xin(3, 1, 0);
+        acc += slot(SLOT_LE + 3) * xin(4, 0, -1);
+        acc += slot(SLOT_LE + 4) * xin(4, 0, 0);
+        acc += slot(SLOT_LE + 5) * xin(4, 0, 1);
+        *y0 = acc;
     } else {
-        // E_u row then E_v row of the same node
         double au = 0.0, av = 0.0;
-        const double* Au = A + (size_t)SLOT_EU * N;
-        const double* Av = A + (size_t)SLOT_EV * N;
 #pragma unroll
         for (int d = -1; d <= 1; ++d) {
-            int qu = p0 + d, qv = p0 + d * Wp;
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                au += Au[(size_t)((d + 1) * 3 + cc) * N] * x[cc * P + qu];
-                av += Av[(size_t)((d + 1) * 3 + cc) * N] * x[cc * P + qv];
+                au += slot((d + 1) * 3 + cc) * xin(cc, d, 0);
+                av += slot(12 + (d + 1) * 3 + cc) * xin(cc, 0, d);
             }
-            au += Au[(size_t)(9 + d + 1) * N] * x[3 * P + qu];
-            av += Av[(size_t)(9 + d + 1) * N] * x[4 * P + qv];
+            au += slot(9 + d + 1) * xin(3, d, 0);
+            av += slot(12 + 9 + d + 1) * xin(4, 0, d);
         }
-        return g == 3 ? au : av;  // g == 3 computes both, see caller
+        *y0 = au;
+        *y1 = av;
     }
-    return acc;
 }
 
-// y = A x over the 4N row groups; optionally partial sums of x . y per block
+__device__ __forceinline__ int row_slot0(int g) { return g < 3 ? g * 45 : SLOT_EU; }
+
+// y = A x; optionally partial sums of x . y per block (one block per tile)
 __global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, const CGScal* sc, int check_done) {
     if (check_done && ld_done(sc)) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int tile = blockIdx.x;
+    int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    int u = (tile % a.ntx) * TILE + (lane & 7), v = (tile / a.ntx) * TILE + (lane >> 3);
     double dotp = 0.0;
-    if (t < 4 * a.N) {
-        int g = t / a.N, k = t - g * a.N;
-        int u = k % a.W, v = k / a.W;
-        int p0 = padidx(a.Wp, u, v);
+    if (u < a.W && v < a.Lg) {
+        const double* At = a.A + a_base(a.ntx, u, v);
+        const int p0 = padidx(a.Wp, u, v);
+        const double* x = a.x;
+        const int Wp = a.Wp, P = a.P;
+        auto xin = [&](int c, int du, int dv) { return x[c * P + p0 + du + dv * Wp]; };
+        const double* Ag = At + (size_t)row_slot0(g) * TILE_N;
+        auto slot = [&](int i) { return Ag[(size_t)i * TILE_N]; };
+        double y0 = 0.0, y1 = 0.0;
+        stencil_rows(g, slot, xin, &y0, &y1);
         if (g < 3) {
-            double r = spmv_row(a, k, g);
-            a.y[g * a.P + p0] = r;
-            dotp = r * a.x[g * a.P + p0];
+            a.y[g * P + p0] = y0;
+            dotp = y0 * x[g * P + p0];
         } else {
-            // both E rows in one thread (12 + 12 slots)
-            double au = 0.0, av = 0.0;
-            const double* Au = a.A + k + (size_t)SLOT_EU * a.N;
-            const double* Av = a.A + k + (size_t)SLOT_EV * a.N;
-            size_t N = a.N;
-#pragma unroll
-            for (int d = -1; d <= 1; ++d) {
-                int qu = p0 + d, qv = p0 + d * a.Wp;
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) {
-                    au += Au[(size_t)((d + 1) * 3 + cc) * N] * a.x[cc * a.P + qu];
-                    av += Av[(size_t)((d + 1) * 3 + cc) * N] * a.x[cc * a.P + qv];
-                }
-                au += Au[(size_t)(9 + d + 1) * N] * a.x[3 * a.P + qu];
-                av += Av[(size_t)(9 + d + 1) * N] * a.x[4 * a.P + qv];
-            }
-            a.y[3 * a.P + p0] = au;
-            a.y[4 * a.P + p0] = av;
-            dotp = au * a.x[3 * a.P + p0] + av * a.x[4 * a.P + p0];
+            a.y[3 * P + p0] = y0;
+            a.y[4 * P + p0] = y1;
+            dotp = y0 * x[3 * P + p0] + y1 * x[4 * P + p0];
         }
     }
     if (partials) {
@@ -462,6 +479,214 @@ __global__ void __launch_bounds__(BLK) k_cg_p(VecArgs a, CGPart Pt, CGScal* sc, 
         int i = vec_index(a, t, c);
         a.pb[0][i] = a.invd[i] * a.r[i] + beta * a.pb[0][i];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Single-reduction Jacobi-PCG: the reference solver's recurrences (src/Solvers.cpp:22-24,
+// Eigen's ConjugateGradient with DiagonalPreconditioner) with one kernel and one grid-wide
+// reduction per iteration.  With u = D^-1 r, w = A u, s = A p kept as vectors,
+//   p_j = u_j + beta_j p_{j-1}      s_j = w_j + beta_j s_{j-1}      (= A p_j by linearity)
+//   x_{j+1} = x_j + alpha_j p_j     r_{j+1} = r_j - alpha_j s_j
+//   u_{j+1} = D^-1 r_{j+1}          w_{j+1} = A u_{j+1}
+//   beta_j = gamma_j / gamma_{j-1},  gamma = r.u,
+//   alpha_j = gamma_j / p_j.A p_j, with p_j.A p_j expanded as
+//     u_j.w_j + beta_j (u_j.s_{j-1} + p_{j-1}.w_j) + beta_j^2 p_{j-1}.s_{j-1},
+// whose five inner products are all known at the end of the previous iteration.  Nothing
+// assumes A symmetric (it is not: the reference writes the L-L blocks transposed), so the
+// iterates are the reference's up to rounding, with the same stopping rule on the
+// recursive |r|^2 < tol^2 |b|^2 and the same iteration count convention.
+// A block owns one 8 x 8 tile.  It recomputes the element-wise recurrences on the
+// tile's 2-node halo itself (bit-identical to the owner's values) and keeps
+// u_{j+1} of tile + halo in LDS for the stencil product, so no second grid-wide
+// synchronisation is needed.  Vectors read on the halo are double-buffered by
+// iteration parity; u is never stored.
+// ---------------------------------------------------------------------------
+constexpr int HALO_W = TILE + 4;
+constexpr int HALO_N = HALO_W * HALO_W;
+
+struct Cg1Args {
+    const double* A; const double* invd;
+    double* x;
+    double* r[2]; double* w[2]; double* p[2]; double* s[2];
+    double* part[2];        // per tile, 6 x nT by parity: r.u, u.w, u.s, p.w, p.s, r.r
+    const double* bbpart;   // per vector block of k_cg_init: b.b
+    const double* rrpart;   // per vector block of k_cg_init: r_0.r_0
+    int nbb;
+    CGScal* sc;
+    int W, Lg, N, Wp, P, ntx, nT;
+};
+
+// init = 1: w_0 = A D^-1 r_0 and the partials of the next iteration (r_0 in r[0]; p and
+// s count as zero); block 0 also sets the stopping threshold max(tol^2 |b|^2, DBL_MIN)
+// and maxit.  init = 0: iteration j with parity q = j & 1 (reads r, w from [q] and p, s
+// from [1-q]; writes the opposite); tol and maxit are not used.
+// Every load that does not depend on alpha and beta -- the tile's matrix rows, the
+// partials, the scalars and the vector entries of tile + halo -- is issued before the
+// first barrier, so one memory latency covers them all.
+constexpr int CG1_NE = (5 * HALO_N + BLK - 1) / BLK;   // vector entries per thread
+
+__global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int q, int init, double tol, int maxit) {
+    __shared__ double su[5][HALO_N];
+    __shared__ double sp[5][TILE_N];
+    __shared__ double sred[8];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int tu0 = (tile % a.ntx) * TILE, tv0 = (tile / a.ntx) * TILE;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const int rq = init ? 0 : q;           // parity of r, w read
+    // (a) this thread's matrix rows into registers
+    const int nu = tu0 + (lane & 7), nv = tv0 + (lane >> 3);
+    const bool node_in = nu < a.W && nv < a.Lg;
+    double Ar[45];
+    {
+        const double* Ag = a.A + a_base(a.ntx, nu < a.W ? nu : 0, nv < a.Lg ? nv : 0) + (size_t)row_slot0(g) * TILE_N;
+        const int ns = g < 3 ? 45 : 24;
+#pragma unroll
+        for (int i = 0; i < 45; ++i) Ar[i] = (node_in && i < ns) ? Ag[(size_t)i * TILE_N] : 0.0;
+    }
+    // (b) partials and scalars
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int done = 0;
+    if (init) {
+        for (int i = threadIdx.x; i < a.nbb; i += BLK) { acc[0] += a.bbpart[i]; acc[1] += a.rrpart[i]; }
+    } else {
+        const double* pp = a.part[q];
+        for (int i = threadIdx.x; i < a.nT; i += BLK)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) acc[m] += pp[m * a.nT + i];
+        if (threadIdx.x == 0) done = __atomic_load_n(&a.sc->done, __ATOMIC_RELAXED);
+    }
+    // (c) vector entries of tile + halo
+    const double* r_in = a.r[rq];
+    const double* w_in = a.w[rq];
+    const double* s_in = a.s[1 - q];
+    const double* p_in = a.p[1 - q];
+    double e_id[CG1_NE], e_r[CG1_NE], e_w[CG1_NE], e_s[CG1_NE], e_p[CG1_NE], e_x[CG1_NE];
+    size_t e_i[CG1_NE];
+    bool e_in[CG1_NE], e_own[CG1_NE];
+#pragma unroll
+    for (int t = 0; t < CG1_NE; ++t) {
+        int e = threadIdx.x + t * BLK;
+        int c = e / HALO_N, ln = e - c * HALO_N;
+        int lx = ln % HALO_W, ly = ln / HALO_W;
+        int u = tu0 - 2 + lx, v = tv0 - 2 + ly;
+        e_own[t] = e < 5 * HALO_N && lx >= 2 && lx < TILE + 2 && ly >= 2 && ly < TILE + 2;
+        e_in[t] = e < 5 * HALO_N && u >= 0 && u < a.W && v >= 0 && v < a.Lg;
+        e_i[t] = e_in[t] ? (size_t)c * a.P + padidx(a.Wp, u, v) : 0;
+        e_id[t] = e_r[t] = e_w[t] = e_s[t] = e_p[t] = e_x[t] = 0.0;
+        if (e_in[t]) {
+            e_id[t] = a.invd[e_i[t]];
+            e_r[t] = r_in[e_i[t]];
+            if (!init) {
+                e_w[t] = w_in[e_i[t]];
+                e_s[t] = s_in[e_i[t]];
+                if (e_own[t]) { e_p[t] = p_in[e_i[t]]; e_x[t] = a.x[e_i[t]]; }
+            }
+        }
+    }
+    // (d) reduce the partials; every thread gets the sums
+    block_sum_n(acc);
+    if (threadIdx.x == 0) {
+        for (int m = 0; m < 6; ++m) sred[m] = acc[m];
+        sred[6] = (double)done;
+    }
+    __syncthreads();
+    double alpha = 0.0, beta = 0.0;
+    bool first = false;
+    if (init) {
+        if (lead) {
+            CGScal* sc = a.sc;
+            double bb = sred[0];
+            sc->bb = bb;
+            sc->rr = sred[1];
+            double thr = tol * tol * bb;
+            sc->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+            sc->maxit = maxit;
+            sc->itc[0] = 0; sc->itc[1] = 0; sc->fin_iters = 0;
+            sc->zero_rhs = bb == 0.0;
+            sc->done = (bb == 0.0 || maxit <= 0) ? 1 : 0;
+        }
+    } else {
+        if (sred[6] != 0.0) return;   // block-uniform
+        const double gam = sred[0], d1 = sred[1], d2 = sred[2], d3 = sred[3], d4 = sred[4], rr = sred[5];
+        const CGScal* sc = a.sc;
+        const int j = sc->itc[1 - q];
+        if (rr < sc->thr || j >= sc->maxit) {
+            // Stop: r_j met the threshold (the reference loop broke in iteration j-1,
+            // which it does not count) or maxit iterations are done.
+            if (lead) {
+                a.sc->rr = rr;
+                a.sc->fin_iters = rr < sc->thr ? (j > 0 ? j - 1 : 0) : j;
+                a.sc->done = 1;
+            }
+            return;
+        }
+        first = j == 0;
+        beta = first ? 0.0 : gam / sc->gamma[1 - q];
+        const double pAp = first ? d1 : d1 + beta * (d2 + d3) + beta * beta * d4;
+        alpha = gam / pAp;
+        if (lead) { a.sc->gamma[q] = gam; a.sc->alpha[q] = alpha; a.sc->itc[q] = j + 1; a.sc->rr = rr; }
+    }
+    // (e) element-wise recurrences on tile + halo; u_{j+1} and p_j into LDS
+    double* r_out = a.r[1 - q];
+    double* s_out = a.s[q];
+    double* p_out = a.p[q];
+    double* w_out = a.w[init ? 0 : 1 - q];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc[m] = 0.0;   // r.u, u.w, u.s, p.w, p.s, r.r
+#pragma unroll
+    for (int t = 0; t < CG1_NE; ++t) {
+        int e = threadIdx.x + t * BLK;
+        if (e >= 5 * HALO_N) continue;
+        int c = e / HALO_N, ln = e - c * HALO_N;
+        double sj = init ? 0.0 : (first ? e_w[t] : e_w[t] + beta * e_s[t]);
+        double rn = e_r[t] - alpha * sj;
+        double un = e_id[t] * rn;
+        double pj = 0.0;
+        if (e_in[t] && e_own[t]) {
+            if (!init) {
+                pj = e_id[t] * e_r[t];
+                if (!first) pj += beta * e_p[t];
+                a.x[e_i[t]] = e_x[t] + alpha * pj;
+                p_out[e_i[t]] = pj;
+                s_out[e_i[t]] = sj;
+                r_out[e_i[t]] = rn;
+            }
+            acc[0] += rn * un;
+            acc[2] += un * sj;
+            acc[4] += pj * sj;
+            acc[5] += rn * rn;
+        }
+        su[c][ln] = un;
+        if (e_own[t]) {
+            int lx = ln % HALO_W, ly = ln / HALO_W;
+            sp[c][(lx - 2) + (ly - 2) * TILE] = pj;
+        }
+    }
+    __syncthreads();
+    // (f) w_{j+1} = A u_{j+1} on the tile's own nodes, from the registers
+    if (node_in) {
+        const int l0 = (2 + (lane & 7)) + (2 + (lane >> 3)) * HALO_W;
+        auto xin = [&](int c, int du, int dv) { return su[c][l0 + du + dv * HALO_W]; };
+        auto slot = [&](int i) { return Ar[i]; };
+        double y0 = 0.0, y1 = 0.0;
+        stencil_rows(g, slot, xin, &y0, &y1);
+        const int p0 = padidx(a.Wp, nu, nv);
+        if (g < 3) {
+            w_out[g * a.P + p0] = y0;
+            acc[1] += y0 * su[g][l0];
+            acc[3] += y0 * sp[g][lane];
+        } else {
+            w_out[3 * a.P + p0] = y0;
+            w_out[4 * a.P + p0] = y1;
+            acc[1] += y0 * su[3][l0] + y1 * su[4][l0];
+            acc[3] += y0 * sp[3][lane] + y1 * sp[4][lane];
+        }
+    }
+    block_sum_n(acc);
+    double* part_out = a.part[init ? 0 : 1 - q];
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 6; ++m) part_out[m * a.nT + blockIdx.x] = acc[m];
 }
 
 // ---------------------------------------------------------------------------
@@ -580,6 +805,13 @@ struct yc_ctx {
     int graph_chunk;
     int last_iters;
     int ks_cg_blocks;
+    // matrix tiles and the single-reduction CG
+    int ntx, nty, nT;
+    size_t a_size;
+    int cg_variant;                         // YC_CG_REFERENCE or YC_CG_FUSED
+    double *cr1, *cw1, *cp1, *cs0, *cs1;    // second parity buffers (r, w, p) and s
+    double* part1;                          // 2 x 3 x nT partials
+    hipGraphExec_t graph1;
 };
 
 namespace {
@@ -587,7 +819,7 @@ namespace {
 ClothDev make_dev(const yc_ctx* c) {
     ClothDev C;
     const yc_desc& d = c->d;
-    C.W = c->W; C.Lg = c->Lg; C.N = c->N; C.Nc = c->Nc;
+    C.W = c->W; C.Lg = c->Lg; C.N = c->N; C.Nc = c->Nc; C.ntx = c->ntx;
     C.n_edges = c->Lg * (c->W - 1) + c->W * (c->Lg - 1);
     C.n_bends = c->Lg * (c->W - 2) + c->W * (c->Lg - 2);
     C.L = d.L; C.R = d.R;
@@ -629,7 +861,7 @@ VecArgs make_vec(yc_ctx* c) {
 
 SpmvArgs make_spmv(yc_ctx* c, const double* in, double* out) {
     SpmvArgs a;
-    a.A = c->A; a.x = in; a.y = out; a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P;
+    a.A = c->A; a.x = in; a.y = out; a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx;
     return a;
 }
 
@@ -666,6 +898,31 @@ int build_graph(yc_ctx* c) {
     for (int i = 0; i < c->graph_chunk; ++i) cg_launch_iteration(c, i & 1);   // chunk is even
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     HIPCHK(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return YC_OK;
+}
+
+Cg1Args make_cg1(yc_ctx* c) {
+    Cg1Args a;
+    a.A = c->A; a.invd = c->invd; a.x = c->cx;
+    a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1;
+    a.p[0] = c->cp; a.p[1] = c->cp1; a.s[0] = c->cs0; a.s[1] = c->cs1;
+    a.part[0] = c->part1; a.part[1] = c->part1 + 6 * (size_t)c->nT;
+    a.bbpart = c->part.bb; a.rrpart = c->part.rr; a.nbb = c->part.nV;
+    a.sc = c->sc;
+    a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx; a.nT = c->nT;
+    return a;
+}
+
+int build_graph1(yc_ctx* c) {
+    if (c->graph1) return YC_OK;
+    hipGraph_t g;
+    Cg1Args a = make_cg1(c);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < c->graph_chunk; ++i)   // chunk is even: parity continues across chunks
+        hipLaunchKernelGGL(k_cg1, dim3(c->nT), dim3(BLK), 0, c->stream, a, i & 1, 0, 0.0, 0);
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    HIPCHK(hipGraphInstantiate(&c->graph1, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
     return YC_OK;
 }
@@ -729,11 +986,16 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
                            (const CGScal*)c->sc, 0);
         has_x0 = 1;
     }
+    const bool fused = c->cg_variant == YC_CG_FUSED;
     hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, has_x0);
-    hipLaunchKernelGGL(k_cg_init2, dim3(1), dim3(BLK), 0, c->stream, c->part, c->sc, tol, maxit);
+    if (fused)
+        hipLaunchKernelGGL(k_cg1, dim3(c->nT), dim3(BLK), 0, c->stream, make_cg1(c), 0, 1, tol, maxit);
+    else
+        hipLaunchKernelGGL(k_cg_init2, dim3(1), dim3(BLK), 0, c->stream, c->part, c->sc, tol, maxit);
     HIPCHK(hipGetLastError());
-    int rc = build_graph(c);
+    int rc = fused ? build_graph1(c) : build_graph(c);
     if (rc) return rc;
+    hipGraphExec_t graph = fused ? c->graph1 : c->graph;
     HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     long long launched = 0;
@@ -741,7 +1003,7 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
     int ahead = std::max(1, (int)(0.9 * c->last_iters) / c->graph_chunk);
     while (!c->h_sc->done && launched < (long long)maxit + c->graph_chunk) {
         int n = launched == 0 ? ahead : 1;
-        for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+        for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(graph, c->stream));
         launched += (long long)n * c->graph_chunk;
         HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -749,7 +1011,7 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
     if (c->h_sc->zero_rhs) HIPCHK(hipMemsetAsync(c->cx, 0, (size_t)5 * c->P * sizeof(double), c->stream));
     c->solved = true;
     double rel = c->h_sc->bb > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bb) : 0.0;
-    c->last_iters = std::max(c->h_sc->itc[0], c->h_sc->itc[1]);
+    c->last_iters = fused ? c->h_sc->fin_iters : std::max(c->h_sc->itc[0], c->h_sc->itc[1]);
     if (iters) *iters = c->last_iters;
     if (relres) *relres = rel;
     if (!std::isfinite(rel)) return fail(YC_ERR_NONFINITE, "NaN/Inf in the solution");
@@ -802,6 +1064,9 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     c->Wp = c->W + 4; c->Hp = c->Lg + 4;
     c->P = ((c->Wp * c->Hp + 63) / 64) * 64;
     c->graph_chunk = 32;
+    c->ntx = (c->W + TILE - 1) / TILE; c->nty = (c->Lg + TILE - 1) / TILE; c->nT = c->ntx * c->nty;
+    c->a_size = (size_t)c->nT * NSLOT * TILE_N;
+    c->cg_variant = YC_CG_REFERENCE;
     int rc = YC_OK;
     auto chk = [&](int r) { if (r && !rc) rc = r; };
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return fail(YC_ERR_HIP, "no HIP device"); }
@@ -810,14 +1075,17 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     chk(dalloc(&c->x, 3 * N)); chk(dalloc(&c->v, 3 * N)); chk(dalloc(&c->xfix, 3 * N));
     chk(dalloc(&c->uv, 2 * N)); chk(dalloc(&c->uvd, 2 * N)); chk(dalloc(&c->uvbar, 2 * N)); chk(dalloc(&c->nrm, 3 * N));
     chk(dalloc(&c->flags, N));
-    chk(dalloc(&c->A, (size_t)NSLOT * N)); chk(dalloc(&c->F, 5 * N));
+    chk(dalloc(&c->A, c->a_size)); chk(dalloc(&c->F, 5 * N));
     chk(dalloc(&c->is_slide, 2 * N)); chk(dalloc(&c->comp, N)); chk(dalloc(&c->fepart, 2 * N));
     size_t PV = (size_t)5 * c->P;
     chk(dalloc(&c->b, PV)); chk(dalloc(&c->invd, PV)); chk(dalloc(&c->cx, PV)); chk(dalloc(&c->cr, PV));
     chk(dalloc(&c->cp, PV)); chk(dalloc(&c->cy, PV));
     chk(dalloc(&c->ext, (size_t)c->n));
     chk(dalloc(&c->vpad, PV)); chk(dalloc(&c->dprev, PV)); chk(dalloc(&c->Av, PV));
-    c->part.nS = grid_for(4LL * c->N);
+    chk(dalloc(&c->cr1, PV)); chk(dalloc(&c->cw1, PV)); chk(dalloc(&c->cp1, PV));
+    chk(dalloc(&c->cs0, PV)); chk(dalloc(&c->cs1, PV));
+    chk(dalloc(&c->part1, (size_t)12 * c->nT));
+    c->part.nS = c->nT;
     c->part.nV = grid_for(5LL * c->N);
     chk(dalloc(&c->partials, (size_t)c->part.nS + 4 * (size_t)c->part.nV + 64));
     if (!rc) {
@@ -868,8 +1136,10 @@ void yc_destroy(yc_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
-                    c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag};
+                    c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
+                    c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
@@ -956,9 +1226,8 @@ int yc_assemble(yc_ctx* c, double h) {
 int yc_export_csr(yc_ctx* c, int* rowptr, int* col, double* val, int* nnz) {
     if (!c) return fail(YC_ERR_ARG, "null context");
     if (!c->assembled) return fail(YC_ERR_ARG, "export before assemble");
-    size_t N = c->N;
-    std::vector<double> A((size_t)NSLOT * N);
-    HIPCHK(hipMemcpyAsync(A.data(), c->A, A.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> Am(c->a_size);
+    HIPCHK(hipMemcpyAsync(Am.data(), c->A, Am.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     // rows in global order; every structurally possible slot inside the grid
     std::vector<std::vector<std::pair<int, double>>> rows(c->n);
@@ -970,17 +1239,18 @@ int yc_export_csr(yc_ctx* c, int* rowptr, int* col, double* val, int* nnz) {
     auto inside = [&](int u, int v) { return u >= 0 && u < c->W && v >= 0 && v < c->Lg; };
     for (int v = 0; v < c->Lg; ++v)
         for (int u = 0; u < c->W; ++u) {
-            int k = u + v * c->W;
+            const double* At = Am.data() + a_base(c->ntx, u, v);
+            auto A = [&](int slot) { return At[(size_t)slot * TILE_N]; };
             for (int rc = 0; rc < 3; ++rc) {
                 int row = gL(u, v, rc);
                 for (int o = 0; o < 13; ++o) {
                     int uu = u + offu[o], vv = v + offv[o];
                     if (!inside(uu, vv)) continue;
-                    for (int cc = 0; cc < 3; ++cc) rows[row].push_back({gL(uu, vv, cc), A[(size_t)(rc * 45 + o * 3 + cc) * N + k]});
+                    for (int cc = 0; cc < 3; ++cc) rows[row].push_back({gL(uu, vv, cc), A(rc * 45 + o * 3 + cc)});
                 }
                 for (int d = -1; d <= 1; ++d) {
-                    if (inside(u + d, v) && inter(u + d, v)) rows[row].push_back({gE(u + d, v, 0), A[(size_t)(rc * 45 + SLOT_LE + d + 1) * N + k]});
-                    if (inside(u, v + d) && inter(u, v + d)) rows[row].push_back({gE(u, v + d, 1), A[(size_t)(rc * 45 + SLOT_LE + 3 + d + 1) * N + k]});
+                    if (inside(u + d, v) && inter(u + d, v)) rows[row].push_back({gE(u + d, v, 0), A(rc * 45 + SLOT_LE + d + 1)});
+                    if (inside(u, v + d) && inter(u, v + d)) rows[row].push_back({gE(u, v + d, 1), A(rc * 45 + SLOT_LE + 3 + d + 1)});
                 }
             }
             if (!inter(u, v)) continue;
@@ -989,8 +1259,8 @@ int yc_export_csr(yc_ctx* c, int* rowptr, int* col, double* val, int* nnz) {
                 int base = a == 0 ? SLOT_EU : SLOT_EV;
                 for (int d = -1; d <= 1; ++d) {
                     int uu = u + (a == 0 ? d : 0), vv = v + (a == 1 ? d : 0);
-                    for (int cc = 0; cc < 3; ++cc) rows[row].push_back({gL(uu, vv, cc), A[(size_t)(base + (d + 1) * 3 + cc) * N + k]});
-                    if (inter(uu, vv)) rows[row].push_back({gE(uu, vv, a), A[(size_t)(base + 9 + d + 1) * N + k]});
+                    for (int cc = 0; cc < 3; ++cc) rows[row].push_back({gL(uu, vv, cc), A(base + (d + 1) * 3 + cc)});
+                    if (inter(uu, vv)) rows[row].push_back({gE(uu, vv, a), A(base + 9 + d + 1)});
                 }
             }
         }
@@ -1125,6 +1395,13 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
     }
     if (total_iters) *total_iters = tot;
     return worst;
+}
+
+int yc_set_cg_variant(yc_ctx* c, int variant) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (variant != YC_CG_REFERENCE && variant != YC_CG_FUSED) return fail(YC_ERR_ARG, "unknown CG variant");
+    c->cg_variant = variant;
+    return YC_OK;
 }
 
 int yc_time_spmv(yc_ctx* c, int reps, double* avg_ms) {

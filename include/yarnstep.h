@@ -114,6 +114,17 @@ int yc_update(yc_ctx* ctx, double h);
  * yc_solve (NULL = zero guess).  info may be NULL. */
 int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_step_info* info);
 
+/* CG variant of yc_solve / yc_step / yc_run and the warm-started forms.
+ * YC_CG_REFERENCE (default): the reference solver's Jacobi-PCG recurrences in its order
+ *   (src/Solvers.cpp:22-24), three kernels and two reductions per iteration.
+ * YC_CG_FUSED: the single-reduction (Chronopoulos-Gear) form of the same Jacobi-PCG --
+ *   the same Krylov iterates in exact arithmetic, the same stopping rule on |r|^2 and
+ *   the same iteration count convention -- with one kernel per iteration.  Solutions
+ *   agree with the reference form to the solve tolerance; counts may differ by rounding. */
+#define YC_CG_REFERENCE 0
+#define YC_CG_FUSED 1
+int yc_set_cg_variant(yc_ctx* ctx, int variant);
+
 /* Runs `steps` SimSteps back to back on the device with zero-guess CG; returns the
  * total CG iterations.  Host synchronises once per CG chunk only. */
 int yc_run(yc_ctx* ctx, double h, int steps, double tol, int maxit, long long* total_iters);
