@@ -508,7 +508,7 @@ struct Cg1Args {
     const double* A; const double* invd;
     double* x;
     double* r[2]; double* w[2]; double* p[2]; double* s[2];
-    double* part[2];        // per tile, 6 x nT by parity: r.u, u.w, u.s, p.w, p.s, r.r
+    double* part[2];        // per tile, 6 x (KP * BLK) by parity: r.u, u.w, u.s, p.w, p.s, r.r
     const double* bbpart;   // per vector block of k_cg_init: b.b
     const double* rrpart;   // per vector block of k_cg_init: r_0.r_0
     int nbb;
@@ -524,8 +524,13 @@ struct Cg1Args {
 // partials, the scalars and the vector entries of tile + halo -- is issued before the
 // first barrier, so one memory latency covers them all.
 constexpr int CG1_NE = (5 * HALO_N + BLK - 1) / BLK;   // vector entries per thread
+constexpr int CG1_MAXP = 4;                              // partial rounds: nT <= 4 * BLK
 
-__global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int q, int init, double tol, int maxit) {
+// KP = ceil(nT / BLK): partial sums per thread.  The partial arrays are KP * BLK long
+// per quantity and zero past nT, so every load below is unconditional (a conditional
+// load would make the compiler wait for it inside its branch).
+template <int KP, int q>
+__global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int init, double tol, int maxit) {
     __shared__ double su[5][HALO_N];
     __shared__ double sp[5][TILE_N];
     __shared__ double sred[8];
@@ -534,29 +539,38 @@ __global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int q, int init, double 
     const int tu0 = (tile % a.ntx) * TILE, tv0 = (tile / a.ntx) * TILE;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int rq = init ? 0 : q;           // parity of r, w read
-    // (a) this thread's matrix rows into registers
+    const int PS = KP * BLK;               // stride of the partial arrays
+    // (a) scalars of the previous iteration and the partials (first, so that waiting
+    // for them does not wait for the larger loads issued after them).  They are read
+    // in the init launch too and ignored there, so that no load sits in a branch.
+    const CGScal* sc = a.sc;
+    const int done = sc->done, j = sc->itc[1 - q], s_maxit = sc->maxit;
+    const double s_thr = sc->thr, gprev = sc->gamma[1 - q];
+    const double* pp = a.part[q];
+    double pv[KP][6];
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pv[k][m] = pp[m * PS + threadIdx.x + k * BLK];
+    // (b) this thread's matrix rows into registers (a node outside the grid reads the
+    // tile's first node and discards the result)
     const int nu = tu0 + (lane & 7), nv = tv0 + (lane >> 3);
     const bool node_in = nu < a.W && nv < a.Lg;
     double Ar[45];
     {
-        const double* Ag = a.A + a_base(a.ntx, nu < a.W ? nu : 0, nv < a.Lg ? nv : 0) + (size_t)row_slot0(g) * TILE_N;
-        const int ns = g < 3 ? 45 : 24;
+        const double* Ag = a.A + a_base(a.ntx, node_in ? nu : tu0, node_in ? nv : tv0) + (size_t)row_slot0(g) * TILE_N;
+        if (g < 3) {
 #pragma unroll
-        for (int i = 0; i < 45; ++i) Ar[i] = (node_in && i < ns) ? Ag[(size_t)i * TILE_N] : 0.0;
-    }
-    // (b) partials and scalars
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int done = 0;
-    if (init) {
-        for (int i = threadIdx.x; i < a.nbb; i += BLK) { acc[0] += a.bbpart[i]; acc[1] += a.rrpart[i]; }
-    } else {
-        const double* pp = a.part[q];
-        for (int i = threadIdx.x; i < a.nT; i += BLK)
+            for (int i = 0; i < 45; ++i) Ar[i] = Ag[(size_t)i * TILE_N];
+        } else {
 #pragma unroll
-            for (int m = 0; m < 6; ++m) acc[m] += pp[m * a.nT + i];
-        if (threadIdx.x == 0) done = __atomic_load_n(&a.sc->done, __ATOMIC_RELAXED);
+            for (int i = 0; i < 24; ++i) Ar[i] = Ag[(size_t)i * TILE_N];
+#pragma unroll
+            for (int i = 24; i < 45; ++i) Ar[i] = 0.0;
+        }
     }
-    // (c) vector entries of tile + halo
+    // (c) vector entries of tile + halo.  Entries outside the grid read padded index 0
+    // (a padding corner, always zero).
     const double* r_in = a.r[rq];
     const double* w_in = a.w[rq];
     const double* s_in = a.s[1 - q];
@@ -573,18 +587,24 @@ __global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int q, int init, double 
         e_own[t] = e < 5 * HALO_N && lx >= 2 && lx < TILE + 2 && ly >= 2 && ly < TILE + 2;
         e_in[t] = e < 5 * HALO_N && u >= 0 && u < a.W && v >= 0 && v < a.Lg;
         e_i[t] = e_in[t] ? (size_t)c * a.P + padidx(a.Wp, u, v) : 0;
-        e_id[t] = e_r[t] = e_w[t] = e_s[t] = e_p[t] = e_x[t] = 0.0;
-        if (e_in[t]) {
-            e_id[t] = a.invd[e_i[t]];
-            e_r[t] = r_in[e_i[t]];
-            if (!init) {
-                e_w[t] = w_in[e_i[t]];
-                e_s[t] = s_in[e_i[t]];
-                if (e_own[t]) { e_p[t] = p_in[e_i[t]]; e_x[t] = a.x[e_i[t]]; }
-            }
-        }
+        const size_t io = e_own[t] ? e_i[t] : 0;
+        e_id[t] = a.invd[e_i[t]];
+        e_r[t] = r_in[e_i[t]];
+        e_w[t] = w_in[e_i[t]];
+        e_s[t] = s_in[e_i[t]];
+        e_p[t] = p_in[io];
+        e_x[t] = a.x[io];
     }
     // (d) reduce the partials; every thread gets the sums
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (init) {
+        for (int i = threadIdx.x; i < a.nbb; i += BLK) { acc[0] += a.bbpart[i]; acc[1] += a.rrpart[i]; }
+    } else {
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) acc[m] += pv[k][m];
+    }
     block_sum_n(acc);
     if (threadIdx.x == 0) {
         for (int m = 0; m < 6; ++m) sred[m] = acc[m];
@@ -609,33 +629,30 @@ __global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int q, int init, double 
     } else {
         if (sred[6] != 0.0) return;   // block-uniform
         const double gam = sred[0], d1 = sred[1], d2 = sred[2], d3 = sred[3], d4 = sred[4], rr = sred[5];
-        const CGScal* sc = a.sc;
-        const int j = sc->itc[1 - q];
-        if (rr < sc->thr || j >= sc->maxit) {
+        if (rr < s_thr || j >= s_maxit) {
             // Stop: r_j met the threshold (the reference loop broke in iteration j-1,
             // which it does not count) or maxit iterations are done.
             if (lead) {
                 a.sc->rr = rr;
-                a.sc->fin_iters = rr < sc->thr ? (j > 0 ? j - 1 : 0) : j;
+                a.sc->fin_iters = rr < s_thr ? (j > 0 ? j - 1 : 0) : j;
                 a.sc->done = 1;
             }
             return;
         }
         first = j == 0;
-        beta = first ? 0.0 : gam / sc->gamma[1 - q];
+        beta = first ? 0.0 : gam / gprev;
         const double pAp = first ? d1 : d1 + beta * (d2 + d3) + beta * beta * d4;
         alpha = gam / pAp;
         if (lead) { a.sc->gamma[q] = gam; a.sc->alpha[q] = alpha; a.sc->itc[q] = j + 1; a.sc->rr = rr; }
     }
-    // (e) element-wise recurrences on tile + halo; u_{j+1} and p_j into LDS
-    double* r_out = a.r[1 - q];
-    double* s_out = a.s[q];
-    double* p_out = a.p[q];
-    double* w_out = a.w[init ? 0 : 1 - q];
+    // (e) element-wise recurrences on tile + halo; u_{j+1} and p_j into LDS.  The
+    // global stores wait until after the LDS barrier, which then does not wait for them.
 #pragma unroll
     for (int m = 0; m < 6; ++m) acc[m] = 0.0;   // r.u, u.w, u.s, p.w, p.s, r.r
+    double o_x[CG1_NE], o_p[CG1_NE], o_s[CG1_NE], o_r[CG1_NE];
 #pragma unroll
     for (int t = 0; t < CG1_NE; ++t) {
+        o_x[t] = o_p[t] = o_s[t] = o_r[t] = 0.0;
         int e = threadIdx.x + t * BLK;
         if (e >= 5 * HALO_N) continue;
         int c = e / HALO_N, ln = e - c * HALO_N;
@@ -647,10 +664,10 @@ __global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int q, int init, double 
             if (!init) {
                 pj = e_id[t] * e_r[t];
                 if (!first) pj += beta * e_p[t];
-                a.x[e_i[t]] = e_x[t] + alpha * pj;
-                p_out[e_i[t]] = pj;
-                s_out[e_i[t]] = sj;
-                r_out[e_i[t]] = rn;
+                o_x[t] = e_x[t] + alpha * pj;
+                o_p[t] = pj;
+                o_s[t] = sj;
+                o_r[t] = rn;
             }
             acc[0] += rn * un;
             acc[2] += un * sj;
@@ -665,28 +682,48 @@ __global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int q, int init, double 
     }
     __syncthreads();
     // (f) w_{j+1} = A u_{j+1} on the tile's own nodes, from the registers
+    double y0 = 0.0, y1 = 0.0;
     if (node_in) {
         const int l0 = (2 + (lane & 7)) + (2 + (lane >> 3)) * HALO_W;
         auto xin = [&](int c, int du, int dv) { return su[c][l0 + du + dv * HALO_W]; };
         auto slot = [&](int i) { return Ar[i]; };
-        double y0 = 0.0, y1 = 0.0;
         stencil_rows(g, slot, xin, &y0, &y1);
-        const int p0 = padidx(a.Wp, nu, nv);
         if (g < 3) {
-            w_out[g * a.P + p0] = y0;
             acc[1] += y0 * su[g][l0];
             acc[3] += y0 * sp[g][lane];
         } else {
-            w_out[3 * a.P + p0] = y0;
-            w_out[4 * a.P + p0] = y1;
             acc[1] += y0 * su[3][l0] + y1 * su[4][l0];
             acc[3] += y0 * sp[3][lane] + y1 * sp[4][lane];
         }
     }
     block_sum_n(acc);
+    // (g) all global stores last, so that no barrier above waits for them
     double* part_out = a.part[init ? 0 : 1 - q];
     if (threadIdx.x == 0)
-        for (int m = 0; m < 6; ++m) part_out[m * a.nT + blockIdx.x] = acc[m];
+        for (int m = 0; m < 6; ++m) part_out[m * PS + blockIdx.x] = acc[m];
+    if (node_in) {
+        double* w_out = a.w[init ? 0 : 1 - q];
+        const int p0 = padidx(a.Wp, nu, nv);
+        if (g < 3) {
+            w_out[g * a.P + p0] = y0;
+        } else {
+            w_out[3 * a.P + p0] = y0;
+            w_out[4 * a.P + p0] = y1;
+        }
+    }
+    if (!init) {
+        double* r_out = a.r[1 - q];
+        double* s_out = a.s[q];
+        double* p_out = a.p[q];
+#pragma unroll
+        for (int t = 0; t < CG1_NE; ++t)
+            if (e_in[t] && e_own[t]) {
+                a.x[e_i[t]] = o_x[t];
+                p_out[e_i[t]] = o_p[t];
+                s_out[e_i[t]] = o_s[t];
+                r_out[e_i[t]] = o_r[t];
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -810,7 +847,8 @@ struct yc_ctx {
     size_t a_size;
     int cg_variant;                         // YC_CG_REFERENCE or YC_CG_FUSED
     double *cr1, *cw1, *cp1, *cs0, *cs1;    // second parity buffers (r, w, p) and s
-    double* part1;                          // 2 x 3 x nT partials
+    double* part1;                          // 2 parities x 6 x (cg1_kp * BLK) partials
+    int cg1_kp;                             // partial rounds of the fused CG kernel
     hipGraphExec_t graph1;
 };
 
@@ -907,11 +945,27 @@ Cg1Args make_cg1(yc_ctx* c) {
     a.A = c->A; a.invd = c->invd; a.x = c->cx;
     a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1;
     a.p[0] = c->cp; a.p[1] = c->cp1; a.s[0] = c->cs0; a.s[1] = c->cs1;
-    a.part[0] = c->part1; a.part[1] = c->part1 + 6 * (size_t)c->nT;
+    a.part[0] = c->part1; a.part[1] = c->part1 + 6 * (size_t)c->cg1_kp * BLK;
     a.bbpart = c->part.bb; a.rrpart = c->part.rr; a.nbb = c->part.nV;
     a.sc = c->sc;
     a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx; a.nT = c->nT;
     return a;
+}
+
+template <int Q>
+void launch_cg1_q(yc_ctx* c, const Cg1Args& a, int init, double tol, int maxit) {
+    dim3 grid(c->nT), blk(BLK);
+    switch (c->cg1_kp) {
+        case 1: hipLaunchKernelGGL((k_cg1<1, Q>), grid, blk, 0, c->stream, a, init, tol, maxit); break;
+        case 2: hipLaunchKernelGGL((k_cg1<2, Q>), grid, blk, 0, c->stream, a, init, tol, maxit); break;
+        case 3: hipLaunchKernelGGL((k_cg1<3, Q>), grid, blk, 0, c->stream, a, init, tol, maxit); break;
+        default: hipLaunchKernelGGL((k_cg1<4, Q>), grid, blk, 0, c->stream, a, init, tol, maxit); break;
+    }
+}
+
+void launch_cg1(yc_ctx* c, const Cg1Args& a, int q, int init, double tol, int maxit) {
+    if (q == 0) launch_cg1_q<0>(c, a, init, tol, maxit);
+    else launch_cg1_q<1>(c, a, init, tol, maxit);
 }
 
 int build_graph1(yc_ctx* c) {
@@ -920,7 +974,7 @@ int build_graph1(yc_ctx* c) {
     Cg1Args a = make_cg1(c);
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < c->graph_chunk; ++i)   // chunk is even: parity continues across chunks
-        hipLaunchKernelGGL(k_cg1, dim3(c->nT), dim3(BLK), 0, c->stream, a, i & 1, 0, 0.0, 0);
+        launch_cg1(c, a, i & 1, 0, 0.0, 0);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     HIPCHK(hipGraphInstantiate(&c->graph1, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
@@ -989,7 +1043,7 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
     const bool fused = c->cg_variant == YC_CG_FUSED;
     hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, has_x0);
     if (fused)
-        hipLaunchKernelGGL(k_cg1, dim3(c->nT), dim3(BLK), 0, c->stream, make_cg1(c), 0, 1, tol, maxit);
+        launch_cg1(c, make_cg1(c), 0, 1, tol, maxit);
     else
         hipLaunchKernelGGL(k_cg_init2, dim3(1), dim3(BLK), 0, c->stream, c->part, c->sc, tol, maxit);
     HIPCHK(hipGetLastError());
@@ -1084,7 +1138,8 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     chk(dalloc(&c->vpad, PV)); chk(dalloc(&c->dprev, PV)); chk(dalloc(&c->Av, PV));
     chk(dalloc(&c->cr1, PV)); chk(dalloc(&c->cw1, PV)); chk(dalloc(&c->cp1, PV));
     chk(dalloc(&c->cs0, PV)); chk(dalloc(&c->cs1, PV));
-    chk(dalloc(&c->part1, (size_t)12 * c->nT));
+    c->cg1_kp = std::min((c->nT + BLK - 1) / BLK, CG1_MAXP);
+    chk(dalloc(&c->part1, (size_t)12 * c->cg1_kp * BLK));
     c->part.nS = c->nT;
     c->part.nV = grid_for(5LL * c->N);
     chk(dalloc(&c->partials, (size_t)c->part.nS + 4 * (size_t)c->part.nV + 64));
@@ -1400,6 +1455,8 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
 int yc_set_cg_variant(yc_ctx* c, int variant) {
     if (!c) return fail(YC_ERR_ARG, "null context");
     if (variant != YC_CG_REFERENCE && variant != YC_CG_FUSED) return fail(YC_ERR_ARG, "unknown CG variant");
+    if (variant == YC_CG_FUSED && c->nT > CG1_MAXP * BLK)
+        return fail(YC_ERR_ARG, "fused CG supports up to 1024 matrix tiles (256 x 256 crossings)");
     c->cg_variant = variant;
     return YC_OK;
 }
