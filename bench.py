@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="time only the zero-guess CG path (profiling runs)")
-    ap.add_argument("--cg", default="fused", choices=["fused", "reference"],
+    ap.add_argument("--cg", default="persistent", choices=["persistent", "fused", "reference"],
                     help="CG form of the timed runs (the reference form is also timed once, zero guess)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run 5 launches of a 256 MiB -> 256 MiB FP64 negation, the known-size kernel "
@@ -92,7 +92,11 @@ def main():
 
     def timed(warm, variant=None):
         cloth = ys.Cloth(spec)
-        cloth.set_cg_variant(variant or args.cg)
+        try:
+            cloth.set_cg_variant(variant or args.cg)
+        except ys.YcError:   # the persistent form needs every tile resident at once
+            args.cg = "fused"
+            cloth.set_cg_variant(args.cg)
         cloth.set_ClothState(state)
         cloth.set_ext_force(ext)
         run = (lambda n: cloth.run_warm(net, h, n, tol=tol)) if warm else (lambda n: cloth.run(h, n, tol=tol))

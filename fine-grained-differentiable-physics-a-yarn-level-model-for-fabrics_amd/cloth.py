@@ -177,9 +177,10 @@ class Cloth:
             _abi.check(rc)
         return tot.value, rc
 
-    # CG variants: "reference" (the reference solver's three-kernel order) or "fused"
-    # (the same recurrences with one kernel and one reduction per iteration)
-    CG_VARIANTS = {"reference": 0, "fused": 1}
+    # CG variants: "reference" (the reference solver's three-kernel order), "fused" (the
+    # same recurrences with one kernel and one reduction per iteration) or "persistent"
+    # (the fused recurrences in one cooperative launch per solve)
+    CG_VARIANTS = {"reference": 0, "fused": 1, "persistent": 2}
 
     def set_cg_variant(self, name):
         _abi.check(_abi.load_library().yc_set_cg_variant(self._h, self.CG_VARIANTS[name]))

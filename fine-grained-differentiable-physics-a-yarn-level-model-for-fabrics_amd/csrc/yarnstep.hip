@@ -166,9 +166,17 @@ struct AsmOut {
 
 __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
 
+// Node of thread t in the assembly kernels: one block per 8 x 8 matrix tile, so that a
+// wavefront's read-modify-writes of one slot are 512 contiguous bytes.
+__device__ __forceinline__ int tile_node(const ClothDev& C) {
+    int u = (blockIdx.x % C.ntx) * TILE + (threadIdx.x & (TILE - 1));
+    int v = (blockIdx.x / C.ntx) * TILE + (threadIdx.x / TILE);
+    return (u < C.W && v < C.Lg) ? u + v * C.W : -1;
+}
+
 __global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double h) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= C.N) return;
+    int k = tile_node(C);
+    if (k < 0) return;
     RowSink S;
     S.init(&C, k, O.A, h);
     int u = S.mu_, v = S.mv_;
@@ -238,8 +246,8 @@ template <class Sink>
 __device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp);
 
 __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double h) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= C.N) return;
+    int k = tile_node(C);
+    if (k < 0) return;
     RowSink S;
     S.init(&C, k, O.A, h);
     int u = S.mu_, v = S.mv_;
@@ -727,6 +735,267 @@ __global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int init, double tol, in
 }
 
 // ---------------------------------------------------------------------------
+// Persistent form of the single-reduction CG: one cooperative launch runs the whole
+// solve.  Each block owns one 8 x 8 tile for all iterations and keeps the tile's rows
+// of A (the 20 MB that every per-iteration kernel would otherwise stream from memory
+// again), D^-1, x and p in registers.  Blocks exchange only r, w, s on the tile
+// halos and their six partial sums per iteration.
+// Cross-block data goes through relaxed device-scope loads and stores (sc1: coherent
+// across the XCDs without L2 write-back or invalidation).  A block publishes an
+// iteration by storing its partials, waiting until all of its stores have completed
+// (s_waitcnt 0 in every thread, then a barrier), and only then storing its tag
+// (solve epoch << 32 | stage) in the same 64-byte slot.  A block starts iteration j
+// when every slot of parity j & 1 carries stage j; since no block can then be more than
+// one stage ahead, parity double buffers suffice.  Every wait is bounded: a block that
+// spins longer than spin_limit sets err and all blocks leave at their next wait.
+// ---------------------------------------------------------------------------
+struct CgSlot {
+    double v[6];
+    unsigned long long tag;
+    unsigned long long pad;
+};
+
+struct CgpArgs {
+    const double* A; const double* invd; const double* bbpart; const double* rrpart;
+    double* x;
+    double* r[2]; double* w[2]; double* s[2];
+    CgSlot* slot[2];
+    CGScal* sc;
+    int* err;
+    unsigned long long epoch;
+    double tol;
+    int maxit, nbb, spin_limit;
+    int W, Lg, Wp, P, ntx, nT;
+};
+
+__device__ __forceinline__ double ld_dev(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until every slot of `sl` carries `want` (or the spin limit / a peer's error),
+// then sums the slots' partials in a fixed order; sred[0..5] receive the sums, sred[6]
+// is non-zero when the block must leave.  Block-uniform.
+template <int KP>
+__device__ void cgp_gather(const CgpArgs& a, const CgSlot* sl, unsigned long long want, double* sred) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        int i = threadIdx.x + k * BLK;
+        if (i >= a.nT) continue;
+        int spins = 0;
+        while (__hip_atomic_load(&sl[i].tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+            if (++spins > a.spin_limit || __hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        int i = threadIdx.x + k * BLK;
+        if (ok && i < a.nT)
+#pragma unroll
+            for (int m = 0; m < 6; ++m) acc[m] += ld_dev(&sl[i].v[m]);
+    }
+    int bad = __syncthreads_or(!ok);
+    block_sum_n(acc);
+    if (threadIdx.x == 0) {
+        for (int m = 0; m < 6; ++m) sred[m] = acc[m];
+        sred[6] = bad ? 1.0 : 0.0;
+        if (bad) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+}
+
+// Publishes this block's partials and tag after all of its stores have completed.
+__device__ __forceinline__ void cgp_publish(const CgpArgs& a, CgSlot* sl, const double* acc, unsigned long long tag) {
+    if (threadIdx.x == 0)
+        for (int m = 0; m < 6; ++m) st_dev(&sl[blockIdx.x].v[m], acc[m]);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_store(&sl[blockIdx.x].tag, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int KP>
+__global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
+    __shared__ double su[5][HALO_N];
+    __shared__ double sp[5][TILE_N];
+    __shared__ double sred[8];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int tu0 = (tile % a.ntx) * TILE, tv0 = (tile / a.ntx) * TILE;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    // resident data: matrix rows, entry indices, D^-1, x and p of the owned entries
+    const int nu = tu0 + (lane & 7), nv = tv0 + (lane >> 3);
+    const bool node_in = nu < a.W && nv < a.Lg;
+    const int p0 = node_in ? padidx(a.Wp, nu, nv) : 0;
+    const int l0 = (2 + (lane & 7)) + (2 + (lane >> 3)) * HALO_W;
+    double Ar[45];
+    {
+        const double* Ag = a.A + a_base(a.ntx, node_in ? nu : tu0, node_in ? nv : tv0) + (size_t)row_slot0(g) * TILE_N;
+        if (g < 3) {
+#pragma unroll
+            for (int i = 0; i < 45; ++i) Ar[i] = Ag[(size_t)i * TILE_N];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 24; ++i) Ar[i] = Ag[(size_t)i * TILE_N];
+#pragma unroll
+            for (int i = 24; i < 45; ++i) Ar[i] = 0.0;
+        }
+    }
+    int e_i[CG1_NE], e_ln[CG1_NE], e_c[CG1_NE];
+    bool e_in[CG1_NE], e_own[CG1_NE];
+    double e_id[CG1_NE], x_reg[CG1_NE], p_reg[CG1_NE];
+#pragma unroll
+    for (int t = 0; t < CG1_NE; ++t) {
+        int e = threadIdx.x + t * BLK;
+        int c = e / HALO_N, ln = e - c * HALO_N;
+        int lx = ln % HALO_W, ly = ln / HALO_W;
+        int u = tu0 - 2 + lx, v = tv0 - 2 + ly;
+        e_c[t] = c; e_ln[t] = ln;
+        e_own[t] = e < 5 * HALO_N && lx >= 2 && lx < TILE + 2 && ly >= 2 && ly < TILE + 2;
+        e_in[t] = e < 5 * HALO_N && u >= 0 && u < a.W && v >= 0 && v < a.Lg;
+        e_i[t] = e_in[t] ? c * a.P + padidx(a.Wp, u, v) : 0;
+        e_id[t] = a.invd[e_i[t]];
+        x_reg[t] = a.x[e_own[t] ? e_i[t] : 0];
+        p_reg[t] = 0.0;
+    }
+    // stopping threshold from the b.b and r_0.r_0 partials of k_cg_init (every block
+    // reduces them in the same order, so all blocks hold the same values)
+    double bb, rr0;
+    {
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = threadIdx.x; i < a.nbb; i += BLK) { acc[0] += a.bbpart[i]; acc[1] += a.rrpart[i]; }
+        block_sum_n(acc);
+        if (threadIdx.x == 0) { sred[0] = acc[0]; sred[1] = acc[1]; }
+        __syncthreads();
+        bb = sred[0]; rr0 = sred[1];
+    }
+    double thr = a.tol * a.tol * bb;
+    thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+    if (bb == 0.0 || a.maxit <= 0) {
+        if (lead) {
+            a.sc->bb = bb; a.sc->rr = rr0; a.sc->thr = thr; a.sc->maxit = a.maxit;
+            a.sc->zero_rhs = bb == 0.0; a.sc->fin_iters = 0; a.sc->itc[0] = a.sc->itc[1] = 0; a.sc->done = 1;
+        }
+        return;
+    }
+    // stage -1: w_0 = A D^-1 r_0 and the partials of iteration 0
+    double alpha = 0.0, beta = 0.0, gprev = 1.0;
+    for (int j = -1;; ++j) {
+        const int q = j & 1;
+        const bool init = j < 0;
+        const bool first = j == 0;
+        if (!init) {
+            cgp_gather<KP>(a, a.slot[q], (a.epoch << 32) | (unsigned long long)j, sred);
+            if (sred[6] != 0.0) return;   // error: block-uniform
+            const double gam = sred[0], d1 = sred[1], d2 = sred[2], d3 = sred[3], d4 = sred[4], rr = sred[5];
+            if (rr < thr || j >= a.maxit) {
+                if (lead) {
+                    a.sc->bb = bb; a.sc->rr = rr; a.sc->thr = thr; a.sc->maxit = a.maxit; a.sc->zero_rhs = 0;
+                    a.sc->fin_iters = rr < thr ? (j > 0 ? j - 1 : 0) : j;
+                    a.sc->itc[0] = a.sc->itc[1] = a.sc->fin_iters;
+                    a.sc->done = 1;
+                }
+                break;
+            }
+            beta = first ? 0.0 : gam / gprev;
+            const double pAp = first ? d1 : d1 + beta * (d2 + d3) + beta * beta * d4;
+            alpha = gam / pAp;
+            gprev = gam;
+        }
+        // inputs of tile + halo: r_j, w_j, s_{j-1}
+        const double* r_in = a.r[init ? 0 : q];
+        const double* w_in = a.w[q];
+        const double* s_in = a.s[1 - q];
+        double e_r[CG1_NE], e_w[CG1_NE], e_s[CG1_NE];
+#pragma unroll
+        for (int t = 0; t < CG1_NE; ++t) {
+            e_r[t] = ld_dev(r_in + e_i[t]);
+            e_w[t] = init ? 0.0 : ld_dev(w_in + e_i[t]);
+            e_s[t] = (init || first) ? 0.0 : ld_dev(s_in + e_i[t]);
+        }
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // r.u, u.w, u.s, p.w, p.s, r.r
+        double o_s[CG1_NE], o_r[CG1_NE];
+#pragma unroll
+        for (int t = 0; t < CG1_NE; ++t) {
+            o_s[t] = o_r[t] = 0.0;
+            if (threadIdx.x + t * BLK >= 5 * HALO_N) continue;
+            double sj = e_w[t] + beta * e_s[t];
+            double rn = e_r[t] - alpha * sj;
+            double un = e_id[t] * rn;
+            double pj = 0.0;
+            if (e_in[t] && e_own[t]) {
+                if (!init) {
+                    pj = e_id[t] * e_r[t] + beta * p_reg[t];
+                    x_reg[t] += alpha * pj;
+                    p_reg[t] = pj;
+                    o_s[t] = sj;
+                    o_r[t] = rn;
+                }
+                acc[0] += rn * un;
+                acc[2] += un * sj;
+                acc[4] += pj * sj;
+                acc[5] += rn * rn;
+            }
+            su[e_c[t]][e_ln[t]] = un;
+            if (e_own[t]) {
+                int lx = e_ln[t] % HALO_W, ly = e_ln[t] / HALO_W;
+                sp[e_c[t]][(lx - 2) + (ly - 2) * TILE] = pj;
+            }
+        }
+        __syncthreads();
+        double y0 = 0.0, y1 = 0.0;
+        if (node_in) {
+            auto xin = [&](int c, int du, int dv) { return su[c][l0 + du + dv * HALO_W]; };
+            auto slotv = [&](int i) { return Ar[i]; };
+            stencil_rows(g, slotv, xin, &y0, &y1);
+            if (g < 3) {
+                acc[1] += y0 * su[g][l0];
+                acc[3] += y0 * sp[g][lane];
+            } else {
+                acc[1] += y0 * su[3][l0] + y1 * su[4][l0];
+                acc[3] += y0 * sp[3][lane] + y1 * sp[4][lane];
+            }
+        }
+        block_sum_n(acc);
+        // outputs: w_{j+1} (all), r_{j+1} and s_j (iterations only)
+        double* w_out = a.w[init ? 0 : 1 - q];
+        if (node_in) {
+            if (g < 3) {
+                st_dev(w_out + g * a.P + p0, y0);
+            } else {
+                st_dev(w_out + 3 * a.P + p0, y0);
+                st_dev(w_out + 4 * a.P + p0, y1);
+            }
+        }
+        if (!init) {
+            double* r_out = a.r[1 - q];
+            double* s_out = a.s[q];
+#pragma unroll
+            for (int t = 0; t < CG1_NE; ++t)
+                if (e_in[t] && e_own[t]) {
+                    st_dev(s_out + e_i[t], o_s[t]);
+                    st_dev(r_out + e_i[t], o_r[t]);
+                }
+        }
+        cgp_publish(a, a.slot[init ? 0 : 1 - q], acc, (a.epoch << 32) | (unsigned long long)(j + 1));
+    }
+    // the solution of the owned entries
+#pragma unroll
+    for (int t = 0; t < CG1_NE; ++t)
+        if (e_in[t] && e_own[t]) a.x[e_i[t]] = x_reg[t];
+}
+
+// ---------------------------------------------------------------------------
 // Update (src/Cloth.cpp:111-148) and geometry refresh (src/Cloth.cpp:307-348)
 // ---------------------------------------------------------------------------
 struct StateDev {
@@ -849,6 +1118,10 @@ struct yc_ctx {
     double *cr1, *cw1, *cp1, *cs0, *cs1;    // second parity buffers (r, w, p) and s
     double* part1;                          // 2 parities x 6 x (cg1_kp * BLK) partials
     int cg1_kp;                             // partial rounds of the fused CG kernel
+    CgSlot* cgslot;                         // 2 parities x nT slots of the persistent CG
+    int* cg_err;
+    unsigned long long epoch;               // solves run by the persistent CG
+    int coop_ok;                            // the persistent CG fits on the device at once
     hipGraphExec_t graph1;
 };
 
@@ -952,6 +1225,48 @@ Cg1Args make_cg1(yc_ctx* c) {
     return a;
 }
 
+// The persistent CG needs every tile's block resident at once (cooperative launch).
+template <int KP>
+bool cgp_fits_kp(const yc_ctx* c) {
+    int dev = c->device, coop = 0, ncu = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return false;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cgp<KP>, BLK, 0) != hipSuccess) return false;
+    return (long long)per_cu * ncu >= c->nT;
+}
+
+bool cgp_fits(const yc_ctx* c) {
+    switch (c->cg1_kp) {
+        case 1: return cgp_fits_kp<1>(c);
+        case 2: return cgp_fits_kp<2>(c);
+        case 3: return cgp_fits_kp<3>(c);
+        default: return cgp_fits_kp<4>(c);
+    }
+}
+
+int launch_cgp(yc_ctx* c, double tol, int maxit) {
+    CgpArgs a;
+    a.A = c->A; a.invd = c->invd; a.bbpart = c->part.bb; a.rrpart = c->part.rr;
+    a.x = c->cx;
+    a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1; a.s[0] = c->cs0; a.s[1] = c->cs1;
+    a.slot[0] = c->cgslot; a.slot[1] = c->cgslot + c->nT;
+    a.sc = c->sc; a.err = c->cg_err;
+    a.epoch = ++c->epoch;
+    a.tol = tol; a.maxit = maxit; a.nbb = c->part.nV;
+    a.spin_limit = 1 << 18;   // polls of about 1 us each: a quarter second
+    a.W = c->W; a.Lg = c->Lg; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx; a.nT = c->nT;
+    void* args[] = {&a};
+    const void* fn;
+    switch (c->cg1_kp) {
+        case 1: fn = (const void*)k_cgp<1>; break;
+        case 2: fn = (const void*)k_cgp<2>; break;
+        case 3: fn = (const void*)k_cgp<3>; break;
+        default: fn = (const void*)k_cgp<4>; break;
+    }
+    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->nT), dim3(BLK), args, 0, c->stream));
+    return YC_OK;
+}
+
 template <int Q>
 void launch_cg1_q(yc_ctx* c, const Cg1Args& a, int init, double tol, int maxit) {
     dim3 grid(c->nT), blk(BLK);
@@ -1011,9 +1326,10 @@ void from_padded(const yc_ctx* c, const std::vector<double>& pad, double* glob) 
 int do_assemble(yc_ctx* c, double h) {
     HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
-    hipLaunchKernelGGL(k_assemble1, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
+    static_assert(NBLK == TILE_N, "assembly blocks are matrix tiles");
+    hipLaunchKernelGGL(k_assemble1, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_assemble2, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble2, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
                        c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
@@ -1022,6 +1338,19 @@ int do_assemble(yc_ctx* c, double h) {
     c->assembled = true;
     if (c->h_flag[0]) return fail(YC_ERR_HIP, "assembly: stencil slot out of range");
     if (c->h_flag[1]) return fail(YC_ERR_NONFINITE, "NaN/Inf in A or b");
+    return YC_OK;
+}
+
+// Results of a solve from the scalars copied to h_sc.
+int finish_solve(yc_ctx* c, bool fused, int* iters, double* relres) {
+    if (c->h_sc->zero_rhs) HIPCHK(hipMemsetAsync(c->cx, 0, (size_t)5 * c->P * sizeof(double), c->stream));
+    c->solved = true;
+    double rel = c->h_sc->bb > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bb) : 0.0;
+    c->last_iters = fused ? c->h_sc->fin_iters : std::max(c->h_sc->itc[0], c->h_sc->itc[1]);
+    if (iters) *iters = c->last_iters;
+    if (relres) *relres = rel;
+    if (!std::isfinite(rel)) return fail(YC_ERR_NONFINITE, "NaN/Inf in the solution");
+    if (c->h_sc->rr >= c->h_sc->thr && !c->h_sc->zero_rhs) return fail(YC_ERR_NOT_CONVERGED, "CG stopped at maxit above tolerance");
     return YC_OK;
 }
 
@@ -1040,8 +1369,21 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
                            (const CGScal*)c->sc, 0);
         has_x0 = 1;
     }
-    const bool fused = c->cg_variant == YC_CG_FUSED;
+    const bool persistent = c->cg_variant == YC_CG_PERSISTENT;
+    const bool fused = c->cg_variant == YC_CG_FUSED || persistent;
     hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, has_x0);
+    if (persistent) {
+        int rc = launch_cgp(c, tol, maxit);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_flag + 2, c->cg_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_flag[2]) {
+            (void)hipMemsetAsync(c->cg_err, 0, sizeof(int), c->stream);
+            return fail(YC_ERR_HIP, "persistent CG: a block waited past its spin limit");
+        }
+        return finish_solve(c, true, iters, relres);
+    }
     if (fused)
         launch_cg1(c, make_cg1(c), 0, 1, tol, maxit);
     else
@@ -1062,15 +1404,7 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
         HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    if (c->h_sc->zero_rhs) HIPCHK(hipMemsetAsync(c->cx, 0, (size_t)5 * c->P * sizeof(double), c->stream));
-    c->solved = true;
-    double rel = c->h_sc->bb > 0 ? std::sqrt(c->h_sc->rr / c->h_sc->bb) : 0.0;
-    c->last_iters = fused ? c->h_sc->fin_iters : std::max(c->h_sc->itc[0], c->h_sc->itc[1]);
-    if (iters) *iters = c->last_iters;
-    if (relres) *relres = rel;
-    if (!std::isfinite(rel)) return fail(YC_ERR_NONFINITE, "NaN/Inf in the solution");
-    if (c->h_sc->rr >= c->h_sc->thr && !c->h_sc->zero_rhs) return fail(YC_ERR_NOT_CONVERGED, "CG stopped at maxit above tolerance");
-    return YC_OK;
+    return finish_solve(c, fused, iters, relres);
 }
 
 int do_update(yc_ctx* c, double h) {
@@ -1139,6 +1473,8 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     chk(dalloc(&c->cr1, PV)); chk(dalloc(&c->cw1, PV)); chk(dalloc(&c->cp1, PV));
     chk(dalloc(&c->cs0, PV)); chk(dalloc(&c->cs1, PV));
     c->cg1_kp = std::min((c->nT + BLK - 1) / BLK, CG1_MAXP);
+    chk(dalloc(&c->cgslot, (size_t)2 * c->nT));
+    chk(dalloc(&c->cg_err, 1));
     chk(dalloc(&c->part1, (size_t)12 * c->cg1_kp * BLK));
     c->part.nS = c->nT;
     c->part.nV = grid_for(5LL * c->N);
@@ -1155,6 +1491,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     if (!rc && hipHostMalloc((void**)&c->h_sc, sizeof(CGScal)) != hipSuccess) rc = fail(YC_ERR_HIP, "pinned alloc");
     if (!rc && hipHostMalloc((void**)&c->h_flag, 4 * sizeof(int)) != hipSuccess) rc = fail(YC_ERR_HIP, "pinned alloc");
     if (rc) { yc_destroy(c); return rc; }
+    c->coop_ok = cgp_fits(c) ? 1 : 0;
     // constructor rest state (src/Cloth.cpp:52-99) and flags
     std::vector<double> x(3 * N), uv(2 * N);
     std::vector<unsigned char> fl(N, 0);
@@ -1194,7 +1531,7 @@ void yc_destroy(yc_ctx* c) {
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
-                    c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1};
+                    c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
@@ -1454,9 +1791,12 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
 
 int yc_set_cg_variant(yc_ctx* c, int variant) {
     if (!c) return fail(YC_ERR_ARG, "null context");
-    if (variant != YC_CG_REFERENCE && variant != YC_CG_FUSED) return fail(YC_ERR_ARG, "unknown CG variant");
-    if (variant == YC_CG_FUSED && c->nT > CG1_MAXP * BLK)
+    if (variant != YC_CG_REFERENCE && variant != YC_CG_FUSED && variant != YC_CG_PERSISTENT)
+        return fail(YC_ERR_ARG, "unknown CG variant");
+    if (variant != YC_CG_REFERENCE && c->nT > CG1_MAXP * BLK)
         return fail(YC_ERR_ARG, "fused CG supports up to 1024 matrix tiles (256 x 256 crossings)");
+    if (variant == YC_CG_PERSISTENT && !c->coop_ok)
+        return fail(YC_ERR_ARG, "persistent CG: the cloth's tiles do not fit on the device at once");
     c->cg_variant = variant;
     return YC_OK;
 }

@@ -123,6 +123,10 @@ int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_s
  *   agree with the reference form to the solve tolerance; counts may differ by rounding. */
 #define YC_CG_REFERENCE 0
 #define YC_CG_FUSED 1
+/* YC_CG_PERSISTENT: the fused recurrences in one cooperative launch per solve; each
+ * 8 x 8 tile's block keeps its rows of A in registers for the whole solve.  Needs every
+ * tile resident at once (YC_ERR_ARG otherwise, e.g. above about 128 x 128 crossings). */
+#define YC_CG_PERSISTENT 2
 int yc_set_cg_variant(yc_ctx* ctx, int variant);
 
 /* Runs `steps` SimSteps back to back on the device with zero-guess CG; returns the

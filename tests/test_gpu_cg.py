@@ -55,8 +55,11 @@ def test_tiled_assembly_matches_oracle(ys, cs, idx):
     g.close()
 
 
+VARIANTS = ["reference", "fused", "persistent"]
+
+
 @pytest.mark.parametrize("idx", range(3))
-@pytest.mark.parametrize("variant", ["reference", "fused"])
+@pytest.mark.parametrize("variant", VARIANTS)
 def test_cg_variants_match_oracle(ys, cs, idx, variant):
     name, spec = _specs(cs)[idx]
     say("case %s variant %s" % (name, variant))
@@ -80,7 +83,7 @@ def test_cg_variants_match_oracle(ys, cs, idx, variant):
     g.close()
 
 
-@pytest.mark.parametrize("variant", ["reference", "fused"])
+@pytest.mark.parametrize("variant", VARIANTS)
 def test_cg_variants_maxit_and_warm_start(ys, cs, variant):
     """maxit stops with the reference's count and residual; a start at the solution
     returns at once with zero iterations."""
@@ -106,10 +109,11 @@ def test_cg_variants_maxit_and_warm_start(ys, cs, variant):
     g.close()
 
 
-def test_fused_trajectory_100_steps(ys, cs):
+@pytest.mark.parametrize("variant", ["fused", "persistent"])
+def test_fused_trajectory_100_steps(ys, cs, variant):
     spec, state, ext, steps = cs.workload("homo5")
     g = ys.Cloth(spec)
-    g.set_cg_variant("fused")
+    g.set_cg_variant(variant)
     g.set_ClothState(state)
     r = OracleCloth(spec)
     r.set_state(state)
@@ -133,7 +137,7 @@ def test_fused_heter64_steps_match_reference_variant(ys, cs):
     to the solve tolerance and the iteration counts stay within 1 per step."""
     spec, state, ext, _ = cs.workload("heter64_twill")
     out = {}
-    for variant in ("reference", "fused"):
+    for variant in VARIANTS:
         g = ys.Cloth(spec)
         g.set_cg_variant(variant)
         g.set_ClothState(state)
@@ -144,8 +148,10 @@ def test_fused_heter64_steps_match_reference_variant(ys, cs):
             its.append(it)
         out[variant] = (g.get_ClothState(), its)
         g.close()
-    (sa, ia), (sb, ib) = out["reference"], out["fused"]
-    e = max(rel(sb[k], sa[k]) for k in ("LPos", "LVel", "EPos", "EVel"))
-    say("  iters ref %s fused %s |dstate| %.2e" % (ia, ib, e))
-    assert max(abs(a - b) for a, b in zip(ia, ib)) <= 1
-    assert e < 1e-7
+    sa, ia = out["reference"]
+    for v in VARIANTS[1:]:
+        sb, ib = out[v]
+        e = max(rel(sb[k], sa[k]) for k in ("LPos", "LVel", "EPos", "EVel"))
+        say("  iters ref %s %s %s |dstate| %.2e" % (ia, v, ib, e))
+        assert max(abs(a - b) for a, b in zip(ia, ib)) <= 1
+        assert e < 1e-7
