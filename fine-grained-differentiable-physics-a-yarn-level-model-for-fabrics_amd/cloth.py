@@ -177,6 +177,50 @@ class Cloth:
             _abi.check(rc)
         return tot.value, rc
 
+    # --- forward-mode tangents of SimStep (differentiation) ---
+    # parameter order of a direction (YC_P_* in include/yarnstep.h)
+    PARAMS = ("rho0", "rho1", "Y0", "Y1", "B0", "B1", "S", "mu", "kc", "kf", "df", "handle_k")
+    STATE_KEYS = ("LPos", "LPosFix", "LVel", "EPos", "EPosBar", "EVel")
+    GRAD_MODES = {"ref": 0, "exact": 1}
+
+    def tangent_begin(self, dparams, dstates=None):
+        """dparams: (ndir, 12) parameter tangents; dstates: list of ndir state dicts
+        (the get_ClothState keys) or None for zero state tangents."""
+        dp = np.ascontiguousarray(np.atleast_2d(dparams), dtype=np.float64)
+        if dp.shape[1] != len(self.PARAMS):
+            raise ValueError("expected %d parameter tangents per direction" % len(self.PARAMS))
+        ndir = dp.shape[0]
+        ds = _NULL
+        if dstates is not None:
+            flat = np.concatenate([np.concatenate([np.ascontiguousarray(st[k], dtype=np.float64).ravel()
+                                                   for k in self.STATE_KEYS]) for st in dstates])
+            self._dstate = flat
+            ds = _dp(flat)
+        self._dparam = dp
+        _abi.check(_abi.load_library().yc_tangent_begin(self._h, ndir, _dp(dp), ds))
+        self.ndir = ndir
+
+    def step_tangent(self, h, mode="exact", tol=1e-10, maxit=None):
+        """SimStep plus the tangents of all directions through it; returns (iters, relres, status)."""
+        if maxit is None:
+            maxit = 2 * self.n
+        info = _abi.YcStepInfo()
+        rc = _abi.load_library().yc_step_tangent(self._h, float(h), self.GRAD_MODES[mode], float(tol), int(maxit),
+                                                  ctypes.byref(info))
+        _abi.check(rc)
+        return info.cg_iters, info.cg_relres, rc
+
+    def tangent_get(self, d):
+        N = self.N
+        s = {k: np.zeros((N, 3)) for k in ("LPos", "LPosFix", "LVel")}
+        s.update({k: np.zeros((N, 2)) for k in ("EPos", "EPosBar", "EVel")})
+        _abi.check(_abi.load_library().yc_tangent_get(self._h, int(d), _dp(s["LPos"]), _dp(s["LPosFix"]), _dp(s["LVel"]),
+                                                      _dp(s["EPos"]), _dp(s["EPosBar"]), _dp(s["EVel"])))
+        return s
+
+    def tangent_end(self):
+        _abi.check(_abi.load_library().yc_tangent_end(self._h))
+
     # CG variants: "reference" (the reference solver's three-kernel order), "fused" (the
     # same recurrences with one kernel and one reduction per iteration) or "persistent"
     # (the fused recurrences in one cooperative launch per solve)

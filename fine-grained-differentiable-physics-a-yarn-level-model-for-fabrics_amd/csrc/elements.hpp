@@ -52,12 +52,26 @@ struct ClothDev {
     const double* uvbar;  // 2 planes
     const double* nrm;    // 3 planes (zero on the boundary)
     const unsigned char* flags;  // bit0 flip_norm, bit1 handle
+    // Tangent run (dd kernels only): directional derivatives of the state arrays (same
+    // layouts; null = zero) and of the material parameters, and the gradient mode.
+    const double *dx, *dv, *dxfix, *duv, *duvd, *duvbar, *dnrm;
+    double drho[2], dks[2], dkb[2], dkc, dkf, ddf, dmu, dS, dhandle_k;
+    int ref_mode;   // 1: the reference's detached quantities (non-jamming shear angle)
 };
 
-YS_DEV d3 pos(const ClothDev& C, int k) { return D3(C.x[k], C.x[C.N + k], C.x[2 * C.N + k]); }
-YS_DEV d3 vel(const ClothDev& C, int k) { return D3(C.v[k], C.v[C.N + k], C.v[2 * C.N + k]); }
-YS_DEV double uvc(const ClothDev& C, int k, int c) { return C.uv[c * C.N + k]; }
-YS_DEV double uvdc(const ClothDev& C, int k, int c) { return C.uvd[c * C.N + k]; }
+// A parameter or state value as scalar type T (dd carries its tangent).
+template <class T> YS_DEV T par(double v, double d);
+template <> YS_DEV double par<double>(double v, double) { return v; }
+template <> YS_DEV dd par<dd>(double v, double d) { return dd(v, d); }
+YS_DEV double tget(const double* p, int i) { return p ? p[i] : 0.0; }
+
+template <class T> YS_DEV v3<T> plane3(const double* a, const double* da, int N, int k) {
+    return V3<T>(par<T>(a[k], tget(da, k)), par<T>(a[N + k], tget(da, N + k)), par<T>(a[2 * N + k], tget(da, 2 * N + k)));
+}
+template <class T = double> YS_DEV v3<T> pos(const ClothDev& C, int k) { return plane3<T>(C.x, C.dx, C.N, k); }
+template <class T = double> YS_DEV v3<T> vel(const ClothDev& C, int k) { return plane3<T>(C.v, C.dv, C.N, k); }
+template <class T = double> YS_DEV T uvc(const ClothDev& C, int k, int c) { return par<T>(C.uv[c * C.N + k], tget(C.duv, c * C.N + k)); }
+template <class T = double> YS_DEV T uvdc(const ClothDev& C, int k, int c) { return par<T>(C.uvd[c * C.N + k], tget(C.duvd, c * C.N + k)); }
 YS_DEV bool interior(const ClothDev& C, int u, int v) { return u > 0 && u < C.W - 1 && v > 0 && v < C.Lg - 1; }
 YS_DEV bool flip(const ClothDev& C, int k) { return (C.flags[k] & 1) != 0; }
 YS_DEV int crossing(const ClothDev& C, int u, int v) { return (u - 1) + (v - 1) * (C.W - 2); }
@@ -71,12 +85,73 @@ YS_DEV int omp_owner(int n, int i) {
 
 // crimp copy of a node: x - R n on warp yarns, x + R n on weft yarns, x on the
 // boundary (src/Cloth.cpp:321-325)
-YS_DEV d3 crimp_pos(const ClothDev& C, int u, int v, int c) {
+template <class T> YS_DEV v3<T> crimp_pos(const ClothDev& C, int u, int v, int c) {
     int k = u + v * C.W;
-    d3 p = pos(C, k);
+    v3<T> p = pos<T>(C, k);
     if (!interior(C, u, v)) return p;
-    d3 n = D3(C.nrm[k], C.nrm[C.N + k], C.nrm[2 * C.N + k]);
+    v3<T> n = plane3<T>(C.nrm, C.dnrm, C.N, k);
     return c == 0 ? p - C.R * n : p + C.R * n;
+}
+
+// Slot of the entry in row component rc of a node, column component cc of the node at
+// offset (du, dv); -1 outside the stencil.
+__host__ __device__ inline int slot_of(int rc, int du, int dv, int cc) {
+    if (rc < 3) {
+        if (cc < 3) {
+            int o;
+            if (dv == 0 && du == 0) o = 0;
+            else if (dv == 0 && du == -1) o = 1;
+            else if (dv == 0 && du == 1) o = 2;
+            else if (du == 0 && dv == -1) o = 3;
+            else if (du == 0 && dv == 1) o = 4;
+            else if (dv == 0 && du == -2) o = 5;
+            else if (dv == 0 && du == 2) o = 6;
+            else if (du == 0 && dv == -2) o = 7;
+            else if (du == 0 && dv == 2) o = 8;
+            else if (du == -1 && dv == -1) o = 9;
+            else if (du == 1 && dv == -1) o = 10;
+            else if (du == -1 && dv == 1) o = 11;
+            else if (du == 1 && dv == 1) o = 12;
+            else return -1;
+            return rc * 45 + o * 3 + cc;
+        }
+        if (cc == 3 && dv == 0 && du >= -1 && du <= 1) return rc * 45 + SLOT_LE + du + 1;
+        if (cc == 4 && du == 0 && dv >= -1 && dv <= 1) return rc * 45 + SLOT_LE + 3 + dv + 1;
+        return -1;
+    }
+    if (rc == 3) {
+        if (dv != 0 || du < -1 || du > 1) return -1;
+        if (cc < 3) return SLOT_EU + (du + 1) * 3 + cc;
+        if (cc == 3) return SLOT_EU + 9 + du + 1;
+        return -1;
+    }
+    if (du != 0 || dv < -1 || dv > 1) return -1;
+    if (cc < 3) return SLOT_EV + (dv + 1) * 3 + cc;
+    if (cc == 4) return SLOT_EV + 9 + dv + 1;
+    return -1;
+}
+
+// Inverse of slot_of.
+__host__ __device__ inline void slot_decode(int s, int& rc, int& du, int& dv, int& cc) {
+    const int odu[13] = {0, -1, 1, 0, 0, -2, 2, 0, 0, -1, 1, -1, 1};
+    const int odv[13] = {0, 0, 0, -1, 1, 0, 0, -2, 2, -1, -1, 1, 1};
+    if (s < SLOT_EU) {
+        rc = s / 45;
+        int r = s - rc * 45;
+        if (r < SLOT_LE) { du = odu[r / 3]; dv = odv[r / 3]; cc = r % 3; }
+        else if (r < SLOT_LE + 3) { cc = 3; du = r - SLOT_LE - 1; dv = 0; }
+        else { cc = 4; du = 0; dv = r - SLOT_LE - 4; }
+    } else if (s < SLOT_EV) {
+        rc = 3;
+        int r = s - SLOT_EU;
+        dv = 0;
+        if (r < 9) { du = r / 3 - 1; cc = r % 3; } else { du = r - 10; cc = 3; }
+    } else {
+        rc = 4;
+        int r = s - SLOT_EV;
+        du = 0;
+        if (r < 9) { dv = r / 3 - 1; cc = r % 3; } else { dv = r - 10; cc = 4; }
+    }
 }
 
 enum Kind { KIND_M = 0, KIND_KX = 1, KIND_KV = 2 };
@@ -84,143 +159,127 @@ enum Kind { KIND_M = 0, KIND_KX = 1, KIND_KV = 2 };
 // ---------------------------------------------------------------------------
 // Row sink: accumulates the rows of node "me" of A, F, M v, K_v^T v and the
 // friction inputs.  A slots are read-modify-written in HBM (one owner per slot).
+// T = double: the step.  T = dd: its tangent; A's slots are not touched and the
+// tangents of the A entries go to dA when dA is set (exact mode).  With `detach`
+// (the reference's gradient), M v and K_v^T v take tangents through v only, as the
+// reference builds M and K_v outside the autograd graph (src/MatrixMp.cpp:46-98).
 // ---------------------------------------------------------------------------
-struct RowSink {
+template <class T>
+struct RowSinkT {
     const ClothDev* C;
     int me, mu_, mv_;
     bool inter;
     double* A;          // slot 0 of node "me"; slots TILE_N apart
+    double* dA;         // tangent run: slot 0 of node "me" in the tangent matrix, or null
+    bool detach;
     double coef[3];     // +1, -h^2, -h
-    double F[5], Mv[5], KvTv[5];
-    double KxEE[2], KvEE[2], FEpart[2];
-    d3 SBC_L;
-    d3 SBC_LE[2];
+    T F[5], Mv[5], KvTv[5];
+    T KxEE[2], KvEE[2], FEpart[2];
+    v3<T> SBC_L;
+    v3<T> SBC_LE[2];
     int fric_owner;
     int bad;
 
-    YS_DEV void init(const ClothDev* c, int k, double* a, double h) {
+    YS_DEV void init(const ClothDev* c, int k, double* a, double h, double* da = nullptr, bool det = false) {
         C = c; me = k; mu_ = k % c->W; mv_ = k / c->W; inter = interior(*c, mu_, mv_);
-        A = a + a_base(c->ntx, mu_, mv_); coef[0] = 1.0; coef[1] = -(h * h); coef[2] = -h;
-        for (int i = 0; i < 5; ++i) { F[i] = 0; Mv[i] = 0; KvTv[i] = 0; }
-        KxEE[0] = KxEE[1] = KvEE[0] = KvEE[1] = FEpart[0] = FEpart[1] = 0;
-        SBC_L = D3(0, 0, 0); SBC_LE[0] = SBC_L; SBC_LE[1] = SBC_L;
+        size_t base = a_base(c->ntx, mu_, mv_);
+        A = a ? a + base : nullptr; dA = da ? da + base : nullptr; detach = det;
+        coef[0] = 1.0; coef[1] = -(h * h); coef[2] = -h;
+        for (int i = 0; i < 5; ++i) { F[i] = T(0.0); Mv[i] = T(0.0); KvTv[i] = T(0.0); }
+        KxEE[0] = KxEE[1] = KvEE[0] = KvEE[1] = FEpart[0] = FEpart[1] = T(0.0);
+        SBC_L = V3<T>(T(0.0), T(0.0), T(0.0)); SBC_LE[0] = SBC_L; SBC_LE[1] = SBC_L;
         fric_owner = inter ? omp_owner(c->Nc, crossing(*c, mu_, mv_)) : -1;
         bad = 0;
     }
     YS_DEV int slot(int rc, int kcol, int cc) {
-        int du = kcol % C->W - mu_, dv = kcol / C->W - mv_;
-        if (rc < 3) {
-            if (cc < 3) {
-                int o;
-                if (dv == 0 && du == 0) o = 0;
-                else if (dv == 0 && du == -1) o = 1;
-                else if (dv == 0 && du == 1) o = 2;
-                else if (du == 0 && dv == -1) o = 3;
-                else if (du == 0 && dv == 1) o = 4;
-                else if (dv == 0 && du == -2) o = 5;
-                else if (dv == 0 && du == 2) o = 6;
-                else if (du == 0 && dv == -2) o = 7;
-                else if (du == 0 && dv == 2) o = 8;
-                else if (du == -1 && dv == -1) o = 9;
-                else if (du == 1 && dv == -1) o = 10;
-                else if (du == -1 && dv == 1) o = 11;
-                else if (du == 1 && dv == 1) o = 12;
-                else { bad = 1; return -1; }
-                return rc * 45 + o * 3 + cc;
-            }
-            if (cc == 3 && dv == 0 && du >= -1 && du <= 1) return rc * 45 + SLOT_LE + du + 1;
-            if (cc == 4 && du == 0 && dv >= -1 && dv <= 1) return rc * 45 + SLOT_LE + 3 + dv + 1;
-            bad = 1; return -1;
-        }
-        if (rc == 3) {
-            if (dv != 0 || du < -1 || du > 1) { bad = 1; return -1; }
-            if (cc < 3) return SLOT_EU + (du + 1) * 3 + cc;
-            if (cc == 3) return SLOT_EU + 9 + du + 1;
-            bad = 1; return -1;
-        }
-        if (du != 0 || dv < -1 || dv > 1) { bad = 1; return -1; }
-        if (cc < 3) return SLOT_EV + (dv + 1) * 3 + cc;
-        if (cc == 4) return SLOT_EV + 9 + dv + 1;
-        bad = 1; return -1;
+        int s = slot_of(rc, kcol % C->W - mu_, kcol / C->W - mv_, cc);
+        if (s < 0) bad = 1;
+        return s;
     }
-    YS_DEV void addA(int rc, int kcol, int cc, double val) {
+    YS_DEV void addA(int rc, int kcol, int cc, T x) {
         int s = slot(rc, kcol, cc);
-        if (s >= 0) A[(size_t)s * TILE_N] += val;
+        if (s < 0) return;
+        if (A) A[(size_t)s * TILE_N] += val(x);
+        if (dA) dA[(size_t)s * TILE_N] += tan_of(x);
     }
-    YS_DEV double velL(int k, int c) { return C->v[c * C->N + k]; }
-    YS_DEV double velE(int k, int c) { return C->uvd[c * C->N + k]; }
+    YS_DEV T velL(int k, int c) { return par<T>(C->v[c * C->N + k], tget(C->dv, c * C->N + k)); }
+    YS_DEV T velE(int k, int c) { return par<T>(C->uvd[c * C->N + k], tget(C->duvd, c * C->N + k)); }
+    // a block entry in a product with the velocity (value only when detached)
+    YS_DEV T blk(T b) const { return detach ? T(val(b)) : b; }
 
     // LL block B of (force node F, dof node D): A[3F+v, 3D+u] = B[u][v] (src/MatrixMp.cpp:65-76)
-    YS_DEV void LL(int kind, int kF, int kD, const m33& B) {
+    YS_DEV void LL(int kind, int kF, int kD, const mat3<T>& B) {
         if (kF == me) {
             for (int u = 0; u < 3; ++u)
                 for (int v = 0; v < 3; ++v) {
                     addA(v, kD, u, coef[kind] * B.m[u][v]);
-                    if (kind == KIND_M) Mv[v] += B.m[u][v] * velL(kD, u);
+                    if (kind == KIND_M) Mv[v] += blk(B.m[u][v]) * velL(kD, u);
                 }
         }
         if (kind == KIND_KV && kD == me)
             for (int u = 0; u < 3; ++u)
-                for (int v = 0; v < 3; ++v) KvTv[u] += B.m[u][v] * velL(kF, v);
+                for (int v = 0; v < 3; ++v) KvTv[u] += blk(B.m[u][v]) * velL(kF, v);
     }
     // LE: A[3kL+v, 3N+2e(kE)+c] = B[v]
-    YS_DEV void LE(int kind, int kL, int kE, int c, d3 B) {
-        double b[3] = {B.x, B.y, B.z};
+    YS_DEV void LE(int kind, int kL, int kE, int c, v3<T> B) {
+        T b[3] = {B.x, B.y, B.z};
         if (kL == me)
             for (int v = 0; v < 3; ++v) {
                 addA(v, kE, 3 + c, coef[kind] * b[v]);
-                if (kind == KIND_M) Mv[v] += b[v] * velE(kE, c);
+                if (kind == KIND_M) Mv[v] += blk(b[v]) * velE(kE, c);
             }
         if (kind == KIND_KV && kE == me)
-            for (int v = 0; v < 3; ++v) KvTv[3 + c] += b[v] * velL(kL, v);
+            for (int v = 0; v < 3; ++v) KvTv[3 + c] += blk(b[v]) * velL(kL, v);
     }
     // EL: A[3N+2e(kE)+c, 3kL+u] = B[u]
-    YS_DEV void EL(int kind, int kL, int kE, int c, d3 B) {
-        double b[3] = {B.x, B.y, B.z};
+    YS_DEV void EL(int kind, int kL, int kE, int c, v3<T> B) {
+        T b[3] = {B.x, B.y, B.z};
         if (kE == me)
             for (int u = 0; u < 3; ++u) {
                 addA(3 + c, kL, u, coef[kind] * b[u]);
-                if (kind == KIND_M) Mv[3 + c] += b[u] * velL(kL, u);
+                if (kind == KIND_M) Mv[3 + c] += blk(b[u]) * velL(kL, u);
             }
         if (kind == KIND_KV && kL == me)
-            for (int u = 0; u < 3; ++u) KvTv[u] += b[u] * velE(kE, c);
+            for (int u = 0; u < 3; ++u) KvTv[u] += blk(b[u]) * velE(kE, c);
     }
     // EE: A[3N+2e(kF)+c, 3N+2e(kD)+c] = val
-    YS_DEV void EE(int kind, int kF, int kD, int c, double val) {
+    YS_DEV void EE(int kind, int kF, int kD, int c, T v) {
         if (kF == me) {
-            addA(3 + c, kD, 3 + c, coef[kind] * val);
-            if (kind == KIND_M) Mv[3 + c] += val * velE(kD, c);
+            addA(3 + c, kD, 3 + c, coef[kind] * v);
+            if (kind == KIND_M) Mv[3 + c] += blk(v) * velE(kD, c);
             if (kD == me) {
-                if (kind == KIND_KX) KxEE[c] += val;
-                if (kind == KIND_KV) KvEE[c] += val;
+                if (kind == KIND_KX) KxEE[c] += v;
+                if (kind == KIND_KV) KvEE[c] += v;
             }
         }
-        if (kind == KIND_KV && kD == me) KvTv[3 + c] += val * velE(kF, c);
+        if (kind == KIND_KV && kD == me) KvTv[3 + c] += blk(v) * velE(kF, c);
     }
-    YS_DEV void FL(int k, d3 f) { if (k == me) { F[0] += f.x; F[1] += f.y; F[2] += f.z; } }
+    YS_DEV void FL(int k, v3<T> f) { if (k == me) { F[0] += f.x; F[1] += f.y; F[2] += f.z; } }
     // E force on the crossing of node k; elem_owner is the static owner thread of the
     // emitting element in its own loop, for the partial sum read by the friction
-    YS_DEV void FE(int k, int c, double f, int elem_owner) {
+    YS_DEV void FE(int k, int c, T f, int elem_owner) {
         if (k == me) {
             F[3 + c] += f;
             if (elem_owner == fric_owner) FEpart[c] += f;
         }
     }
-    YS_DEV void SBCL(int k, d3 f) { if (k == me) SBC_L = SBC_L + f; }
-    YS_DEV void SBCLE(int k, int c, d3 f) { if (k == me) SBC_LE[c] = SBC_LE[c] + f; }
+    YS_DEV void SBCL(int k, v3<T> f) { if (k == me) SBC_L = SBC_L + f; }
+    YS_DEV void SBCLE(int k, int c, v3<T> f) { if (k == me) SBC_LE[c] = SBC_LE[c] + f; }
 };
+typedef RowSinkT<double> RowSink;
 
 // The shared placement of an edge's blocks (src/PhysicsCloth.cpp:267-337, 1529-1599):
 // LL four blocks; LE/EL/EE self blocks of each interior end; cross blocks when both
 // ends are interior.  Names: LEab = dF_La/dE_b, ELab = dF_Ea/dL_b, EEab = dF_Ea/dE_b.
+template <class T>
 struct EdgeBlocks {
-    m33 LL00, LL01, LL10, LL11;
-    d3 LE00, LE01, LE10, LE11, EL00, EL01, EL10, EL11;
-    double EE00, EE01, EE10, EE11;
+    mat3<T> LL00, LL01, LL10, LL11;
+    v3<T> LE00, LE01, LE10, LE11, EL00, EL01, EL10, EL11;
+    T EE00, EE01, EE10, EE11;
 };
 
-template <class Sink>
-YS_DEV void emit_edge(Sink& S, int kind, int k0, int k1, bool i0, bool i1, int c, const EdgeBlocks& b, bool ll) {
+template <class T, class Sink>
+YS_DEV void emit_edge(Sink& S, int kind, int k0, int k1, bool i0, bool i1, int c, const EdgeBlocks<T>& b, bool ll) {
     if (ll) {
         S.LL(kind, k0, k0, b.LL00);
         S.LL(kind, k0, k1, b.LL01);
@@ -243,36 +302,38 @@ YS_DEV void emit_edge(Sink& S, int kind, int k0, int k1, bool i0, bool i1, int c
 // Edge: Mass, Gravity, Stretch, Inertia, MDotVDot, ParaColli
 // (src/PhysicsCloth.cpp:250-577, 989-2577; geometry src/ClothUnit.h:134-154)
 // ---------------------------------------------------------------------------
-template <class Sink>
+template <class T, class Sink>
 YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int yarn, int owner) {
+    typedef v3<T> V;
+    typedef mat3<T> M;
     int u1 = u0 + (c == 0), v1 = v0 + (c == 1);
     int k0 = u0 + v0 * C.W, k1 = u1 + v1 * C.W;
     bool i0 = interior(C, u0, v0), i1 = interior(C, u1, v1);
-    d3 x0 = pos(C, k0), x1 = pos(C, k1);
-    d3 q0 = vel(C, k0), q1 = vel(C, k1);
-    double s0 = uvdc(C, k0, c), s1 = uvdc(C, k1, c);
-    double D = uvc(C, k1, c) - uvc(C, k0, c);
-    d3 dx = x1 - x0;
-    d3 w = D3(dx.x / D, dx.y / D, dx.z / D);
-    double l = nrm(dx);
-    d3 d = D3(dx.x / l, dx.y / l, dx.z / l);
-    m33 P = projector(d);
-    double rho = C.rho[yarn], ks = C.ks[yarn];
-    double ww = dot(w, w);
-    double nw = nrm(w);
-    EdgeBlocks b;
+    V x0 = pos<T>(C, k0), x1 = pos<T>(C, k1);
+    V q0 = vel<T>(C, k0), q1 = vel<T>(C, k1);
+    T s0 = uvdc<T>(C, k0, c), s1 = uvdc<T>(C, k1, c);
+    T D = uvc<T>(C, k1, c) - uvc<T>(C, k0, c);
+    V dx = x1 - x0;
+    V w = V3<T>(dx.x / D, dx.y / D, dx.z / D);
+    T l = nrm(dx);
+    V d = V3<T>(dx.x / l, dx.y / l, dx.z / l);
+    M P = projector(d);
+    T rho = par<T>(C.rho[yarn], C.drho[yarn]), ks = par<T>(C.ks[yarn], C.dks[yarn]);
+    T ww = dot(w, w);
+    T nw = nrm(w);
+    EdgeBlocks<T> b;
 
     // two-node mass pattern [[2,1],[1,2]]: mixed velocities
-    d3 V0 = 2.0 * q0 + q1, V1 = q0 + 2.0 * q1;
-    double S0 = 2.0 * s0 + s1, S1 = s0 + 2.0 * s1;
-    d3 Pvs = S0 * q0 + S1 * q1;
-    double Sss = s0 * S0 + s1 * S1;
-    double Vvv = dot(q0, V0) + dot(q1, V1);
+    V V0 = 2.0 * q0 + q1, V1 = q0 + 2.0 * q1;
+    T S0 = 2.0 * s0 + s1, S1 = s0 + 2.0 * s1;
+    V Pvs = S0 * q0 + S1 * q1;
+    T Sss = s0 * S0 + s1 * S1;
+    T Vvv = dot(q0, V0) + dot(q1, V1);
 
     // --- Mass (src/PhysicsCloth.cpp:250-338) ---
     {
-        double cf = (1.0 / 6.0) * rho * D;
-        b.LL00 = I33(2.0 * cf); b.LL11 = b.LL00; b.LL01 = I33(cf); b.LL10 = b.LL01;
+        T cf = (1.0 / 6.0) * rho * D;
+        b.LL00 = I33T<T>(2.0 * cf); b.LL11 = b.LL00; b.LL01 = I33T<T>(cf); b.LL10 = b.LL01;
         b.LE00 = (-2.0 * cf) * w; b.LE11 = b.LE00; b.LE01 = (-1.0 * cf) * w; b.LE10 = b.LE01;
         b.EL00 = b.LE00; b.EL11 = b.LE00; b.EL01 = b.LE01; b.EL10 = b.LE01;
         b.EE00 = 2.0 * cf * ww; b.EE11 = b.EE00; b.EE01 = 0.5 * b.EE00; b.EE10 = b.EE01;
@@ -280,13 +341,13 @@ YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int 
     }
     // --- Gravity_F / Gravity_D (src/PhysicsCloth.cpp:340-425) ---
     {
-        d3 G = D3(C.G[0], C.G[1], C.G[2]);
-        d3 fl = (-0.5 * rho * D) * G;
+        V G = V3<T>(C.G[0], C.G[1], C.G[2]);
+        V fl = (-0.5 * rho * D) * G;
         S.FL(k0, fl); S.FL(k1, fl);
-        double gx = dot(G, x0 + x1);
+        T gx = dot(G, x0 + x1);
         if (i0) S.FE(k0, c, 0.5 * rho * gx, owner);
         if (i1) S.FE(k1, c, -0.5 * rho * gx, owner);
-        d3 g = (0.5 * rho) * G;
+        V g = (0.5 * rho) * G;
         if (i0) { S.EL(KIND_KX, k0, k0, c, g); S.LE(KIND_KX, k0, k0, c, g); }
         if (i1) { S.EL(KIND_KX, k1, k1, c, -g); S.LE(KIND_KX, k1, k1, c, -g); }
         if (i0 && i1) {
@@ -298,20 +359,20 @@ YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int 
     }
     // --- Stretch_F / Stretch_D (src/PhysicsCloth.cpp:427-577) ---
     {
-        d3 f = (ks * (nw - 1.0)) * d;
+        V f = (ks * (nw - 1.0)) * d;
         S.FL(k0, f); S.FL(k1, -f);
-        double sg = c == 0 ? 1.0 : -1.0;
+        T sg = c == 0 ? 1.0 : -1.0;
         S.SBCL(k0, sg * f); S.SBCL(k1, -sg * f);
-        double ew = 0.5 * ks * (pow(nw, 2.0) - 1.0);
+        T ew = 0.5 * ks * (pow(nw, 2.0) - 1.0);
         if (i0) S.FE(k0, c, -ew, owner);
         if (i1) S.FE(k1, c, ew, owner);
-        m33 K = (ks / l) * P - I33(ks / D);
+        M K = (ks / l) * P - I33T<T>(ks / D);
         b.LL00 = K; b.LL11 = K; b.LL01 = -K; b.LL10 = -K;
-        d3 le = (ks * (nw / D)) * d;
+        V le = (ks * (nw / D)) * d;
         b.LE00 = le; b.LE11 = le; b.LE01 = -le; b.LE10 = -le;
-        d3 el = (ks / D) * w;
+        V el = (ks / D) * w;
         b.EL00 = el; b.EL11 = el; b.EL01 = -el; b.EL10 = -el;
-        double ee = -ks * (pow(nw, 2.0) / D);
+        T ee = -ks * (pow(nw, 2.0) / D);
         b.EE00 = ee; b.EE11 = ee; b.EE01 = -ee; b.EE10 = -ee;
         emit_edge(S, KIND_KX, k0, k1, i0, i1, c, b, true);
         if (i0) S.SBCLE(k0, c, sg * le);
@@ -319,28 +380,28 @@ YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int 
     }
     // --- Inertia_F (src/PhysicsCloth.cpp:1071-1186): T_x = 1/2 qd^T dM/dx qd ---
     {
-        d3 Tx0 = (rho / 6.0) * (Pvs - Sss * w);
+        V Tx0 = (rho / 6.0) * (Pvs - Sss * w);
         S.FL(k0, Tx0); S.FL(k1, -Tx0);
-        double Tuv0 = (rho / 12.0) * (ww * Sss - Vvv);
+        T Tuv0 = (rho / 12.0) * (ww * Sss - Vvv);
         if (i0) S.FE(k0, c, Tuv0, owner);
         if (i1) S.FE(k1, c, -Tuv0, owner);
     }
     // --- Inertia_D_Pos (src/PhysicsCloth.cpp:1188-1600) ---
     {
-        double T = (rho / 6.0) * Sss / (D * D);
-        b.LL00 = I33(T); b.LL01 = I33(-T); b.LL10 = I33(-T); b.LL11 = I33(T);
-        d3 le = (-rho / (6.0 * D)) * (Pvs + Sss * w);
+        T Ti = (rho / 6.0) * Sss / (D * D);
+        b.LL00 = I33T<T>(Ti); b.LL01 = I33T<T>(-Ti); b.LL10 = I33T<T>(-Ti); b.LL11 = I33T<T>(Ti);
+        V le = (-rho / (6.0 * D)) * (Pvs + Sss * w);
         b.LE00 = le; b.LE01 = -le; b.LE10 = -le; b.LE11 = le;
-        d3 el = (rho / (6.0 * D)) * ((3.0 * Sss) * w - Pvs);
+        V el = (rho / (6.0 * D)) * ((3.0 * Sss) * w - Pvs);
         b.EL00 = el; b.EL01 = -el; b.EL10 = -el; b.EL11 = el;
-        double ee = (rho / 6.0) * (ww / D) * Sss;
+        T ee = (rho / 6.0) * (ww / D) * Sss;
         b.EE00 = ee; b.EE01 = -ee; b.EE10 = -ee; b.EE11 = ee;
         emit_edge(S, KIND_KX, k0, k1, i0, i1, c, b, true);
     }
     // --- Inertia_D_Vel (src/PhysicsCloth.cpp:1602-1901): entry (i,j) = (dM/dq_i qd)_j ---
     {
-        double r6 = rho / 6.0;
-        b.LL00 = I33(r6 * S0); b.LL01 = I33(r6 * S1); b.LL10 = I33(-r6 * S0); b.LL11 = I33(-r6 * S1);
+        T r6 = rho / 6.0;
+        b.LL00 = I33T<T>(r6 * S0); b.LL01 = I33T<T>(r6 * S1); b.LL10 = I33T<T>(-r6 * S0); b.LL11 = I33T<T>(-r6 * S1);
         b.LE00 = r6 * (V0 - (2.0 * S0) * w); b.LE01 = r6 * (V1 - (2.0 * S1) * w);
         b.LE10 = -b.LE00; b.LE11 = -b.LE01;
         b.EL00 = -r6 * V0; b.EL01 = -r6 * V1; b.EL10 = r6 * V0; b.EL11 = r6 * V1;
@@ -349,37 +410,37 @@ YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int 
     }
     // --- MDotVDot_F / _D_Pos / _D_Vel (src/PhysicsCloth.cpp:1903-2577) ---
     {
-        double Dd = s1 - s0;
-        d3 dq = q1 - q0;
-        d3 wd = (1.0 / (D * D)) * (D * dq - Dd * dx);
-        double wwd = dot(w, wd);
-        double r6 = rho / 6.0;
+        T Dd = s1 - s0;
+        V dq = q1 - q0;
+        V wd = (1.0 / (D * D)) * (D * dq - Dd * dx);
+        T wwd = dot(w, wd);
+        T r6 = rho / 6.0;
         // (Mdot qd) per node part a: L and E components
-        d3 MdL0 = (r6 * Dd) * (V0 - S0 * w) - (r6 * D * S0) * wd;
-        d3 MdL1 = (r6 * Dd) * (V1 - S1 * w) - (r6 * D * S1) * wd;
-        double MdE0 = r6 * Dd * (ww * S0 - dot(w, V0)) + r6 * D * (2.0 * wwd * S0 - dot(wd, V0));
-        double MdE1 = r6 * Dd * (ww * S1 - dot(w, V1)) + r6 * D * (2.0 * wwd * S1 - dot(wd, V1));
+        V MdL0 = (r6 * Dd) * (V0 - S0 * w) - (r6 * D * S0) * wd;
+        V MdL1 = (r6 * Dd) * (V1 - S1 * w) - (r6 * D * S1) * wd;
+        T MdE0 = r6 * Dd * (ww * S0 - dot(w, V0)) + r6 * D * (2.0 * wwd * S0 - dot(wd, V0));
+        T MdE1 = r6 * Dd * (ww * S1 - dot(w, V1)) + r6 * D * (2.0 * wwd * S1 - dot(wd, V1));
         S.FL(k0, -MdL0); S.FL(k1, -MdL1);
         if (i0) S.FE(k0, c, -MdE0, owner);
         if (i1) S.FE(k1, c, -MdE1, owner);
 
         // D_Pos (added as written): x-columns only feed E rows; uv-columns feed L and E rows
         {
-            double r3 = rho / 3.0;
-            b.LL00 = Z33(); b.LL01 = Z33(); b.LL10 = Z33(); b.LL11 = Z33();
+            T r3 = rho / 3.0;
+            b.LL00 = Z33T<T>(); b.LL01 = Z33T<T>(); b.LL10 = Z33T<T>(); b.LL11 = Z33T<T>();
             b.EL00 = (-r3 * S0) * wd; b.EL01 = (r3 * S0) * wd;
             b.EL10 = (-r3 * S1) * wd; b.EL11 = (r3 * S1) * wd;
-            d3 g = (1.0 / (D * D)) * dq - (4.0 * Dd / (D * D * D)) * dx;
-            double gw = dot(g, w);
+            V g = (1.0 / (D * D)) * dq - (4.0 * Dd / (D * D * D)) * dx;
+            T gw = dot(g, w);
             // column uv_end, sgn = -1 (end 0) / +1 (end 1)
             for (int end = 0; end < 2; ++end) {
-                double sg = end == 0 ? -1.0 : 1.0;
-                double wTw_dot_uv = 2.0 * sg * wwd / D - 2.0 * sg * gw;
-                d3 L0 = (r6 * Dd * sg * S0 / D) * w - r6 * MdL0 + (r6 * D * sg * S0) * g;
-                d3 L1 = (r6 * Dd * sg * S1 / D) * w - r6 * MdL1 + (r6 * D * sg * S1) * g;
-                double E0 = r6 * Dd * (sg * dot(w, V0) / D + 2.0 * sg * ww * S0 / D) - r6 * MdE0 +
+                T sg = end == 0 ? -1.0 : 1.0;
+                T wTw_dot_uv = 2.0 * sg * wwd / D - 2.0 * sg * gw;
+                V L0 = (r6 * Dd * sg * S0 / D) * w - r6 * MdL0 + (r6 * D * sg * S0) * g;
+                V L1 = (r6 * Dd * sg * S1 / D) * w - r6 * MdL1 + (r6 * D * sg * S1) * g;
+                T E0 = r6 * Dd * (sg * dot(w, V0) / D + 2.0 * sg * ww * S0 / D) - r6 * MdE0 +
                             r6 * D * (sg * dot(g, V0) + wTw_dot_uv * S0);
-                double E1 = r6 * Dd * (sg * dot(w, V1) / D + 2.0 * sg * ww * S1 / D) - r6 * MdE1 +
+                T E1 = r6 * Dd * (sg * dot(w, V1) / D + 2.0 * sg * ww * S1 / D) - r6 * MdE1 +
                             r6 * D * (sg * dot(g, V1) + wTw_dot_uv * S1);
                 if (end == 0) { b.LE00 = L0; b.LE10 = L1; b.EE00 = E0; b.EE10 = E1; }
                 else { b.LE01 = L0; b.LE11 = L1; b.EE01 = E0; b.EE11 = E1; }
@@ -388,19 +449,19 @@ YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int 
         }
         // D_Vel (subtracted as written)
         {
-            d3 WW = Dd * w + D * wd;
-            double EWq = Dd * ww + 2.0 * D * wwd;
+            V WW = Dd * w + D * wd;
+            T EWq = Dd * ww + 2.0 * D * wwd;
             const double K2[2][2] = {{2.0, 1.0}, {1.0, 2.0}};
-            const d3 Vb[2] = {V0, V1};
-            const double Sb[2] = {S0, S1};
-            m33 LLb[2][2];
-            d3 ELb[2][2], LEb[2][2];
-            double EEb[2][2];
+            const V Vb[2] = {V0, V1};
+            const T Sb[2] = {S0, S1};
+            M LLb[2][2];
+            V ELb[2][2], LEb[2][2];
+            T EEb[2][2];
             for (int ap = 0; ap < 2; ++ap)
                 for (int end = 0; end < 2; ++end) {
-                    double sg = end == 0 ? -1.0 : 1.0;
-                    double k2 = K2[ap][end];
-                    LLb[ap][end] = I33(-r6 * (-sg * Sb[ap] + k2 * Dd));
+                    T sg = end == 0 ? -1.0 : 1.0;
+                    T k2 = K2[ap][end];
+                    LLb[ap][end] = I33T<T>(-r6 * (-sg * Sb[ap] + k2 * Dd));
                     ELb[ap][end] = -r6 * ((-sg) * Vb[ap] + (2.0 * sg * Sb[ap]) * w - k2 * WW);
                     LEb[ap][end] = -r6 * (sg * Vb[ap] - k2 * WW);
                     EEb[ap][end] = -r6 * (-sg * ww * Sb[ap] + k2 * EWq);
@@ -414,12 +475,12 @@ YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int 
     }
     // --- ParaColli_F / _D (src/PhysicsCloth.cpp:989-1069): EE uses the value (Q6) ---
     {
-        double dd = flip(C, k0) != flip(C, k1) ? 4.0 * C.R : 2.0 * C.R;
-        double rl = dd - D > 0.0 ? dd - D : 0.0;
-        double Fu0 = C.kc * C.L * rl;
+        double dpc = flip(C, k0) != flip(C, k1) ? 4.0 * C.R : 2.0 * C.R;
+        T rl = dpc - D > 0.0 ? dpc - D : T(0.0);
+        T Fu0 = par<T>(C.kc, C.dkc) * C.L * rl;
         if (i0) S.FE(k0, c, Fu0, owner);
         if (i1) S.FE(k1, c, -Fu0, owner);
-        double val = -C.kc * C.L * rl;
+        T val = -par<T>(C.kc, C.dkc) * C.L * rl;
         if (i0) S.EE(KIND_KX, k0, k0, c, val);
         if (i1) S.EE(KIND_KX, k1, k1, c, val);
         if (i0 && i1) { S.EE(KIND_KX, k0, k1, c, val); S.EE(KIND_KX, k1, k0, c, val); }
@@ -429,34 +490,38 @@ YS_DEV void edge_element(const ClothDev& C, Sink& S, int u0, int v0, int c, int 
 // ---------------------------------------------------------------------------
 // Bending (src/PhysicsCloth.cpp:579-987; geometry src/ClothUnit.h:197-210)
 // ---------------------------------------------------------------------------
+template <class T>
 struct BendVals {
-    d3 Fx1, Fx2, Fx0;
-    double Fu2, Fu1;
-    m33 L11, L12, L21, L22;
-    d3 L1E1, L2E1, E1L1, E1L2;
-    double E1E1;
+    v3<T> Fx1, Fx2, Fx0;
+    T Fu2, Fu1;
+    mat3<T> L11, L12, L21, L22;
+    v3<T> L1E1, L2E1, E1L1, E1L2;
+    T E1E1;
 };
 
-YS_DEV BendVals bend_values(d3 x2, d3 x0, d3 x1, double D, double kb) {
-    BendVals o;
-    d3 a = x1 - x0, bb = x2 - x0;
-    double l1 = nrm(a), l2 = nrm(bb);
-    d3 d1 = D3(a.x / l1, a.y / l1, a.z / l1), d2 = D3(bb.x / l2, bb.y / l2, bb.z / l2);
-    m33 P1 = projector(d1), P2 = projector(d2);
-    double ct = -dot(d1, d2);
+template <class T>
+YS_DEV BendVals<T> bend_values(v3<T> x2, v3<T> x0, v3<T> x1, T D, T kb) {
+    typedef v3<T> V;
+    typedef mat3<T> M;
+    BendVals<T> o;
+    V a = x1 - x0, bb = x2 - x0;
+    T l1 = nrm(a), l2 = nrm(bb);
+    V d1 = V3<T>(a.x / l1, a.y / l1, a.z / l1), d2 = V3<T>(bb.x / l2, bb.y / l2, bb.z / l2);
+    M P1 = projector(d1), P2 = projector(d2);
+    T ct = -dot(d1, d2);
     ct = fmin(fmax(ct, -1.0), 1.0);   // the reference clamps to [-1+DBL_MIN, 1-DBL_MIN]
-    double th = acos(ct);
-    double s = sin(th), c = cos(th);
-    o.Fx1 = D3(0, 0, 0); o.Fx2 = o.Fx1;
-    o.L11 = Z33(); o.L12 = o.L11; o.L21 = o.L11; o.L22 = o.L11;
+    T th = acos(ct);
+    T s = sin(th), c = cos(th);
+    o.Fx1 = V3<T>(0, 0, 0); o.Fx2 = o.Fx1;
+    o.L11 = Z33T<T>(); o.L12 = o.L11; o.L21 = o.L11; o.L22 = o.L11;
     o.L1E1 = o.Fx1; o.L2E1 = o.Fx1; o.E1L1 = o.Fx1; o.E1L2 = o.Fx1;
     if (s != 0.0) {
-        d3 P1d2 = P1 * d2, P2d1 = P2 * d1;
+        V P1d2 = P1 * d2, P2d1 = P2 * d1;
         o.Fx1 = ((-2.0 * kb * th) / (l1 * D * s)) * P1d2;
         o.Fx2 = ((-2.0 * kb * th) / (l2 * D * s)) * P2d1;
-        double cs2 = c / pow(s, 2.0);
-        m33 A1 = outerP(P1d2, d2, P1);   // P1 d2 d2^T P1
-        m33 A2 = outerP(P2d1, d1, P2);
+        T cs2 = c / pow(s, 2.0);
+        M A1 = outerP(P1d2, d2, P1);   // P1 d2 d2^T P1
+        M A2 = outerP(P2d1, d1, P2);
         o.L11 = ((2.0 * kb) / (pow(l1, 2.0) * D * s)) *
                 (th * (outer(P1d2, d1) + cs2 * A1 + c * P1 + outerP(d1, d2, P1)) - (1.0 / s) * A1);
         o.L12 = ((-2.0 * kb) / (l1 * l2 * D * s)) *
@@ -465,7 +530,7 @@ YS_DEV BendVals bend_values(d3 x2, d3 x0, d3 x1, double D, double kb) {
                 ((th * (P2 - cs2 * outer(P2d1, d2)) + (1.0 / s) * outer(P2d1, d2)) * P1);
         o.L22 = ((2.0 * kb) / (pow(l2, 2.0) * D * s)) *
                 (th * (outer(P2d1, d2) + cs2 * A2 + c * P2 + outerP(d2, d1, P2)) - (1.0 / s) * A2);
-        double c1 = (2.0 * kb * th) / (l1 * pow(D, 2.0) * s), c2 = (2.0 * kb * th) / (l2 * pow(D, 2.0) * s);
+        T c1 = (2.0 * kb * th) / (l1 * pow(D, 2.0) * s), c2 = (2.0 * kb * th) / (l2 * pow(D, 2.0) * s);
         o.L1E1 = c1 * P1d2;
         o.L2E1 = c2 * P2d1;
         o.E1L1 = c1 * vTm(d2, P1);
@@ -479,29 +544,31 @@ YS_DEV BendVals bend_values(d3 x2, d3 x0, d3 x1, double D, double kb) {
 }
 
 // bend along yarn direction c centred on node (uc, vc): n2 = previous, n1 = next
-template <class Sink>
+template <class T, class Sink>
 YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int yarn, int owner) {
+    typedef v3<T> V;
+    typedef mat3<T> M;
     int du = c == 0 ? 1 : 0, dv = c == 1 ? 1 : 0;
     int u2 = uc - du, v2 = vc - dv, u1 = uc + du, v1 = vc + dv;
     int k2 = u2 + v2 * C.W, k0 = uc + vc * C.W, k1 = u1 + v1 * C.W;
     bool i2 = interior(C, u2, v2), i0 = interior(C, uc, vc), i1 = interior(C, u1, v1);
-    double kb = C.kb[yarn];
+    T kb = par<T>(C.kb[yarn], C.dkb[yarn]);
     // Bending_F / Bending_D on the crossing nodes
     {
-        double D = uvc(C, k1, c) - uvc(C, k2, c);
-        BendVals o = bend_values(pos(C, k2), pos(C, k0), pos(C, k1), D, kb);
+        T D = uvc<T>(C, k1, c) - uvc<T>(C, k2, c);
+        BendVals<T> o = bend_values<T>(pos<T>(C, k2), pos<T>(C, k0), pos<T>(C, k1), D, kb);
         S.FL(k2, o.Fx2); S.FL(k0, o.Fx0); S.FL(k1, o.Fx1);
         if (i2) S.FE(k2, c, o.Fu2, owner);
         if (i1) S.FE(k1, c, o.Fu1, owner);
-        m33 L10 = -(o.L11 + o.L12), L20 = -(o.L21 + o.L22);
-        m33 L01 = -(o.L11 + o.L21), L02 = -(o.L12 + o.L22);
-        m33 L00 = -(L10 + L20);
+        M L10 = -(o.L11 + o.L12), L20 = -(o.L21 + o.L22);
+        M L01 = -(o.L11 + o.L21), L02 = -(o.L12 + o.L22);
+        M L00 = -(L10 + L20);
         S.LL(KIND_KX, k2, k2, o.L22); S.LL(KIND_KX, k2, k0, L20); S.LL(KIND_KX, k2, k1, o.L21);
         S.LL(KIND_KX, k0, k2, L02); S.LL(KIND_KX, k0, k0, L00); S.LL(KIND_KX, k0, k1, L01);
         S.LL(KIND_KX, k1, k2, o.L12); S.LL(KIND_KX, k1, k0, L10); S.LL(KIND_KX, k1, k1, o.L11);
-        d3 L2E2 = -o.L2E1, E2L2 = -o.E1L2;
-        d3 L0E1 = -(o.L1E1 + o.L2E1), L0E2 = -L0E1;
-        d3 E1L0 = -(o.E1L1 + o.E1L2), E2L0 = -E1L0;
+        V L2E2 = -o.L2E1, E2L2 = -o.E1L2;
+        V L0E1 = -(o.L1E1 + o.L2E1), L0E2 = -L0E1;
+        V E1L0 = -(o.E1L1 + o.E1L2), E2L0 = -E1L0;
         if (i2) { S.LE(KIND_KX, k2, k2, c, L2E2); S.EL(KIND_KX, k2, k2, c, E2L2); S.EE(KIND_KX, k2, k2, c, o.E1E1); }
         if (i1) { S.LE(KIND_KX, k1, k1, c, o.L1E1); S.EL(KIND_KX, k1, k1, c, o.E1L1); S.EE(KIND_KX, k1, k1, c, o.E1E1); }
         if (i2 && i0) { S.LE(KIND_KX, k0, k2, c, L0E2); S.EL(KIND_KX, k2, k0, c, E2L0); }   // (Q8 placement)
@@ -510,9 +577,9 @@ YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int 
     // Crimp_Bending_F / _D on the +-R n copies: compression inputs only (src/PhysicsCloth.cpp:824-987);
     // the copies keep their construction-time Eulerian coordinates (u L, v L)
     {
-        double D = c == 0 ? (double)u1 * C.L - (double)u2 * C.L : (double)v1 * C.L - (double)v2 * C.L;
-        BendVals o = bend_values(crimp_pos(C, u2, v2, c), crimp_pos(C, uc, vc, c), crimp_pos(C, u1, v1, c), D, kb);
-        double sg = c == 0 ? 1.0 : -1.0;
+        T D = c == 0 ? (double)u1 * C.L - (double)u2 * C.L : (double)v1 * C.L - (double)v2 * C.L;
+        BendVals<T> o = bend_values<T>(crimp_pos<T>(C, u2, v2, c), crimp_pos<T>(C, uc, vc, c), crimp_pos<T>(C, u1, v1, c), D, kb);
+        T sg = c == 0 ? 1.0 : -1.0;
         S.SBCL(k2, sg * o.Fx2); S.SBCL(k0, sg * o.Fx0); S.SBCL(k1, sg * o.Fx1);
         if (i2) S.SBCLE(k2, c, sg * (-o.L2E1));
         if (i1) S.SBCLE(k1, c, sg * o.L1E1);
@@ -522,19 +589,20 @@ YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int 
 // ---------------------------------------------------------------------------
 // Wind_F on one face (src/PhysicsCloth.cpp:3091-3106; src/ClothUnit.h:268-273)
 // ---------------------------------------------------------------------------
-template <class Sink>
+template <class T, class Sink>
 YS_DEV void face_element(const ClothDev& C, Sink& S, int ka, int kb2, int kc) {
-    d3 xa = pos(C, ka), xb = pos(C, kb2), xc = pos(C, kc);
-    d3 cr = cross(xb - xa, xc - xa);
-    double cn = nrm(cr);
-    double area = 0.5 * cn;
-    d3 n = cn == 0.0 ? cr : D3(cr.x / cn, cr.y / cn, cr.z / cn);
-    d3 fv = (vel(C, ka) + vel(C, kb2) + vel(C, kc)) / 3.0;
-    d3 rel = D3(C.wind_vel[0], C.wind_vel[1], C.wind_vel[2]) - fv;
-    double vn = dot(n, rel);
-    d3 vt = rel - vn * n;
-    d3 wf = (C.wind_density * area * fabs(vn) * vn) * n + (C.wind_drag * area) * vt;
-    d3 w3 = wf / 3.0;
+    typedef v3<T> V;
+    V xa = pos<T>(C, ka), xb = pos<T>(C, kb2), xc = pos<T>(C, kc);
+    V cr = cross(xb - xa, xc - xa);
+    T cn = nrm(cr);
+    T area = 0.5 * cn;
+    V n = cn == 0.0 ? cr : V3<T>(cr.x / cn, cr.y / cn, cr.z / cn);
+    V fv = (vel<T>(C, ka) + vel<T>(C, kb2) + vel<T>(C, kc)) / 3.0;
+    V rel = V3<T>(C.wind_vel[0], C.wind_vel[1], C.wind_vel[2]) - fv;
+    T vn = dot(n, rel);
+    V vt = rel - vn * n;
+    V wf = (C.wind_density * area * fabs(vn) * vn) * n + (C.wind_drag * area) * vt;
+    V w3 = wf / 3.0;
     S.FL(ka, w3); S.FL(kb2, w3); S.FL(kc, w3);
 }
 
@@ -542,27 +610,29 @@ YS_DEV void face_element(const ClothDev& C, Sink& S, int ka, int kb2, int kc) {
 // ConSlideFriction at the thread's own crossing (src/PhysicsCloth.cpp:2579-2641),
 // run after every edge / bend of the node has been accumulated.  Returns comp.
 // ---------------------------------------------------------------------------
-YS_DEV double friction(const ClothDev& C, RowSink& S, double* is_slide) {
+template <class T>
+YS_DEV T friction(const ClothDev& C, RowSinkT<T>& S, T* is_slide) {
+    typedef v3<T> V;
     int k = S.me;
-    d3 n = D3(C.nrm[k], C.nrm[C.N + k], C.nrm[2 * C.N + k]);
-    double comp = 0.5 * dot(n, S.SBC_L);
+    V n = plane3<T>(C.nrm, C.dnrm, C.N, k);
+    T comp = 0.5 * dot(n, S.SBC_L);
     comp = comp > 0.0 ? comp : 0.0;
-    double lim = comp * C.mu;
+    T lim = comp * par<T>(C.mu, C.dmu);
     for (int c = 0; c < 2; ++c) {
-        double dl = uvc(C, k, c) - C.uvbar[c * C.N + k];
-        double cEE = 0.5 * dot(n, S.SBC_LE[c]);
-        double FNS = S.F[3 + c];          // F_E before friction (F_E_No_Slide)
-        double Fpt = S.FEpart[c];         // thread-partial F_Es[ID] (Q2)
-        double th = tanh(10.0 * dl);
+        T dl = uvc<T>(C, k, c) - par<T>(C.uvbar[c * C.N + k], tget(C.duvbar, c * C.N + k));
+        T cEE = 0.5 * dot(n, S.SBC_LE[c]);
+        T FNS = S.F[3 + c];          // F_E before friction (F_E_No_Slide)
+        T Fpt = S.FEpart[c];         // thread-partial F_Es[ID] (Q2)
+        T th = tanh(10.0 * dl);
         is_slide[c] = 1.0 / (1.0 + exp(-(1e4 * (fabs(FNS) - lim))));
-        double fric = (-0.5 * (C.kf * dl - th * comp * C.mu) * tanh(10.0 * (lim - Fpt)) +
-                       0.5 * (C.kf * dl + th * comp * C.mu)) - C.df * uvdc(C, k, c);
-        double tl = tanh(10.0 * (lim - FNS));
-        double fpos = (-0.5 * (C.kf - (1.0 - pow(th, 2.0)) * C.mu * comp - th * C.mu * cEE)) * tl -
-                      (0.5 * (C.kf * dl - th * C.mu * comp)) * (1.0 - pow(tl, 2.0)) * (S.KxEE[c] - C.mu * cEE) -
-                      (0.5 * (C.kf + (1.0 - pow(th, 2.0)) * C.mu * comp + th * C.mu * cEE));
-        double tq = tanh(10.0 * lim - FNS);   // (10*fric_lim - F)[c] (Q3)
-        double fvel = (-0.5 * (C.kf * dl - th * C.mu * comp)) * (1.0 - pow(tq, 2.0)) * S.KvEE[c] - C.df;
+        T fric = (-0.5 * (par<T>(C.kf, C.dkf) * dl - th * comp * par<T>(C.mu, C.dmu)) * tanh(10.0 * (lim - Fpt)) +
+                       0.5 * (par<T>(C.kf, C.dkf) * dl + th * comp * par<T>(C.mu, C.dmu))) - par<T>(C.df, C.ddf) * uvdc<T>(C, k, c);
+        T tl = tanh(10.0 * (lim - FNS));
+        T fpos = (-0.5 * (par<T>(C.kf, C.dkf) - (1.0 - pow(th, 2.0)) * par<T>(C.mu, C.dmu) * comp - th * par<T>(C.mu, C.dmu) * cEE)) * tl -
+                      (0.5 * (par<T>(C.kf, C.dkf) * dl - th * par<T>(C.mu, C.dmu) * comp)) * (1.0 - pow(tl, 2.0)) * (S.KxEE[c] - par<T>(C.mu, C.dmu) * cEE) -
+                      (0.5 * (par<T>(C.kf, C.dkf) + (1.0 - pow(th, 2.0)) * par<T>(C.mu, C.dmu) * comp + th * par<T>(C.mu, C.dmu) * cEE));
+        T tq = tanh(10.0 * lim - FNS);   // (10*fric_lim - F)[c] (Q3)
+        T fvel = (-0.5 * (par<T>(C.kf, C.dkf) * dl - th * par<T>(C.mu, C.dmu) * comp)) * (1.0 - pow(tq, 2.0)) * S.KvEE[c] - par<T>(C.df, C.ddf);
         S.F[3 + c] += fric;
         S.EE(KIND_KX, k, k, c, fpos);
         S.EE(KIND_KV, k, k, c, fvel);

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "../../include/yarnstep.h"
 #include "elements.hpp"
@@ -162,6 +163,10 @@ struct AsmOut {
     const double* ext;  // global-layout n vector or null
     int* flag;          // [0] structural error, [1] non-finite
     int Wp, P;          // padded width, plane stride
+    // tangent run: tangents of A (exact mode, else null), F, b, is_slide and comp, and
+    // the comp tangent read by the shear; detach = the reference's gradient
+    double *dA, *dF, *db, *dis_slide, *dcomp;
+    int detach;
 };
 
 __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
@@ -174,34 +179,42 @@ __device__ __forceinline__ int tile_node(const ClothDev& C) {
     return (u < C.W && v < C.Lg) ? u + v * C.W : -1;
 }
 
+// T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
+// src/TrainHelper.cpp:19-20).  T = dd: their directional derivatives along the
+// tangents held in C, into the d* outputs of O.
+template <class T>
 __global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double h) {
+    typedef v3<T> V;
+    constexpr bool tan = !std::is_same<T, double>::value;
     int k = tile_node(C);
     if (k < 0) return;
-    RowSink S;
-    S.init(&C, k, O.A, h);
+    RowSinkT<T> S;
+    S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
     int u = S.mu_, v = S.mv_;
-    for (int s = 0; s < NSLOT; ++s) S.A[(size_t)s * TILE_N] = 0.0;
+    if (S.A) for (int s = 0; s < NSLOT; ++s) S.A[(size_t)s * TILE_N] = 0.0;
+    if (S.dA) for (int s = 0; s < NSLOT; ++s) S.dA[(size_t)s * TILE_N] = 0.0;
     // handle (src/PhysicsCloth.cpp:232-248)
     if (C.flags[k] & 2) {
-        d3 xf = D3(C.xfix[k], C.xfix[C.N + k], C.xfix[2 * C.N + k]);
-        S.FL(k, (-C.handle_k) * (pos(C, k) - xf));
-        S.LL(KIND_KX, k, k, I33(-C.handle_k));
+        V xf = plane3<T>(C.xfix, C.dxfix, C.N, k);
+        T hk = par<T>(C.handle_k, C.dhandle_k);
+        S.FL(k, (-hk) * (pos<T>(C, k) - xf));
+        S.LL(KIND_KX, k, k, I33T<T>(-hk));
     }
     // edges (list order: warp rows, then weft columns; src/Cloth.cpp:190-228)
     int ne_warp = C.Lg * (C.W - 1);
-    if (u > 0) edge_element(C, S, u - 1, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u - 1));
-    if (u < C.W - 1) edge_element(C, S, u, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u));
-    if (v > 0) edge_element(C, S, u, v - 1, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v - 1));
-    if (v < C.Lg - 1) edge_element(C, S, u, v, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v));
+    if (u > 0) edge_element<T>(C, S, u - 1, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u - 1));
+    if (u < C.W - 1) edge_element<T>(C, S, u, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u));
+    if (v > 0) edge_element<T>(C, S, u, v - 1, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v - 1));
+    if (v < C.Lg - 1) edge_element<T>(C, S, u, v, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v));
     // bends centred on u-1, u, u+1 (warp) and v-1, v, v+1 (weft) (src/Cloth.cpp:230-270)
     int nb_warp = C.Lg * (C.W - 2);
     for (int d = -1; d <= 1; ++d) {
         int uc = u + d;
-        if (uc >= 1 && uc <= C.W - 2) bend_element(C, S, uc, v, 0, v & 1, omp_owner(C.n_bends, v * (C.W - 2) + uc - 1));
+        if (uc >= 1 && uc <= C.W - 2) bend_element<T>(C, S, uc, v, 0, v & 1, omp_owner(C.n_bends, v * (C.W - 2) + uc - 1));
     }
     for (int d = -1; d <= 1; ++d) {
         int vc = v + d;
-        if (vc >= 1 && vc <= C.Lg - 2) bend_element(C, S, u, vc, 1, u & 1, omp_owner(C.n_bends, nb_warp + u * (C.Lg - 2) + vc - 1));
+        if (vc >= 1 && vc <= C.Lg - 2) bend_element<T>(C, S, u, vc, 1, u & 1, omp_owner(C.n_bends, nb_warp + u * (C.Lg - 2) + vc - 1));
     }
     // faces of the up to four cells around the node (src/Cloth.cpp:272-286)
     if (C.wind_density != 0.0 || C.wind_drag != 0.0) {
@@ -213,28 +226,38 @@ __global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double
                 if (((cu + cv) & 1) == 0) { f[0][0] = tl; f[0][1] = tr; f[0][2] = br; f[1][0] = tl; f[1][1] = br; f[1][2] = bl; }
                 else { f[0][0] = tl; f[0][1] = tr; f[0][2] = bl; f[1][0] = tr; f[1][1] = br; f[1][2] = bl; }
                 for (int q = 0; q < 2; ++q)
-                    if (f[q][0] == k || f[q][1] == k || f[q][2] == k) face_element(C, S, f[q][0], f[q][1], f[q][2]);
+                    if (f[q][0] == k || f[q][1] == k || f[q][2] == k) face_element<T>(C, S, f[q][0], f[q][1], f[q][2]);
             }
     }
     // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
-    double comp = 0.0, sl[2] = {0.0, 0.0};
-    if (S.inter) comp = friction(C, S, sl);
-    // outputs
-    for (int c = 0; c < 5; ++c) O.F[c * C.N + k] = S.F[c];
-    O.comp[k] = comp;
-    O.is_slide[k] = sl[0]; O.is_slide[C.N + k] = sl[1];
-    O.fepart[k] = S.FEpart[0]; O.fepart[C.N + k] = S.FEpart[1];
+    T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
+    if (S.inter) comp = friction<T>(C, S, sl);
     // b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:20, 63)
     int pk = padidx(O.Wp, u, v);
     int e = S.inter ? crossing(C, u, v) : 0;
-    for (int c = 0; c < 5; ++c) {
-        double bv = 0.0;
-        if (c < 3 || S.inter) {
-            double ext = 0.0;
-            if (O.ext) ext = c < 3 ? O.ext[3 * k + c] : O.ext[3 * C.N + 2 * e + (c - 3)];
-            bv = h * (S.F[c] + ext - S.KvTv[c]) + S.Mv[c];
+    if (!tan) {
+        for (int c = 0; c < 5; ++c) O.F[c * C.N + k] = val(S.F[c]);
+        O.comp[k] = val(comp);
+        O.is_slide[k] = val(sl[0]); O.is_slide[C.N + k] = val(sl[1]);
+        O.fepart[k] = val(S.FEpart[0]); O.fepart[C.N + k] = val(S.FEpart[1]);
+        for (int c = 0; c < 5; ++c) {
+            double bv = 0.0;
+            if (c < 3 || S.inter) {
+                double ext = 0.0;
+                if (O.ext) ext = c < 3 ? O.ext[3 * k + c] : O.ext[3 * C.N + 2 * e + (c - 3)];
+                bv = h * (val(S.F[c]) + ext - val(S.KvTv[c])) + val(S.Mv[c]);
+            }
+            O.b[c * O.P + pk] = bv;
         }
-        O.b[c * O.P + pk] = bv;
+    } else {
+        for (int c = 0; c < 5; ++c) O.dF[c * C.N + k] = tan_of(S.F[c]);
+        O.dcomp[k] = tan_of(comp);
+        O.dis_slide[k] = tan_of(sl[0]); O.dis_slide[C.N + k] = tan_of(sl[1]);
+        for (int c = 0; c < 5; ++c) {
+            double bv = 0.0;   // f_ext is a constant input
+            if (c < 3 || S.inter) bv = h * (tan_of(S.F[c]) - tan_of(S.KvTv[c])) + tan_of(S.Mv[c]);
+            O.db[c * O.P + pk] = bv;
+        }
     }
     if (S.bad) atomicOr(&O.flag[0], 1);
 }
@@ -242,14 +265,17 @@ __global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double
 // Shear segments touching the node (4 own, 2 each as warp neighbour of the left and
 // right crossings, 2 each as weft neighbour of the top and bottom crossings), then the
 // Jacobi diagonal and the NaN/Inf screen of the node's rows.
-template <class Sink>
-__device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp);
+template <class T, class Sink>
+__device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp,
+                              const double* dcomp);
 
+template <class T>
 __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double h) {
+    constexpr bool tan = !std::is_same<T, double>::value;
     int k = tile_node(C);
     if (k < 0) return;
-    RowSink S;
-    S.init(&C, k, O.A, h);
+    RowSinkT<T> S;
+    S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
     int u = S.mu_, v = S.mv_;
     auto K = [&](int uu, int vv) { return uu + vv * C.W; };
     // shear_segs order per crossing: (left,bot), (left,top), (right,bot), (right,top)
@@ -264,13 +290,21 @@ __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double
                 for (int bq = 0; bq < 2; ++bq) {
                     int k1 = nb1[a], k3 = nb3[bq];
                     if (kc != k && k1 != k && k3 != k) continue;
-                    shear_element(C, S, kc, k1, k3, O.comp);
+                    shear_element<T>(C, S, kc, k1, k3, O.comp, tan ? O.dcomp : nullptr);
                 }
         }
     int pk = padidx(O.Wp, u, v);
+    if (tan) {
+        for (int c = 0; c < 3; ++c) {
+            O.dF[c * C.N + k] += tan_of(S.F[c]);
+            O.db[c * O.P + pk] += h * tan_of(S.F[c]);
+        }
+        if (S.bad) atomicOr(&O.flag[0], 1);
+        return;
+    }
     for (int c = 0; c < 3; ++c) {
-        O.F[c * C.N + k] += S.F[c];
-        O.b[c * O.P + pk] += h * S.F[c];
+        O.F[c * C.N + k] += val(S.F[c]);
+        O.b[c * O.P + pk] += h * val(S.F[c]);
     }
     // Jacobi diagonal (the reference solver's DiagonalPreconditioner: 1/a_ii, or 1 if zero)
     bool bad = false;
@@ -1030,14 +1064,15 @@ __global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, 
 // Cloth::NodeNorm (src/Cloth.cpp:440-463): smallest-eigenvalue eigenvector of the 3x3
 // scatter matrix of the 5-point cross (cyclic Jacobi), aligned with
 // (x_right - x_left) x (x_bot - x_top), flipped by the weave pattern.
-__global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= s.N) return;
-    int u = k % s.W, v = k / s.W;
-    if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
-        for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = 0.0;
-        return;
-    }
+// Eigen-decomposition of the 3x3 scatter matrix of the 5-point cross at interior node
+// k: eigenvalues lam, eigenvectors as the columns of V, the index im of the smallest,
+// and the sign sg that the reference applies to that eigenvector.
+struct NodeFrame {
+    double lam[3], V[3][3], d[5][3], sg;
+    int im;
+};
+
+__device__ void node_frame(const StateDev& s, int k, NodeFrame& f) {
     int ids[5] = {k, k - 1, k + 1, k - s.W, k + s.W};
     double p[5][3];
     for (int i = 0; i < 5; ++i) for (int c = 0; c < 3; ++c) p[i][c] = s.x[c * s.N + ids[i]];
@@ -1046,10 +1081,11 @@ __global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
     for (int c = 0; c < 3; ++c) m[c] = m[c] * (1.0 / 5.0);
     double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int i = 0; i < 5; ++i) {
-        double d[3] = {p[i][0] - m[0], p[i][1] - m[1], p[i][2] - m[2]};
-        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) A[a][b] += d[a] * d[b];
+        for (int c = 0; c < 3; ++c) f.d[i][c] = p[i][c] - m[c];
+        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) A[a][b] += f.d[i][a] * f.d[i][b];
     }
-    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double (&V)[3][3] = f.V;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) V[i][j] = i == j ? 1.0 : 0.0;
     for (int sweep = 0; sweep < 64; ++sweep) {
         double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
         double tr = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
@@ -1065,8 +1101,10 @@ __global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
                 for (int r = 0; r < 3; ++r) { double a0 = V[r][pp], a1 = V[r][q]; V[r][pp] = cs * a0 - sn * a1; V[r][q] = sn * a0 + cs * a1; }
             }
     }
+    for (int i = 0; i < 3; ++i) f.lam[i] = A[i][i];
     int im = 0;
     for (int i = 1; i < 3; ++i) if (A[i][i] < A[im][im]) im = i;
+    f.im = im;
     double n[3] = {V[0][im], V[1][im], V[2][im]};
     double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     for (int c = 0; c < 3; ++c) n[c] = n[c] / nn;
@@ -1078,7 +1116,116 @@ __global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
     double cosv = (cr[0] * n[0] + cr[1] * n[1] + cr[2] * n[2]) / (cn * nn2);
     double sg = cosv < 0 ? -1.0 : 1.0;
     if (s.flags[k] & 1) sg = -sg;
-    for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = sg * n[c];
+    f.sg = sg;
+}
+
+__global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.N) return;
+    int u = k % s.W, v = k / s.W;
+    if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
+        for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = 0.0;
+        return;
+    }
+    NodeFrame f;
+    node_frame(s, k, f);
+    // the eigenvector normalised, then signed
+    double n[3] = {f.V[0][f.im], f.V[1][f.im], f.V[2][f.im]};
+    double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = f.sg * (n[c] / nn);
+}
+
+// Tangent of the crossing normals along the position tangent dx: first-order
+// perturbation of the smallest eigenvector of the scatter matrix,
+// dn = sum_{j != min} e_j (e_j^T dC e_min) / (lam_min - lam_j), signed as n (the sign
+// is locally constant).  This is the derivative the reference's autograd takes through
+// torch::svd (src/Cloth.cpp:446) when the singular values are distinct.
+__global__ void __launch_bounds__(NBLK) k_normals_tan(StateDev s, const double* dx, double* dnrm) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.N) return;
+    int u = k % s.W, v = k / s.W;
+    if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
+        for (int c = 0; c < 3; ++c) dnrm[c * s.N + k] = 0.0;
+        return;
+    }
+    NodeFrame f;
+    node_frame(s, k, f);
+    int ids[5] = {k, k - 1, k + 1, k - s.W, k + s.W};
+    double dp[5][3], dm[3] = {0, 0, 0};
+    for (int i = 0; i < 5; ++i) for (int c = 0; c < 3; ++c) { dp[i][c] = dx[c * s.N + ids[i]]; dm[c] += dp[i][c]; }
+    for (int c = 0; c < 3; ++c) dm[c] = dm[c] * (1.0 / 5.0);
+    double dC[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int i = 0; i < 5; ++i) {
+        double dd_[3] = {dp[i][0] - dm[0], dp[i][1] - dm[1], dp[i][2] - dm[2]};
+        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) dC[a][b] += dd_[a] * f.d[i][b] + f.d[i][a] * dd_[b];
+    }
+    // unit eigenvectors
+    double E[3][3];
+    for (int j = 0; j < 3; ++j) {
+        double nn = sqrt(f.V[0][j] * f.V[0][j] + f.V[1][j] * f.V[1][j] + f.V[2][j] * f.V[2][j]);
+        for (int r = 0; r < 3; ++r) E[r][j] = f.V[r][j] / nn;
+    }
+    int im = f.im;
+    double Ce[3];
+    for (int a = 0; a < 3; ++a) Ce[a] = dC[a][0] * E[0][im] + dC[a][1] * E[1][im] + dC[a][2] * E[2][im];
+    double dn[3] = {0, 0, 0};
+    for (int j = 0; j < 3; ++j) {
+        if (j == im) continue;
+        double cj = (E[0][j] * Ce[0] + E[1][j] * Ce[1] + E[2][j] * Ce[2]) / (f.lam[im] - f.lam[j]);
+        for (int r = 0; r < 3; ++r) dn[r] += cj * E[r][j];
+    }
+    for (int c = 0; c < 3; ++c) dnrm[c * s.N + k] = f.sg * dn[c];
+}
+
+// Tangent of Cloth::update (src/Cloth.cpp:111-148): with the solution x_sol and its
+// tangent dsol, v' = x_sol[L], x' = x + h v', x_fix' = x' off the handles,
+// u' = x_sol[E], uv' = uv + h u', uvbar' = uvbar + h is_slide u'.
+struct TanState {
+    double *dx, *dv, *dxfix, *duv, *duvd, *duvbar;
+};
+
+__global__ void __launch_bounds__(NBLK) k_update_tan(StateDev s, TanState t, const double* sol, const double* dsol,
+                                                    const double* is_slide, const double* dis_slide, int Wp, int P, double h) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.N) return;
+    int u = k % s.W, v = k / s.W;
+    int pk = padidx(Wp, u, v);
+    bool fixed = (s.flags[k] & 2) != 0;
+    for (int c = 0; c < 3; ++c) {
+        double dnv = dsol[c * P + pk];
+        t.dv[c * s.N + k] = dnv;
+        double dnx = t.dx[c * s.N + k] + dnv * h;
+        t.dx[c * s.N + k] = dnx;
+        if (!fixed) t.dxfix[c * s.N + k] = dnx;
+    }
+    if (u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1) {
+        for (int c = 0; c < 2; ++c) {
+            double nv = sol[(3 + c) * P + pk], dnv = dsol[(3 + c) * P + pk];
+            t.duvd[c * s.N + k] = dnv;
+            t.duv[c * s.N + k] = t.duv[c * s.N + k] + dnv * h;
+            t.duvbar[c * s.N + k] = t.duvbar[c * s.N + k] + (dis_slide[c * s.N + k] * nv + is_slide[c * s.N + k] * dnv) * h;
+        }
+    }
+}
+
+// At = A^T in the same stencil layout (the stencil is symmetric in structure): the
+// entry of row (n, rr), column (m, cc) of At is A's entry of row (m, cc), column (n, rr).
+__global__ void __launch_bounds__(NBLK) k_transpose(const double* A, double* At, int W, int Lg, int ntx) {
+    int u = (blockIdx.x % ntx) * TILE + (threadIdx.x & (TILE - 1));
+    int v = (blockIdx.x / ntx) * TILE + (threadIdx.x / TILE);
+    if (u >= W || v >= Lg) return;
+    double* out = At + a_base(ntx, u, v);
+    for (int sidx = 0; sidx < NSLOT; ++sidx) {
+        int rr, du, dv, cc;
+        slot_decode(sidx, rr, du, dv, cc);
+        int mu = u + du, mv = v + dv;
+        double x = 0.0;
+        if (mu >= 0 && mu < W && mv >= 0 && mv < Lg) {
+            int s2 = slot_of(cc, -du, -dv, rr);
+            if (s2 >= 0) x = A[a_base(ntx, mu, mv) + (size_t)s2 * TILE_N];
+        }
+        out[(size_t)sidx * TILE_N] = x;
+    }
 }
 
 }  // namespace
@@ -1123,6 +1270,11 @@ struct yc_ctx {
     unsigned long long epoch;               // solves run by the persistent CG
     int coop_ok;                            // the persistent CG fits on the device at once
     hipGraphExec_t graph1;
+    // forward-mode tangents of the step (directional derivatives)
+    int ndir;
+    double* tstate;                         // ndir x TAN_PLANES planes of N
+    std::vector<double> tpar;               // ndir x YC_NPARAM
+    double *dA, *dF, *db, *dis_slide, *dcomp, *At, *xsol, *bsave, *dsol, *trhs;
 };
 
 namespace {
@@ -1145,6 +1297,10 @@ ClothDev make_dev(const yc_ctx* c) {
     C.wind_density = d.wind_density; C.wind_drag = d.wind_drag;
     C.jam_thrd = 2.0 * std::asin(d.R / d.L);   // src/Cloth.cpp:29
     C.x = c->x; C.v = c->v; C.xfix = c->xfix; C.uv = c->uv; C.uvd = c->uvd; C.uvbar = c->uvbar;
+    C.dx = C.dv = C.dxfix = C.duv = C.duvd = C.duvbar = C.dnrm = nullptr;
+    C.drho[0] = C.drho[1] = C.dks[0] = C.dks[1] = C.dkb[0] = C.dkb[1] = 0.0;
+    C.dkc = C.dkf = C.ddf = C.dmu = C.dS = C.dhandle_k = 0.0;
+    C.ref_mode = 0;
     C.nrm = c->nrm; C.flags = c->flags;
     return C;
 }
@@ -1152,6 +1308,8 @@ ClothDev make_dev(const yc_ctx* c) {
 AsmOut make_out(yc_ctx* c) {
     AsmOut O;
     O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
+    O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
+    O.detach = 0;
     O.invdiag = c->invd; O.ext = c->has_ext ? c->ext : nullptr; O.flag = c->flag; O.Wp = c->Wp; O.P = c->P;
     return O;
 }
@@ -1170,9 +1328,9 @@ VecArgs make_vec(yc_ctx* c) {
     return a;
 }
 
-SpmvArgs make_spmv(yc_ctx* c, const double* in, double* out) {
+SpmvArgs make_spmv(yc_ctx* c, const double* in, double* out, const double* Aptr = nullptr) {
     SpmvArgs a;
-    a.A = c->A; a.x = in; a.y = out; a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx;
+    a.A = Aptr ? Aptr : c->A; a.x = in; a.y = out; a.W = c->W; a.Lg = c->Lg; a.N = c->N; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx;
     return a;
 }
 
@@ -1244,9 +1402,9 @@ bool cgp_fits(const yc_ctx* c) {
     }
 }
 
-int launch_cgp(yc_ctx* c, double tol, int maxit) {
+int launch_cgp(yc_ctx* c, double tol, int maxit, const double* Aptr = nullptr) {
     CgpArgs a;
-    a.A = c->A; a.invd = c->invd; a.bbpart = c->part.bb; a.rrpart = c->part.rr;
+    a.A = Aptr ? Aptr : c->A; a.invd = c->invd; a.bbpart = c->part.bb; a.rrpart = c->part.rr;
     a.x = c->cx;
     a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1; a.s[0] = c->cs0; a.s[1] = c->cs1;
     a.slot[0] = c->cgslot; a.slot[1] = c->cgslot + c->nT;
@@ -1327,9 +1485,9 @@ int do_assemble(yc_ctx* c, double h) {
     HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
     static_assert(NBLK == TILE_N, "assembly blocks are matrix tiles");
-    hipLaunchKernelGGL(k_assemble1, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_assemble2, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
                        c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
@@ -1427,6 +1585,122 @@ int dalloc(T** p, size_t count) {
     return YC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Forward-mode tangents of the step.  Direction d holds the tangents of the state
+// (TAN_PLANES planes of N: dx, dv, dxfix (3 each), duv, duvd, duvbar (2 each), dnrm (3))
+// and of the YC_NPARAM material parameters.
+// ---------------------------------------------------------------------------
+constexpr int TAN_PLANES = 18;
+
+TanState tan_state(yc_ctx* c, int d) {
+    double* b = c->tstate + (size_t)d * TAN_PLANES * c->N;
+    size_t N = c->N;
+    TanState t;
+    t.dx = b; t.dv = b + 3 * N; t.dxfix = b + 6 * N; t.duv = b + 9 * N; t.duvd = b + 11 * N; t.duvbar = b + 13 * N;
+    return t;
+}
+double* tan_nrm(yc_ctx* c, int d) { return c->tstate + (size_t)d * TAN_PLANES * c->N + 15 * (size_t)c->N; }
+
+ClothDev make_tan_dev(yc_ctx* c, int d, int ref) {
+    ClothDev C = make_dev(c);
+    TanState t = tan_state(c, d);
+    C.dx = t.dx; C.dv = t.dv; C.dxfix = t.dxfix; C.duv = t.duv; C.duvd = t.duvd; C.duvbar = t.duvbar;
+    C.dnrm = tan_nrm(c, d);
+    const double* p = c->tpar.data() + (size_t)d * YC_NPARAM;
+    const double area = PI_REF * std::pow(c->d.R, 2);
+    for (int y = 0; y < 2; ++y) {
+        int yy = c->d.n_yarns == 1 ? 0 : y;
+        C.drho[y] = p[YC_P_RHO0 + yy];
+        C.dks[y] = p[YC_P_Y0 + yy] * area;    // ks = Y pi R^2 (src/Yarn.h:14)
+        C.dkb[y] = p[YC_P_B0 + yy] * area;    // kb = B pi R^2 (src/Yarn.h:15)
+    }
+    C.dS = p[YC_P_S]; C.dmu = p[YC_P_MU]; C.dkc = p[YC_P_KC]; C.dkf = p[YC_P_KF]; C.ddf = p[YC_P_DF];
+    C.dhandle_k = p[YC_P_HANDLE_K];
+    C.ref_mode = ref;
+    return C;
+}
+
+// Solve A_use x = c->b into c->cx without hipGraphs (the graphs hold c->A), through the
+// persistent CG when it fits, else the reference-order kernels.
+int solve_on(yc_ctx* c, const double* Aptr, double tol, int maxit, int* iters) {
+    hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 0);
+    if (c->coop_ok) {
+        int rc = launch_cgp(c, tol, maxit, Aptr);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_flag + 2, c->cg_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_flag[2]) {
+            (void)hipMemsetAsync(c->cg_err, 0, sizeof(int), c->stream);
+            return fail(YC_ERR_HIP, "persistent CG: a block waited past its spin limit");
+        }
+        if (iters) *iters = c->h_sc->fin_iters;
+    } else {
+        hipLaunchKernelGGL(k_cg_init2, dim3(1), dim3(BLK), 0, c->stream, c->part, c->sc, tol, maxit);
+        HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        long long it = 0;
+        while (!c->h_sc->done && it < (long long)maxit + 64) {
+            for (int i = 0; i < 32; ++i, ++it) {
+                int q = (int)(it & 1);
+                hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->cp, c->cy, Aptr), c->part.pAp,
+                                   (const CGScal*)c->sc, 1);
+                hipLaunchKernelGGL(k_cg_xr, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, (const CGScal*)c->sc, q);
+                hipLaunchKernelGGL(k_cg_p, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, c->sc, q);
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        if (iters) *iters = std::max(c->h_sc->itc[0], c->h_sc->itc[1]);
+    }
+    if (c->h_sc->zero_rhs) HIPCHK(hipMemsetAsync(c->cx, 0, (size_t)5 * c->P * sizeof(double), c->stream));
+    if (c->h_sc->rr >= c->h_sc->thr && !c->h_sc->zero_rhs) return fail(YC_ERR_NOT_CONVERGED, "tangent CG stopped at maxit above tolerance");
+    return YC_OK;
+}
+
+__global__ void k_axpy_neg(const double* a, const double* y, double* out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - y[i];
+}
+
+// Tangent of one step along direction d, after the step's own assembly and solve (the
+// solution is in xsol, the right-hand side in bsave).  exact (mode 1): the derivative of
+// the step as computed, A x = b differentiated as A dx = db - dA x.  ref (mode 0): the
+// reference's gradient: A, M and K_v held constant, the non-jamming shear angle held
+// constant, and the linear solve transposed (A^T dx = db), so that the transpose of this
+// tangent map is the reference's backward solve with A (src/Solvers.cpp:51-53).
+int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int* iters) {
+    const size_t PV = (size_t)5 * c->P;
+    StateDev sd = make_state(c);
+    TanState ts = tan_state(c, d);
+    hipLaunchKernelGGL(k_normals_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd, (const double*)ts.dx,
+                       tan_nrm(c, d));
+    ClothDev C = make_tan_dev(c, d, mode == YC_GRAD_REF);
+    AsmOut O = make_out(c);
+    O.dA = mode == YC_GRAD_EXACT ? c->dA : nullptr;
+    O.dF = c->dF; O.db = c->db; O.dis_slide = c->dis_slide; O.dcomp = c->dcomp;
+    O.detach = mode == YC_GRAD_REF;
+    O.b = c->bsave;   // the tangent kernels write db only; b is not touched
+    hipLaunchKernelGGL(k_assemble1<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
+    hipLaunchKernelGGL(k_assemble2<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
+    HIPCHK(hipGetLastError());
+    if (mode == YC_GRAD_EXACT) {
+        hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->xsol, c->trhs, c->dA), (double*)nullptr,
+                           (const CGScal*)c->sc, 0);
+        hipLaunchKernelGGL(k_axpy_neg, dim3(grid_for((long long)PV)), dim3(BLK), 0, c->stream, (const double*)c->db,
+                           (const double*)c->trhs, c->b, (int)PV);
+    } else {
+        HIPCHK(hipMemcpyAsync(c->b, c->db, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    int rc = solve_on(c, mode == YC_GRAD_EXACT ? c->A : c->At, tol, maxit, iters);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_update_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd, ts, (const double*)c->xsol,
+                       (const double*)c->cx, (const double*)c->is_slide, (const double*)c->dis_slide, c->Wp, c->P, h);
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1444,8 +1718,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     if (desc->n_yarns < 1 || desc->n_yarns > 2) return fail(YC_ERR_ARG, "n_yarns must be 1 or 2");
     if (desc->n_handles < 0 || desc->n_handles > YC_MAX_HANDLES) return fail(YC_ERR_ARG, "too many handles");
     if ((long long)desc->width * desc->length > (1LL << 26)) return fail(YC_ERR_ARG, "cloth too large");
-    yc_ctx* c = new yc_ctx();
-    std::memset((void*)c, 0, sizeof(yc_ctx));
+    yc_ctx* c = new yc_ctx();   // value-initialised: every pointer and count starts at zero
     c->d = *desc;
     c->W = desc->width; c->Lg = desc->length; c->N = c->W * c->Lg; c->Nc = (c->W - 2) * (c->Lg - 2);
     c->n = 3 * c->N + 2 * c->Nc;
@@ -1531,7 +1804,8 @@ void yc_destroy(yc_ctx* c) {
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
-                    c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err};
+                    c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
+                    c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
@@ -1787,6 +2061,111 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
     }
     if (total_iters) *total_iters = tot;
     return worst;
+}
+
+int yc_tangent_end(yc_ctx* c) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    (void)hipStreamSynchronize(c->stream);
+    double** bufs[] = {&c->tstate, &c->dA, &c->dF, &c->db, &c->dis_slide, &c->dcomp, &c->At, &c->xsol, &c->bsave, &c->dsol, &c->trhs};
+    for (double** p : bufs) { if (*p) (void)hipFree(*p); *p = nullptr; }
+    c->ndir = 0;
+    c->tpar.clear();
+    return YC_OK;
+}
+
+int yc_tangent_begin(yc_ctx* c, int ndir, const double* dparam, const double* dstate) {
+    if (!c || ndir <= 0 || !dparam) return fail(YC_ERR_ARG, "bad argument");
+    g_err.clear();
+    yc_tangent_end(c);
+    size_t N = c->N, PV = (size_t)5 * c->P;
+    int rc = YC_OK;
+    auto chk = [&](int r) { if (r && !rc) rc = r; };
+    chk(dalloc(&c->tstate, (size_t)ndir * TAN_PLANES * N));
+    chk(dalloc(&c->dA, c->a_size)); chk(dalloc(&c->At, c->a_size));
+    chk(dalloc(&c->dF, 5 * N)); chk(dalloc(&c->db, PV)); chk(dalloc(&c->dis_slide, 2 * N)); chk(dalloc(&c->dcomp, N));
+    chk(dalloc(&c->xsol, PV)); chk(dalloc(&c->bsave, PV)); chk(dalloc(&c->dsol, PV)); chk(dalloc(&c->trhs, PV));
+    if (rc) { yc_tangent_end(c); return rc; }
+    c->ndir = ndir;
+    c->tpar.assign(dparam, dparam + (size_t)ndir * YC_NPARAM);
+    if (dstate) {
+        std::vector<double> t((size_t)TAN_PLANES * N, 0.0);
+        for (int d = 0; d < ndir; ++d) {
+            const double* src = dstate + (size_t)d * 15 * N;
+            const double *lpos = src, *lposfix = src + 3 * N, *lvel = src + 6 * N;
+            const double *epos = src + 9 * N, *eposbar = src + 11 * N, *evel = src + 13 * N;
+            for (int v = 0; v < c->Lg; ++v)
+                for (int u = 0; u < c->W; ++u) {
+                    int k = u + v * c->W, st = u + v * c->Lg;
+                    for (int q = 0; q < 3; ++q) {
+                        t[0 * N + q * N + k] = lpos[3 * st + q];
+                        t[3 * N + q * N + k] = lvel[3 * st + q];
+                        t[6 * N + q * N + k] = lposfix[3 * st + q];
+                    }
+                    for (int q = 0; q < 2; ++q) {
+                        t[9 * N + q * N + k] = epos[2 * st + q];
+                        t[11 * N + q * N + k] = evel[2 * st + q];
+                        t[13 * N + q * N + k] = eposbar[2 * st + q];
+                    }
+                }
+            HIPCHK(hipMemcpyAsync(c->tstate + (size_t)d * TAN_PLANES * N, t.data(), t.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    }
+    return YC_OK;
+}
+
+int yc_tangent_get(yc_ctx* c, int dir, double* dlpos, double* dlposfix, double* dlvel, double* depos, double* deposbar,
+                   double* devel) {
+    if (!c || dir < 0 || dir >= c->ndir) return fail(YC_ERR_ARG, "bad direction");
+    size_t N = c->N;
+    std::vector<double> t((size_t)TAN_PLANES * N);
+    HIPCHK(hipMemcpyAsync(t.data(), c->tstate + (size_t)dir * TAN_PLANES * N, t.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int v = 0; v < c->Lg; ++v)
+        for (int u = 0; u < c->W; ++u) {
+            int k = u + v * c->W, st = u + v * c->Lg;
+            for (int q = 0; q < 3; ++q) {
+                if (dlpos) dlpos[3 * st + q] = t[0 * N + q * N + k];
+                if (dlvel) dlvel[3 * st + q] = t[3 * N + q * N + k];
+                if (dlposfix) dlposfix[3 * st + q] = t[6 * N + q * N + k];
+            }
+            for (int q = 0; q < 2; ++q) {
+                if (depos) depos[2 * st + q] = t[9 * N + q * N + k];
+                if (devel) devel[2 * st + q] = t[11 * N + q * N + k];
+                if (deposbar) deposbar[2 * st + q] = t[13 * N + q * N + k];
+            }
+        }
+    return YC_OK;
+}
+
+int yc_step_tangent(yc_ctx* c, double h, int mode, double tol, int maxit, yc_step_info* info) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (c->ndir <= 0) return fail(YC_ERR_ARG, "yc_step_tangent before yc_tangent_begin");
+    if (mode != YC_GRAD_REF && mode != YC_GRAD_EXACT) return fail(YC_ERR_ARG, "unknown gradient mode");
+    g_err.clear();
+    int it = 0;
+    double rr = 0;
+    int rc = do_assemble(c, h);
+    if (!rc) rc = do_solve(c, nullptr, tol, maxit, &it, &rr);
+    if (info) { info->cg_iters = it; info->cg_relres = rr; info->status = rc; }
+    if (rc) return rc;   // a tangent needs the converged step
+    const size_t PV = (size_t)5 * c->P;
+    HIPCHK(hipMemcpyAsync(c->xsol, c->cx, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->bsave, c->b, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (mode == YC_GRAD_REF)
+        hipLaunchKernelGGL(k_transpose, dim3(c->nT), dim3(NBLK), 0, c->stream, (const double*)c->A, c->At, c->W, c->Lg, c->ntx);
+    HIPCHK(hipGetLastError());
+    for (int d = 0; d < c->ndir && !rc; ++d) rc = tangent_dir(c, d, mode, h, tol, maxit, nullptr);
+    // restore the step's own right-hand side and solution, then update
+    HIPCHK(hipMemcpyAsync(c->b, c->bsave, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->cx, c->xsol, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (rc) return rc;
+    rc = do_update(c, h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return YC_OK;
 }
 
 int yc_set_cg_variant(yc_ctx* c, int variant) {

@@ -184,6 +184,49 @@ int yc_targets(yc_ctx* ctx, float* out);
 int yc_step_warm(yc_ctx* ctx, yc_mlp* m, double h, double tol, int maxit, yc_step_info* info);
 int yc_run_warm(yc_ctx* ctx, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters);
 
+/* ---------------------------------------------------------------------------
+ * Differentiation: forward-mode tangents (directional derivatives) of SimStep.
+ * Replaces: the gradient the reference obtains by LibTorch autograd through SimStep
+ * with EigenSolver::apply's backward (src/Solvers.h:35-52, src/Solvers.cpp:33-58),
+ * used by the parameter-learning drivers (src/TrainHeter.h, src/YarnSimNetHeter.h).
+ * The tangent of a scalar loss along a parameter direction is the gradient's component
+ * along it, so YC_NPARAM directions give the full parameter gradient.
+ *   YC_GRAD_EXACT: the derivative of the step as computed (A's dependence included).
+ *   YC_GRAD_REF:   the reference's gradient: A, M, K_v held constant (they are built
+ *     outside the autograd graph, src/MatrixMp.cpp:46-98), the non-jamming shear angle
+ *     held constant (.item(), src/PhysicsCloth.cpp:2720-2721), and the solve's backward
+ *     taken with A instead of A^T (src/Solvers.cpp:51-53); as a tangent map that is a
+ *     forward solve with A^T.
+ * Parameter order of a direction: */
+#define YC_P_RHO0 0
+#define YC_P_RHO1 1
+#define YC_P_Y0 2
+#define YC_P_Y1 3
+#define YC_P_B0 4
+#define YC_P_B1 5
+#define YC_P_S 6
+#define YC_P_MU 7
+#define YC_P_KC 8
+#define YC_P_KF 9
+#define YC_P_DF 10
+#define YC_P_HANDLE_K 11
+#define YC_NPARAM 12
+#define YC_GRAD_REF 0
+#define YC_GRAD_EXACT 1
+
+/* Starts ndir tangent directions: dparam ndir x YC_NPARAM, dstate ndir x 15N doubles in
+ * the yc_set_state order (lpos, lposfix, lvel N*3 each; epos, eposbar, evel N*2 each) or
+ * NULL for zero state tangents.  Replaces any directions already held. */
+int yc_tangent_begin(yc_ctx* ctx, int ndir, const double* dparam, const double* dstate);
+/* SimStep (as yc_step, zero CG guess) and the tangents of all directions through it;
+ * the tangent solves use tol and maxit too. */
+int yc_step_tangent(yc_ctx* ctx, double h, int mode, double tol, int maxit, yc_step_info* info);
+/* State tangent of direction dir, in the yc_get_state layout; any pointer may be NULL. */
+int yc_tangent_get(yc_ctx* ctx, int dir, double* dlpos, double* dlposfix, double* dlvel,
+                   double* depos, double* deposbar, double* devel);
+/* Frees the tangent buffers. */
+int yc_tangent_end(yc_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,9 @@ def lib():
         L.yo_update.argtypes = [ctypes.c_void_p, dp, ctypes.c_double]
         L.yo_step.argtypes = [ctypes.c_void_p, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int, ip, dp]
         L.yo_owner.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.yo_freeze_shear_angle.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.yo_set_frozen_shear_angle.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
+        L.yo_get_frozen_shear_angle.argtypes = [ctypes.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -180,6 +183,19 @@ class OracleCloth:
         x0a = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         st = lib().yo_solve(self._h, float(tol), int(maxit), _dp(x0a), _dp(x), ctypes.byref(it), ctypes.byref(rr))
         return x, it.value, rr.value, st
+
+    def freeze_shear_angle(self, on=True):
+        """Hold the non-jamming shear's (phi - pi/2) at the current state's values."""
+        lib().yo_freeze_shear_angle(self._h, 1 if on else 0)
+
+    def frozen_shear_angle(self):
+        out = np.zeros(8 * self.n + 16)
+        k = lib().yo_get_frozen_shear_angle(self._h, _dp(out))
+        return out[:k].copy()
+
+    def set_frozen_shear_angle(self, pb):
+        pb = np.ascontiguousarray(pb, dtype=np.float64)
+        lib().yo_set_frozen_shear_angle(self._h, _dp(pb), int(pb.size))
 
     def update(self, new_vel, h):
         nv = np.ascontiguousarray(new_vel, dtype=np.float64)

@@ -216,6 +216,11 @@ struct yo_cloth {
     void Crimp_Bending(const Bend& b);
     void ConSlideFriction(int k);
     void Shear_FD(const Shear& s);
+    // Checker option: hold the non-jamming shear's (phi - pi/2) at stored values, the
+    // quantity the reference takes through .item() (src/PhysicsCloth.cpp:2720-2721), so
+    // that differences of F reproduce the reference's detached derivative.
+    std::vector<double> frozen_pb;
+    double shear_phi(const Shear& s) const;
     void Wind_F(const Face& f);
     void Constraint(int k);
     void edge_blocks(SpAcc& S, const Edge& e, const M3& L0L0, const M3& L0L1, const M3& L1L0, const M3& L1L1,
@@ -1022,6 +1027,7 @@ void yo_cloth::Shear_FD(const Shear& s) {
     }
     // non-jamming branch (src/PhysicsCloth.cpp:2720-2726, 3056-3088)
     double pb = phi - PI / 2;
+    if (!frozen_pb.empty()) pb = frozen_pb[&s - shears.data()];
     V3 F_x1 = scl((kx * L * pb) / (l1 * sn), P1d3);
     V3 F_x3 = scl((kx * L * pb) / (l3 * sn), P3d1);
     V3 F_x0 = scl(-1, add(F_x1, F_x3));
@@ -1042,6 +1048,12 @@ void yo_cloth::Shear_FD(const Shear& s) {
     addLL(Kx, s.n0, s.n0, L0L0); addLL(Kx, s.n0, s.n1, L0L1); addLL(Kx, s.n0, s.n3, L0L3);
     addLL(Kx, s.n1, s.n0, L1L0); addLL(Kx, s.n1, s.n1, L1L1); addLL(Kx, s.n1, s.n3, L1L3);
     addLL(Kx, s.n3, s.n0, L3L0); addLL(Kx, s.n3, s.n1, L3L1); addLL(Kx, s.n3, s.n3, L3L3);
+}
+
+double yo_cloth::shear_phi(const Shear& s) const {
+    V3 a = sub(x[s.n1], x[s.n0]), b3 = sub(x[s.n3], x[s.n0]);
+    V3 d1 = scl(1.0 / norm(a), a), d3 = scl(1.0 / norm(b3), b3);
+    return std::acos(dot(d1, d3));
 }
 
 // Wind_F (src/PhysicsCloth.cpp:3091-3106) with Face::update (src/ClothUnit.h:268-273)
@@ -1216,6 +1228,21 @@ void yo_set_state(yo_cloth* c, const double* lpos, const double* lposfix, const 
 }
 
 int yo_assemble(yo_cloth* c, double h, const double* ext_force) { return c->assemble(h, ext_force); }
+
+void yo_freeze_shear_angle(yo_cloth* c, int on) {
+    c->frozen_pb.clear();
+    if (!on) return;
+    for (const Shear& s : c->shears) c->frozen_pb.push_back(c->shear_phi(s) - PI / 2);
+}
+
+void yo_set_frozen_shear_angle(yo_cloth* c, const double* pb, int n) {
+    c->frozen_pb.assign(pb, pb + n);
+}
+
+int yo_get_frozen_shear_angle(const yo_cloth* c, double* pb) {
+    for (size_t i = 0; i < c->frozen_pb.size(); ++i) pb[i] = c->frozen_pb[i];
+    return (int)c->frozen_pb.size();
+}
 
 int yo_nnz(const yo_cloth* c) { return (int)c->col.size(); }
 

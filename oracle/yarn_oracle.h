@@ -57,6 +57,13 @@ int yo_assemble(yo_cloth* c, double h, const double* ext_force);
 
 /* Results of the last yo_assemble. */
 int yo_nnz(const yo_cloth* c);
+/* Checker option for the reference's detached gradient: hold the non-jamming shear's
+ * (phi - pi/2) (src/PhysicsCloth.cpp:2720-2721) at the current state's values (on = 1),
+ * at given values, or release it (on = 0). */
+void yo_freeze_shear_angle(yo_cloth* c, int on);
+void yo_set_frozen_shear_angle(yo_cloth* c, const double* pb, int n);
+int yo_get_frozen_shear_angle(const yo_cloth* c, double* pb);
+
 /* which: 0 = A, 1 = M, 2 = K_x (F_*_Pos), 3 = K_v (F_*_Vel); CSR of the global n x n matrix */
 void yo_get_csr(const yo_cloth* c, int which, int* rowptr, int* col, double* val);
 void yo_get_dense(const yo_cloth* c, int which, double* dense /* n*n row-major */);
