@@ -11,3 +11,4 @@ from . import cloth_spec  # noqa: F401
 from ._abi import (YcError, NotConverged, NonFinite, YC_OK, YC_ERR_NONFINITE,  # noqa: F401
                    YC_ERR_NOT_CONVERGED, YC_ERR_HIP, YC_ERR_ARG, lib_path, load_library)
 from .cloth import Cloth, PhysicsCloth, SimStep  # noqa: F401
+from . import grad  # noqa: F401

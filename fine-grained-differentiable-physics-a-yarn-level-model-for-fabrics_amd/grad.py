@@ -1,0 +1,107 @@
+"""Gradients of yarn-level rollouts with respect to the material parameters.
+
+The reference learns per-yarn density, stretch and bend moduli and the cloth's shear
+stiffness and friction by backpropagating a position loss through SimStep
+(src/TrainHeter.h:81-146, loss src/TrainHeter.h:211-226, autograd through
+EigenSolver::apply, src/Solvers.h:35-52).  Here the step's tangents are propagated
+forward on the GPU (Cloth.step_tangent, one direction per parameter), so the Jacobian of
+the whole trajectory with respect to the parameters is available after the forward
+rollout and the backward pass is a contraction with it: no checkpoint, no recompute
+and O(1) memory in the rollout length.
+
+Two gradients are offered: "ref", the reference's own (M, K_v, A and the non-jamming
+shear angle held constant, backward solve with A instead of A^T), and "exact", the
+derivative of the step as computed.
+"""
+import copy
+
+import numpy as np
+
+from .cloth import Cloth
+
+PARAMS = Cloth.PARAMS
+
+
+def get_param(spec, name):
+    if name[:-1] in ("rho", "Y", "B") and name[-1] in "01":
+        return spec["yarns"][int(name[-1])][name[:-1]]
+    return spec[name]
+
+
+def with_params(spec, names, values):
+    """Copy of spec with the named parameters set."""
+    s = copy.deepcopy(spec)
+    for n, v in zip(names, values):
+        if n[:-1] in ("rho", "Y", "B") and n[-1] in "01":
+            s["yarns"][int(n[-1])][n[:-1]] = float(v)
+        else:
+            s[n] = float(v)
+    return s
+
+
+def rollout_jacobian(spec, state, steps, names, mode="ref", tol=1e-10, maxit=None, ext_force=None, cg="persistent"):
+    """Runs `steps` SimSteps from `state` and returns (positions, jac): positions
+    (steps, N, 3) after each step, jac (len(names), steps, N, 3) their derivatives with
+    respect to the named parameters."""
+    if len(spec["yarns"]) == 1 and any(n in ("rho1", "Y1", "B1") for n in names):
+        raise ValueError("a one-yarn cloth has no second yarn parameters")
+    c = Cloth(spec)
+    try:
+        try:
+            c.set_cg_variant(cg)
+        except Exception:
+            c.set_cg_variant("fused")
+        c.set_ClothState(state)
+        c.set_ext_force(ext_force)
+        dp = np.zeros((len(names), len(PARAMS)))
+        for i, n in enumerate(names):
+            dp[i, PARAMS.index(n)] = 1.0
+        c.tangent_begin(dp)
+        N = c.N
+        pos = np.zeros((steps, N, 3))
+        jac = np.zeros((len(names), steps, N, 3))
+        for t in range(steps):
+            c.step_tangent(spec["h"], mode, tol=tol, maxit=maxit)
+            pos[t] = c.get_ClothState()["LPos"]
+            for i in range(len(names)):
+                jac[i, t] = c.tangent_get(i)["LPos"]
+        c.tangent_end()
+    finally:
+        c.close()
+    return pos, jac
+
+
+def position_loss(pos, target):
+    """Mean squared position error over steps and nodes (src/TrainHeter.h:211-226)."""
+    return float(np.mean((pos - target) ** 2))
+
+
+def loss_and_grad(spec, state, steps, names, target, mode="ref", **kw):
+    pos, jac = rollout_jacobian(spec, state, steps, names, mode=mode, **kw)
+    r = pos - target
+    loss = float(np.mean(r ** 2))
+    g = np.array([2.0 * np.sum(r * jac[i]) / r.size for i in range(len(names))])
+    return loss, g
+
+
+def make_autograd_function():
+    """torch.autograd.Function over the rollout: forward(theta) -> positions
+    (steps, N, 3) as float64; backward contracts the upstream gradient with the
+    forward-mode Jacobian saved in the forward pass."""
+    import torch
+
+    class YarnRollout(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta, spec, state, steps, names, mode):
+            vals = theta.detach().cpu().numpy().astype(np.float64)
+            pos, jac = rollout_jacobian(with_params(spec, names, vals), state, steps, names, mode=mode)
+            ctx.save_for_backward(torch.from_numpy(jac))
+            return torch.from_numpy(pos)
+
+        @staticmethod
+        def backward(ctx, g):
+            (jac,) = ctx.saved_tensors
+            grad = (jac * g.unsqueeze(0)).reshape(jac.shape[0], -1).sum(dim=1)
+            return grad.to(torch.float64), None, None, None, None, None
+
+    return YarnRollout

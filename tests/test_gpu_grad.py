@@ -279,3 +279,45 @@ def test_ref_and_exact_modes_differ(ys, cs):
     d = rel(out[1], out[0])
     say("  |ref - exact| / |exact| = %.3e" % d)
     assert d > 1e-3
+
+
+def test_autograd_rollout_gradient_matches_fd(ys, cs):
+    """The torch.autograd.Function over an 8-step rollout: the gradient of the position
+    loss against a target trajectory (seeded ground-truth parameters, as the parameter-
+    learning drivers use) matches the oracle's central difference of the same loss along
+    the gradient direction, in exact mode."""
+    import torch
+    from importlib import import_module
+    grad = import_module(ys.__name__ + ".grad")
+    spec = _spec(cs)
+    state = natural_state(cs, spec)
+    names = ["rho0", "Y1", "B0", "S"]
+    truth = np.array([grad.get_param(spec, n) for n in names])
+    steps = 8
+    target, _ = grad.rollout_jacobian(spec, state, steps, names, mode="exact", tol=1e-14, maxit=20000)
+    theta0 = truth * np.array([1.3, 0.8, 1.2, 0.7])
+    fn = grad.make_autograd_function()
+    theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    pos = fn.apply(theta, grad.with_params(spec, names, theta0), state, steps, names, "exact")
+    loss = torch.mean((pos - torch.from_numpy(target)) ** 2)
+    loss.backward()
+    g = theta.grad.numpy()
+    # central difference of the oracle's loss along g, in units of the parameters' sizes
+    dirn = g * theta0 ** 2
+    dirn = dirn / np.linalg.norm(dirn / theta0)
+
+    def oracle_loss(vals):
+        r = OracleCloth(grad.with_params(spec, names, vals))
+        r.set_state(state)
+        out = []
+        for _ in range(steps):
+            st, _, _ = r.step(spec["h"], None, tol=1e-14, maxit=20000)
+            assert st == 0
+            out.append(r.get_state()["LPos"])
+        return np.mean((np.array(out) - target) ** 2)
+
+    e = 1e-4
+    fd = (oracle_loss(theta0 + e * dirn) - oracle_loss(theta0 - e * dirn)) / (2 * e)
+    an = float(g @ dirn)
+    say("  loss %.4e  directional derivative: autograd %.8e  oracle fd %.8e  rel %.2e" % (float(loss), an, fd, abs(an - fd) / abs(fd)))
+    assert abs(an - fd) <= 1e-5 * abs(fd)
