@@ -111,16 +111,37 @@ __device__ double block_sum(double v) {
     return s;  // valid in thread 0
 }
 
+// DPP move of a double (both 32-bit halves) with the given control and masks; lanes that
+// are masked off or read outside their row get 0.
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ __forceinline__ double dpp_mov(double v) {
+    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, BANK_MASK, true);
+    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, BANK_MASK, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront into lane 63, in a fixed order: row_shr 1, 2, 4, 8
+// build each 16-lane row's sum in its lane 15, row_bcast 15 and 31 carry the rows on.
+// Register-to-register (DPP), without the LDS round trips of shuffles.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = v + dpp_mov<0x111, 0xf, 0xf>(v);   // row_shr:1
+    v = v + dpp_mov<0x112, 0xf, 0xf>(v);   // row_shr:2
+    v = v + dpp_mov<0x114, 0xf, 0xf>(v);   // row_shr:4
+    v = v + dpp_mov<0x118, 0xf, 0xf>(v);   // row_shr:8
+    v = v + dpp_mov<0x142, 0xa, 0xf>(v);   // row_bcast:15 into rows 1 and 3
+    v = v + dpp_mov<0x143, 0xc, 0xf>(v);   // row_bcast:31 into rows 2 and 3
+    return v;                               // the total is in lane 63
+}
+
 // Several block sums in one pass (fixed order); results valid in thread 0.
 template <int M>
 __device__ void block_sum_n(double (&v)[M]) {
     __shared__ double sh[M][BLK / 64];
 #pragma unroll
-    for (int m = 0; m < M; ++m)
-        for (int off = 32; off > 0; off >>= 1) v[m] += __shfl_down(v[m], off, 64);
+    for (int m = 0; m < M; ++m) v[m] = wave_sum_dpp(v[m]);
     int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     __syncthreads();
-    if (lane == 0)
+    if (lane == 63)
 #pragma unroll
         for (int m = 0; m < M; ++m) sh[m][wid] = v[m];
     __syncthreads();
@@ -809,11 +830,10 @@ __device__ __forceinline__ void st_dev(double* p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Waits until every slot of `sl` carries `want` (or the spin limit / a peer's error),
-// then sums the slots' partials in a fixed order; sred[0..5] receive the sums, sred[6]
-// is non-zero when the block must leave.  Block-uniform.
+// Waits until every slot of `sl` carries `want` (or the spin limit / a peer's error).
+// Returns false, block-uniformly, when the block must leave.
 template <int KP>
-__device__ void cgp_gather(const CgpArgs& a, const CgSlot* sl, unsigned long long want, double* sred) {
+__device__ bool cgp_wait(const CgpArgs& a, const CgSlot* sl, unsigned long long want) {
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
@@ -828,22 +848,9 @@ __device__ void cgp_gather(const CgpArgs& a, const CgSlot* sl, unsigned long lon
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < KP; ++k) {
-        int i = threadIdx.x + k * BLK;
-        if (ok && i < a.nT)
-#pragma unroll
-            for (int m = 0; m < 6; ++m) acc[m] += ld_dev(&sl[i].v[m]);
-    }
     int bad = __syncthreads_or(!ok);
-    block_sum_n(acc);
-    if (threadIdx.x == 0) {
-        for (int m = 0; m < 6; ++m) sred[m] = acc[m];
-        sred[6] = bad ? 1.0 : 0.0;
-        if (bad) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
+    if (bad && threadIdx.x == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return !bad;
 }
 
 // Publishes this block's partials and tag after all of its stores have completed.
@@ -928,9 +935,40 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
         const int q = j & 1;
         const bool init = j < 0;
         const bool first = j == 0;
+        if (!init && !cgp_wait<KP>(a, a.slot[q], (a.epoch << 32) | (unsigned long long)j)) return;   // error
+        // one round trip for the partials of every tile and the inputs of tile + halo
+        // (r_j, w_j, s_{j-1}); loads whose values an init or first stage ignores are
+        // still issued (their buffers exist), so that none of them sits in a branch
+        const CgSlot* sl = a.slot[q];
+        double pv[KP][6];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            int i = threadIdx.x + k * BLK;
+            int ic = i < a.nT ? i : a.nT - 1;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) pv[k][m] = ld_dev(&sl[ic].v[m]);
+        }
+        const double* r_in = a.r[init ? 0 : q];
+        const double* w_in = a.w[q];
+        const double* s_in = a.s[1 - q];
+        double e_r[CG1_NE], e_w[CG1_NE], e_s[CG1_NE];
+#pragma unroll
+        for (int t = 0; t < CG1_NE; ++t) {
+            e_r[t] = ld_dev(r_in + e_i[t]);
+            e_w[t] = ld_dev(w_in + e_i[t]);
+            e_s[t] = ld_dev(s_in + e_i[t]);
+        }
         if (!init) {
-            cgp_gather<KP>(a, a.slot[q], (a.epoch << 32) | (unsigned long long)j, sred);
-            if (sred[6] != 0.0) return;   // error: block-uniform
+            double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < KP; ++k)
+                if (threadIdx.x + k * BLK < a.nT)
+#pragma unroll
+                    for (int m = 0; m < 6; ++m) acc[m] += pv[k][m];
+            block_sum_n(acc);
+            if (threadIdx.x == 0)
+                for (int m = 0; m < 6; ++m) sred[m] = acc[m];
+            __syncthreads();
             const double gam = sred[0], d1 = sred[1], d2 = sred[2], d3 = sred[3], d4 = sred[4], rr = sred[5];
             if (rr < thr || j >= a.maxit) {
                 if (lead) {
@@ -946,16 +984,12 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
             alpha = gam / pAp;
             gprev = gam;
         }
-        // inputs of tile + halo: r_j, w_j, s_{j-1}
-        const double* r_in = a.r[init ? 0 : q];
-        const double* w_in = a.w[q];
-        const double* s_in = a.s[1 - q];
-        double e_r[CG1_NE], e_w[CG1_NE], e_s[CG1_NE];
+        if (init || first) {
 #pragma unroll
-        for (int t = 0; t < CG1_NE; ++t) {
-            e_r[t] = ld_dev(r_in + e_i[t]);
-            e_w[t] = init ? 0.0 : ld_dev(w_in + e_i[t]);
-            e_s[t] = (init || first) ? 0.0 : ld_dev(s_in + e_i[t]);
+            for (int t = 0; t < CG1_NE; ++t) {
+                e_s[t] = 0.0;
+                if (init) e_w[t] = 0.0;
+            }
         }
         double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // r.u, u.w, u.s, p.w, p.s, r.r
         double o_s[CG1_NE], o_r[CG1_NE];
