@@ -107,7 +107,7 @@ __device__ double block_sum(double v) {
     __syncthreads();
     double s = 0.0;
     if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+        for (int i = 0; i < BLK / 64; ++i) s += sh[i];
     return s;  // valid in thread 0
 }
 
@@ -149,7 +149,7 @@ __device__ void block_sum_n(double (&v)[M]) {
 #pragma unroll
         for (int m = 0; m < M; ++m) {
             double t = 0.0;
-            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[m][i];
+            for (int i = 0; i < BLK / 64; ++i) t += sh[m][i];
             v[m] = t;
         }
 }
@@ -157,7 +157,7 @@ __device__ void block_sum_n(double (&v)[M]) {
 // Sum of partials[0..n) in a fixed order by one block (all threads participate).
 __device__ double block_sum_partials(const double* partials, int n) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+    for (int i = threadIdx.x; i < n; i += BLK) s += partials[i];
     return block_sum(s);
 }
 
@@ -457,7 +457,7 @@ __device__ __forceinline__ int vec_index(const VecArgs& a, int t, int& c) {
 // init: r = b - y (y = A x0 when has_x0, else x = 0, r = b); p = D^-1 r; partials of
 // r.r, r.z (into parity buffer 1, read by iteration 0) and b.b
 __global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, CGPart P, int has_x0) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.x * BLK + threadIdx.x;
     double rr = 0.0, rz = 0.0, bb = 0.0;
     if (t < 5 * a.N) {
         int c;
@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(BLK) k_cg_xr(VecArgs a, CGPart P, const CGScal
     double pAp = bcast(block_sum_partials(P.pAp, P.nS));
     double rz = bcast(block_sum_partials(P.rz[1 - q], P.nV));
     double alpha = rz / pAp;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.x * BLK + threadIdx.x;
     double rr = 0.0, rzn = 0.0;
     if (t < 5 * a.N) {
         int c;
@@ -536,7 +536,7 @@ __global__ void __launch_bounds__(BLK) k_cg_p(VecArgs a, CGPart Pt, CGScal* sc, 
     int itn = sc->itc[1 - q] + 1;
     if (lead) { sc->rr = rr; sc->itc[q] = itn; if (itn >= sc->maxit) sc->done = 1; }
     double beta = rzn / rzo;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.x * BLK + threadIdx.x;
     if (t < 5 * a.N) {
         int c;
         int i = vec_index(a, t, c);
@@ -1073,7 +1073,7 @@ struct StateDev {
 };
 
 __global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.x * NBLK + threadIdx.x;
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
     int pk = padidx(Wp, u, v);
@@ -1154,7 +1154,7 @@ __device__ void node_frame(const StateDev& s, int k, NodeFrame& f) {
 }
 
 __global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.x * NBLK + threadIdx.x;
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
     if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
@@ -1175,7 +1175,7 @@ __global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
 // is locally constant).  This is the derivative the reference's autograd takes through
 // torch::svd (src/Cloth.cpp:446) when the singular values are distinct.
 __global__ void __launch_bounds__(NBLK) k_normals_tan(StateDev s, const double* dx, double* dnrm) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.x * NBLK + threadIdx.x;
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
     if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
@@ -1220,7 +1220,7 @@ struct TanState {
 
 __global__ void __launch_bounds__(NBLK) k_update_tan(StateDev s, TanState t, const double* sol, const double* dsol,
                                                     const double* is_slide, const double* dis_slide, int Wp, int P, double h) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.x * NBLK + threadIdx.x;
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
     int pk = padidx(Wp, u, v);
@@ -1694,7 +1694,7 @@ int solve_on(yc_ctx* c, const double* Aptr, double tol, int maxit, int* iters) {
 }
 
 __global__ void k_axpy_neg(const double* a, const double* y, double* out, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.x * BLK + threadIdx.x;
     if (i < n) out[i] = a[i] - y[i];
 }
 
