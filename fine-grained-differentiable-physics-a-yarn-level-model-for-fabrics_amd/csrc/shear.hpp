@@ -8,9 +8,17 @@ namespace {
 // src/ClothUnit.h:240-248), stiffness scaled by the centre's compression.  The
 // jamming branch keeps the reference's expressions as written, including the
 // operator-precedence forms (1 - c^2 * H) and the kx second derivatives without PI.
-template <class T, class Sink>
-__device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp_arr,
-                              const double* dcomp_arr) {
+// Forces and LL blocks of one segment: F[i] on node i and block B[i][j] = dF_i/dx_j, for
+// i, j over (n0, n1, n3).
+template <class T>
+struct ShearOut {
+    v3<T> F[3];
+    mat3<T> B[3][3];
+};
+
+template <class T>
+__device__ void shear_compute(const ClothDev& C, int k0, int k1, int k3, const double* comp_arr, const double* dcomp_arr,
+                              ShearOut<T>& out) {
     typedef v3<T> V;
     typedef mat3<T> M;
     const double c = 3.0;        // src/PhysicsCloth.cpp:2649
@@ -135,14 +143,69 @@ __device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3
         F31 = ((kx * L) / (l1 * l3 * sn)) * ((pb * (cs2 * outer(P3d1, d3v) - (1.0 / sn) * outer(P3d1, d3v))) * P1);
         F33 = ((kx * L) / (pow(l3, 2.0) * sn)) * (pb * (-outer(P3d1, d3v) + cs2 * A31 - cs * P3 - outerP(d3v, d1, P3)) - (1.0 / sn) * A31);
     }
-    V Fx0 = -(Fx1 + Fx3);
-    S.FL(k0, Fx0); S.FL(k1, Fx1); S.FL(k3, Fx3);
-    M F10 = -(F11 + F13), F30 = -(F31 + F33);
-    M F01 = -(F11 + F31), F03 = -(F13 + F33);
-    M F00 = -(F10 + F30);
-    S.LL(KIND_KX, k0, k0, F00); S.LL(KIND_KX, k0, k1, F01); S.LL(KIND_KX, k0, k3, F03);
-    S.LL(KIND_KX, k1, k0, F10); S.LL(KIND_KX, k1, k1, F11); S.LL(KIND_KX, k1, k3, F13);
-    S.LL(KIND_KX, k3, k0, F30); S.LL(KIND_KX, k3, k1, F31); S.LL(KIND_KX, k3, k3, F33);
+    out.F[0] = -(Fx1 + Fx3); out.F[1] = Fx1; out.F[2] = Fx3;
+    out.B[1][0] = -(F11 + F13); out.B[2][0] = -(F31 + F33);
+    out.B[0][1] = -(F11 + F31); out.B[0][2] = -(F13 + F33);
+    out.B[0][0] = -(out.B[1][0] + out.B[2][0]);
+    out.B[1][1] = F11; out.B[1][2] = F13; out.B[2][1] = F31; out.B[2][2] = F33;
+}
+
+// The segment's contributions to the rows of node role r (0: n0, 1: n1, 2: n3), in the
+// order the whole segment emits them.
+template <class T, class Sink>
+__device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& F, const mat3<T> (&B)[3]) {
+    S.FL(k[r], F);
+    for (int j = 0; j < 3; ++j) S.LL(KIND_KX, k[r], k[j], B[j]);
+}
+
+template <class T, class Sink>
+__device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp_arr,
+                              const double* dcomp_arr) {
+    ShearOut<T> o;
+    shear_compute<T>(C, k0, k1, k3, comp_arr, dcomp_arr, o);
+    const int k[3] = {k0, k1, k3};
+    S.FL(k0, o.F[0]); S.FL(k1, o.F[1]); S.FL(k3, o.F[2]);
+    for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 3; ++j) S.LL(KIND_KX, k[r], k[j], o.B[r][j]);
+}
+
+// Segment s = 4 e + q of interior crossing e, q in the reference's shear_segs order
+// (left,bot), (left,top), (right,bot), (right,top) (src/Cloth.cpp:170-181): its nodes.
+__device__ __forceinline__ void shear_nodes(const ClothDev& C, int s, int (&k)[3]) {
+    int e = s >> 2, q = s & 3;
+    int cu = 1 + e % (C.W - 2), cv = 1 + e / (C.W - 2);
+    int kc = cu + cv * C.W;
+    k[0] = kc;
+    k[1] = (q >> 1) == 0 ? kc - 1 : kc + 1;
+    k[2] = (q & 1) == 0 ? kc + C.W : kc - C.W;
+}
+
+// Element-major shear: one thread per segment computes it once and stores, for each of
+// its three nodes, the 30 doubles that node's rows receive (force 3 + three LL blocks
+// 27), as planes rec[(r * 30 + f) * nseg + s].
+__global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* comp, double* rec, int nseg) {
+    int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= nseg) return;
+    int k[3];
+    shear_nodes(C, s, k);
+    ShearOut<double> o;
+    shear_compute<double>(C, k[0], k[1], k[2], comp, nullptr, o);
+    for (int r = 0; r < 3; ++r) {
+        double* base = rec + (size_t)r * 30 * nseg + s;
+        base[0] = o.F[r].x; base[(size_t)nseg] = o.F[r].y; base[(size_t)2 * nseg] = o.F[r].z;
+        for (int j = 0; j < 3; ++j)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) base[(size_t)(3 + j * 9 + a * 3 + b) * nseg] = o.B[r][j].m[a][b];
+    }
+}
+
+// Reads the role-r record of segment s.
+__device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]) {
+    const double* base = rec + (size_t)r * 30 * nseg + s;
+    F = D3(base[0], base[(size_t)nseg], base[(size_t)2 * nseg]);
+    for (int j = 0; j < 3; ++j)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) B[j].m[a][b] = base[(size_t)(3 + j * 9 + a * 3 + b) * nseg];
 }
 
 }  // namespace

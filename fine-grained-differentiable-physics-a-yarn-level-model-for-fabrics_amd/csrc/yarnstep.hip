@@ -188,6 +188,9 @@ struct AsmOut {
     // the comp tangent read by the shear; detach = the reference's gradient
     double *dA, *dF, *db, *dis_slide, *dcomp;
     int detach;
+    // value run: element-major shear records (k_shear_elem) replayed by k_assemble2
+    const double* shear_rec;
+    int nseg;
 };
 
 __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
@@ -289,6 +292,9 @@ __global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double
 template <class T, class Sink>
 __device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp,
                               const double* dcomp);
+template <class T, class Sink>
+__device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& F, const mat3<T> (&B)[3]);
+__device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
 
 template <class T>
 __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double h) {
@@ -311,7 +317,18 @@ __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double
                 for (int bq = 0; bq < 2; ++bq) {
                     int k1 = nb1[a], k3 = nb3[bq];
                     if (kc != k && k1 != k && k3 != k) continue;
-                    shear_element<T>(C, S, kc, k1, k3, O.comp, tan ? O.dcomp : nullptr);
+                    if constexpr (tan) {
+                        shear_element<T>(C, S, kc, k1, k3, O.comp, O.dcomp);
+                    } else {
+                        // replay the segment's record for this node's role
+                        int sidx = 4 * crossing(C, cu, cv) + 2 * a + bq;
+                        int r = kc == k ? 0 : (k1 == k ? 1 : 2);
+                        const int kk[3] = {kc, k1, k3};
+                        d3 F;
+                        m33 B[3];
+                        shear_load(O.shear_rec, O.nseg, sidx, r, F, B);
+                        emit_shear_role<double>(S, kk, r, F, B);
+                    }
                 }
         }
     int pk = padidx(O.Wp, u, v);
@@ -1309,6 +1326,7 @@ struct yc_ctx {
     double* tstate;                         // ndir x TAN_PLANES planes of N
     std::vector<double> tpar;               // ndir x YC_NPARAM
     double *dA, *dF, *db, *dis_slide, *dcomp, *At, *xsol, *bsave, *dsol, *trhs;
+    double* shear_rec;                      // 3 roles x 30 x (4 Nc) shear records
 };
 
 namespace {
@@ -1344,6 +1362,7 @@ AsmOut make_out(yc_ctx* c) {
     O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
     O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
     O.detach = 0;
+    O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
     O.invdiag = c->invd; O.ext = c->has_ext ? c->ext : nullptr; O.flag = c->flag; O.Wp = c->Wp; O.P = c->P;
     return O;
 }
@@ -1521,6 +1540,9 @@ int do_assemble(yc_ctx* c, double h) {
     static_assert(NBLK == TILE_N, "assembly blocks are matrix tiles");
     hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
+    if (c->Nc > 0)
+        hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
+                           c->shear_rec, 4 * c->Nc);
     hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
@@ -1781,6 +1803,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     chk(dalloc(&c->cs0, PV)); chk(dalloc(&c->cs1, PV));
     c->cg1_kp = std::min((c->nT + BLK - 1) / BLK, CG1_MAXP);
     chk(dalloc(&c->cgslot, (size_t)2 * c->nT));
+    chk(dalloc(&c->shear_rec, (size_t)90 * 4 * c->Nc + 1));
     chk(dalloc(&c->cg_err, 1));
     chk(dalloc(&c->part1, (size_t)12 * c->cg1_kp * BLK));
     c->part.nS = c->nT;
@@ -1839,7 +1862,8 @@ void yc_destroy(yc_ctx* c) {
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
                     c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
-                    c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs};
+                    c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs,
+                    c->shear_rec};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
