@@ -202,6 +202,13 @@ struct RowSinkT {
         if (A) A[(size_t)s * TILE_N] += val(x);
         if (dA) dA[(size_t)s * TILE_N] += tan_of(x);
     }
+    // accumulate into element i of a small array through a switch, so that an index known
+    // only at run time (the yarn direction c) does not force the array into scratch memory
+    YS_DEV static void add5(T (&a)[5], int i, T x) {
+        switch (i) { case 0: a[0] += x; break; case 1: a[1] += x; break; case 2: a[2] += x; break;
+                     case 3: a[3] += x; break; default: a[4] += x; break; }
+    }
+    YS_DEV static void add2(T (&a)[2], int i, T x) { if (i == 0) a[0] += x; else a[1] += x; }
     YS_DEV T velL(int k, int c) { return par<T>(C->v[c * C->N + k], tget(C->dv, c * C->N + k)); }
     YS_DEV T velE(int k, int c) { return par<T>(C->uvd[c * C->N + k], tget(C->duvd, c * C->N + k)); }
     // a block entry in a product with the velocity (value only when detached)
@@ -226,10 +233,10 @@ struct RowSinkT {
         if (kL == me)
             for (int v = 0; v < 3; ++v) {
                 addA(v, kE, 3 + c, coef[kind] * b[v]);
-                if (kind == KIND_M) Mv[v] += blk(b[v]) * velE(kE, c);
+                if (kind == KIND_M) add5(Mv, v, blk(b[v]) * velE(kE, c));
             }
         if (kind == KIND_KV && kE == me)
-            for (int v = 0; v < 3; ++v) KvTv[3 + c] += blk(b[v]) * velL(kL, v);
+            for (int v = 0; v < 3; ++v) add5(KvTv, 3 + c, blk(b[v]) * velL(kL, v));
     }
     // EL: A[3N+2e(kE)+c, 3kL+u] = B[u]
     YS_DEV void EL(int kind, int kL, int kE, int c, v3<T> B) {
@@ -237,34 +244,37 @@ struct RowSinkT {
         if (kE == me)
             for (int u = 0; u < 3; ++u) {
                 addA(3 + c, kL, u, coef[kind] * b[u]);
-                if (kind == KIND_M) Mv[3 + c] += blk(b[u]) * velL(kL, u);
+                if (kind == KIND_M) add5(Mv, 3 + c, blk(b[u]) * velL(kL, u));
             }
         if (kind == KIND_KV && kL == me)
-            for (int u = 0; u < 3; ++u) KvTv[u] += blk(b[u]) * velE(kE, c);
+            for (int u = 0; u < 3; ++u) add5(KvTv, u, blk(b[u]) * velE(kE, c));
     }
     // EE: A[3N+2e(kF)+c, 3N+2e(kD)+c] = val
     YS_DEV void EE(int kind, int kF, int kD, int c, T v) {
         if (kF == me) {
             addA(3 + c, kD, 3 + c, coef[kind] * v);
-            if (kind == KIND_M) Mv[3 + c] += blk(v) * velE(kD, c);
+            if (kind == KIND_M) add5(Mv, 3 + c, blk(v) * velE(kD, c));
             if (kD == me) {
-                if (kind == KIND_KX) KxEE[c] += v;
-                if (kind == KIND_KV) KvEE[c] += v;
+                if (kind == KIND_KX) add2(KxEE, c, v);
+                if (kind == KIND_KV) add2(KvEE, c, v);
             }
         }
-        if (kind == KIND_KV && kD == me) KvTv[3 + c] += blk(v) * velE(kF, c);
+        if (kind == KIND_KV && kD == me) add5(KvTv, 3 + c, blk(v) * velE(kF, c));
     }
     YS_DEV void FL(int k, v3<T> f) { if (k == me) { F[0] += f.x; F[1] += f.y; F[2] += f.z; } }
     // E force on the crossing of node k; elem_owner is the static owner thread of the
     // emitting element in its own loop, for the partial sum read by the friction
     YS_DEV void FE(int k, int c, T f, int elem_owner) {
         if (k == me) {
-            F[3 + c] += f;
-            if (elem_owner == fric_owner) FEpart[c] += f;
+            add5(F, 3 + c, f);
+            if (elem_owner == fric_owner) add2(FEpart, c, f);
         }
     }
     YS_DEV void SBCL(int k, v3<T> f) { if (k == me) SBC_L = SBC_L + f; }
-    YS_DEV void SBCLE(int k, int c, v3<T> f) { if (k == me) SBC_LE[c] = SBC_LE[c] + f; }
+    YS_DEV void SBCLE(int k, int c, v3<T> f) {
+        if (k != me) return;
+        if (c == 0) SBC_LE[0] = SBC_LE[0] + f; else SBC_LE[1] = SBC_LE[1] + f;
+    }
 };
 typedef RowSinkT<double> RowSink;
 
@@ -583,6 +593,107 @@ YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int 
         S.SBCL(k2, sg * o.Fx2); S.SBCL(k0, sg * o.Fx0); S.SBCL(k1, sg * o.Fx1);
         if (i2) S.SBCLE(k2, c, sg * (-o.L2E1));
         if (i1) S.SBCLE(k1, c, sg * o.L1E1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element-major bends (value run): one thread per bend computes the bend and its crimp
+// twin once and stores, for each of its three nodes (role 0: n2, 1: n0, 2: n1), the
+// values that node's rows receive, in the order bend_element emits them.  A record of
+// BEND_REC doubles per role, planes rec[(r * BEND_REC + f) * nb + b]; b is the bend's
+// index in the reference's list (warp bends v * (W-2) + uc - 1, then weft bends).
+// ---------------------------------------------------------------------------
+constexpr int BEND_REC = 48;
+
+YS_DEV void bend_decode(const ClothDev& C, int b, int& uc, int& vc, int& c, int& yarn) {
+    int nb_warp = C.Lg * (C.W - 2);
+    if (b < nb_warp) { c = 0; vc = b / (C.W - 2); uc = 1 + b % (C.W - 2); yarn = vc & 1; }
+    else { int q = b - nb_warp; c = 1; uc = q / (C.Lg - 2); vc = 1 + q % (C.Lg - 2); yarn = uc & 1; }
+}
+
+struct RecW {   // strided writer of one role's record
+    double* p; size_t st;
+    YS_DEV void put(int f, double x) const { p[(size_t)f * st] = x; }
+    YS_DEV void put3(int f, d3 v) const { put(f, v.x); put(f + 1, v.y); put(f + 2, v.z); }
+    YS_DEV void putm(int f, const m33& m) const { for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) put(f + 3 * a + b, m.m[a][b]); }
+};
+struct RecR {   // strided reader
+    const double* p; size_t st;
+    YS_DEV double get(int f) const { return p[(size_t)f * st]; }
+    YS_DEV d3 get3(int f) const { return D3(get(f), get(f + 1), get(f + 2)); }
+    YS_DEV m33 getm(int f) const { m33 m; for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) m.m[a][b] = get(f + 3 * a + b); return m; }
+};
+
+__global__ void __launch_bounds__(256) k_bend_elem(ClothDev C, double* rec, int nb) {
+    int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nb) return;
+    int uc, vc, c, yarn;
+    bend_decode(C, b, uc, vc, c, yarn);
+    int du = c == 0 ? 1 : 0, dv = c == 1 ? 1 : 0;
+    int u2 = uc - du, v2 = vc - dv, u1 = uc + du, v1 = vc + dv;
+    int k2 = u2 + v2 * C.W, k0 = uc + vc * C.W, k1 = u1 + v1 * C.W;
+    double kb = C.kb[yarn];
+    RecW w0 = {rec + b, (size_t)nb}, w1 = {rec + (size_t)BEND_REC * nb + b, (size_t)nb},
+         w2 = {rec + (size_t)2 * BEND_REC * nb + b, (size_t)nb};
+    {
+        double D = uvc(C, k1, c) - uvc(C, k2, c);
+        BendVals<double> o = bend_values<double>(pos(C, k2), pos(C, k0), pos(C, k1), D, kb);
+        m33 L10 = -(o.L11 + o.L12), L20 = -(o.L21 + o.L22);
+        m33 L01 = -(o.L11 + o.L21), L02 = -(o.L12 + o.L22);
+        m33 L00 = -(L10 + L20);
+        d3 L2E2 = -o.L2E1, E2L2 = -o.E1L2;
+        d3 L0E1 = -(o.L1E1 + o.L2E1), L0E2 = -L0E1;
+        d3 E1L0 = -(o.E1L1 + o.E1L2), E2L0 = -E1L0;
+        w0.put3(0, o.Fx2); w0.put(3, o.Fu2); w0.putm(4, o.L22); w0.putm(13, L20); w0.putm(22, o.L21);
+        w0.put3(31, L2E2); w0.put3(34, E2L2); w0.put(37, o.E1E1);
+        w1.put3(0, o.Fx0); w1.putm(3, L02); w1.putm(12, L00); w1.putm(21, L01);
+        w1.put3(30, L0E2); w1.put3(33, E2L0); w1.put3(36, L0E1);
+        w2.put3(0, o.Fx1); w2.put(3, o.Fu1); w2.putm(4, o.L12); w2.putm(13, L10); w2.putm(22, o.L11);
+        w2.put3(31, o.L1E1); w2.put3(34, o.E1L1); w2.put(37, o.E1E1); w2.put3(38, E1L0);
+    }
+    {
+        double D = c == 0 ? (double)u1 * C.L - (double)u2 * C.L : (double)v1 * C.L - (double)v2 * C.L;
+        BendVals<double> o = bend_values<double>(crimp_pos<double>(C, u2, v2, c), crimp_pos<double>(C, uc, vc, c),
+                                                 crimp_pos<double>(C, u1, v1, c), D, kb);
+        double sg = c == 0 ? 1.0 : -1.0;
+        w0.put3(38, sg * o.Fx2); w0.put3(41, sg * (-o.L2E1));
+        w1.put3(39, sg * o.Fx0);
+        w2.put3(41, sg * o.Fx1); w2.put3(44, sg * o.L1E1);
+    }
+}
+
+// Replays bend b's record for node role r, in bend_element's order restricted to the
+// calls that reach that node's rows.
+template <class Sink>
+YS_DEV void emit_bend_role(const ClothDev& C, Sink& S, const double* rec, int nb, int b, int r, int owner) {
+    int uc, vc, c, yarn;
+    bend_decode(C, b, uc, vc, c, yarn);
+    int du = c == 0 ? 1 : 0, dv = c == 1 ? 1 : 0;
+    int u2 = uc - du, v2 = vc - dv, u1 = uc + du, v1 = vc + dv;
+    int k2 = u2 + v2 * C.W, k0 = uc + vc * C.W, k1 = u1 + v1 * C.W;
+    bool i2 = interior(C, u2, v2), i0 = interior(C, uc, vc), i1 = interior(C, u1, v1);
+    RecR R = {rec + (size_t)r * BEND_REC * nb + b, (size_t)nb};
+    if (r == 0) {
+        S.FL(k2, R.get3(0));
+        if (i2) S.FE(k2, c, R.get(3), owner);
+        S.LL(KIND_KX, k2, k2, R.getm(4)); S.LL(KIND_KX, k2, k0, R.getm(13)); S.LL(KIND_KX, k2, k1, R.getm(22));
+        if (i2) { S.LE(KIND_KX, k2, k2, c, R.get3(31)); S.EL(KIND_KX, k2, k2, c, R.get3(34)); S.EE(KIND_KX, k2, k2, c, R.get(37)); }
+        S.SBCL(k2, R.get3(38));
+        if (i2) S.SBCLE(k2, c, R.get3(41));
+    } else if (r == 1) {
+        S.FL(k0, R.get3(0));
+        S.LL(KIND_KX, k0, k2, R.getm(3)); S.LL(KIND_KX, k0, k0, R.getm(12)); S.LL(KIND_KX, k0, k1, R.getm(21));
+        if (i2 && i0) { S.LE(KIND_KX, k0, k2, c, R.get3(30)); S.EL(KIND_KX, k2, k0, c, R.get3(33)); }
+        if (i0 && i1) S.LE(KIND_KX, k0, k1, c, R.get3(36));
+        S.SBCL(k0, R.get3(39));
+    } else {
+        S.FL(k1, R.get3(0));
+        if (i1) S.FE(k1, c, R.get(3), owner);
+        S.LL(KIND_KX, k1, k2, R.getm(4)); S.LL(KIND_KX, k1, k0, R.getm(13)); S.LL(KIND_KX, k1, k1, R.getm(22));
+        if (i1) { S.LE(KIND_KX, k1, k1, c, R.get3(31)); S.EL(KIND_KX, k1, k1, c, R.get3(34)); S.EE(KIND_KX, k1, k1, c, R.get(37)); }
+        if (i0 && i1) S.EL(KIND_KX, k0, k1, c, R.get3(38));
+        S.SBCL(k1, R.get3(41));
+        if (i1) S.SBCLE(k1, c, R.get3(44));
     }
 }
 

@@ -188,12 +188,19 @@ struct AsmOut {
     // the comp tangent read by the shear; detach = the reference's gradient
     double *dA, *dF, *db, *dis_slide, *dcomp;
     int detach;
-    // value run: element-major shear records (k_shear_elem) replayed by k_assemble2
+    // value run: element-major shear and bend records (k_shear_elem, k_bend_elem)
+    // replayed by the node kernels
     const double* shear_rec;
     int nseg;
+    const double* bend_rec;
 };
 
 __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
+
+// Bends in the node kernel: computed in place (the default), or replayed from the
+// element-major records of k_bend_elem.  Measured at 128 x 128: replaying is slower
+// (455 us against 360 us for the node kernel), so the records are off.
+constexpr bool USE_BEND_REC = false;
 
 // Node of thread t in the assembly kernels: one block per 8 x 8 matrix tile, so that a
 // wavefront's read-modify-writes of one slot are 512 contiguous bytes.
@@ -207,7 +214,7 @@ __device__ __forceinline__ int tile_node(const ClothDev& C) {
 // src/TrainHelper.cpp:19-20).  T = dd: their directional derivatives along the
 // tangents held in C, into the d* outputs of O.
 template <class T>
-__global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double h) {
+__global__ void __launch_bounds__(NBLK, 1) k_assemble1(ClothDev C, AsmOut O, double h) {
     typedef v3<T> V;
     constexpr bool tan = !std::is_same<T, double>::value;
     int k = tile_node(C);
@@ -224,34 +231,45 @@ __global__ void __launch_bounds__(NBLK) k_assemble1(ClothDev C, AsmOut O, double
         S.FL(k, (-hk) * (pos<T>(C, k) - xf));
         S.LL(KIND_KX, k, k, I33T<T>(-hk));
     }
-    // edges (list order: warp rows, then weft columns; src/Cloth.cpp:190-228)
+    // edges (list order: warp rows, then weft columns; src/Cloth.cpp:190-228).  One call
+    // site in a loop that is not unrolled: the inlined element code is large, and four or
+    // six copies of it would not fit the instruction cache.
     int ne_warp = C.Lg * (C.W - 1);
-    if (u > 0) edge_element<T>(C, S, u - 1, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u - 1));
-    if (u < C.W - 1) edge_element<T>(C, S, u, v, 0, v & 1, omp_owner(C.n_edges, v * (C.W - 1) + u));
-    if (v > 0) edge_element<T>(C, S, u, v - 1, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v - 1));
-    if (v < C.Lg - 1) edge_element<T>(C, S, u, v, 1, u & 1, omp_owner(C.n_edges, ne_warp + u * (C.Lg - 1) + v));
-    // bends centred on u-1, u, u+1 (warp) and v-1, v, v+1 (weft) (src/Cloth.cpp:230-270)
-    int nb_warp = C.Lg * (C.W - 2);
-    for (int d = -1; d <= 1; ++d) {
-        int uc = u + d;
-        if (uc >= 1 && uc <= C.W - 2) bend_element<T>(C, S, uc, v, 0, v & 1, omp_owner(C.n_bends, v * (C.W - 2) + uc - 1));
+#pragma unroll 1
+    for (int ei = 0; ei < 4; ++ei) {
+        int eu, ev, ec, idx;
+        if (ei == 0) { if (u == 0) continue; eu = u - 1; ev = v; ec = 0; idx = v * (C.W - 1) + u - 1; }
+        else if (ei == 1) { if (u == C.W - 1) continue; eu = u; ev = v; ec = 0; idx = v * (C.W - 1) + u; }
+        else if (ei == 2) { if (v == 0) continue; eu = u; ev = v - 1; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v - 1; }
+        else { if (v == C.Lg - 1) continue; eu = u; ev = v; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v; }
+        edge_element<T>(C, S, eu, ev, ec, ec == 0 ? (v & 1) : (u & 1), omp_owner(C.n_edges, idx));
     }
-    for (int d = -1; d <= 1; ++d) {
-        int vc = v + d;
-        if (vc >= 1 && vc <= C.Lg - 2) bend_element<T>(C, S, u, vc, 1, u & 1, omp_owner(C.n_bends, nb_warp + u * (C.Lg - 2) + vc - 1));
+    // bends centred on u-1, u, u+1 (warp) and v-1, v, v+1 (weft) (src/Cloth.cpp:230-270):
+    // a bend centred d after the node holds it as n2 (role 0), centred on it as n0
+    // (role 1), centred d before it as n1 (role 2)
+    int nb_warp = C.Lg * (C.W - 2);
+#pragma unroll 1
+    for (int bi6 = 0; bi6 < 6; ++bi6) {
+        int d = bi6 % 3 - 1, dir = bi6 / 3;
+        int uc = dir == 0 ? u + d : u, vc = dir == 0 ? v : v + d;
+        if (dir == 0 && (uc < 1 || uc > C.W - 2)) continue;
+        if (dir == 1 && (vc < 1 || vc > C.Lg - 2)) continue;
+        int bi = dir == 0 ? v * (C.W - 2) + uc - 1 : nb_warp + u * (C.Lg - 2) + vc - 1;
+        if constexpr (USE_BEND_REC && !tan) emit_bend_role(C, S, O.bend_rec, C.n_bends, bi, 1 - d, omp_owner(C.n_bends, bi));
+        else bend_element<T>(C, S, uc, vc, dir, dir == 0 ? (v & 1) : (u & 1), omp_owner(C.n_bends, bi));
     }
     // faces of the up to four cells around the node (src/Cloth.cpp:272-286)
     if (C.wind_density != 0.0 || C.wind_drag != 0.0) {
-        for (int cu = u - 1; cu <= u; ++cu)
-            for (int cv = v - 1; cv <= v; ++cv) {
-                if (cu < 0 || cv < 0 || cu > C.W - 2 || cv > C.Lg - 2) continue;
-                int tl = cu + cv * C.W, tr = tl + 1, bl = tl + C.W, br = bl + 1;
-                int f[2][3];
-                if (((cu + cv) & 1) == 0) { f[0][0] = tl; f[0][1] = tr; f[0][2] = br; f[1][0] = tl; f[1][1] = br; f[1][2] = bl; }
-                else { f[0][0] = tl; f[0][1] = tr; f[0][2] = bl; f[1][0] = tr; f[1][1] = br; f[1][2] = bl; }
-                for (int q = 0; q < 2; ++q)
-                    if (f[q][0] == k || f[q][1] == k || f[q][2] == k) face_element<T>(C, S, f[q][0], f[q][1], f[q][2]);
-            }
+#pragma unroll 1
+        for (int fi = 0; fi < 8; ++fi) {
+            int cu = u - 1 + ((fi >> 2) & 1), cv = v - 1 + ((fi >> 1) & 1), q = fi & 1;
+            if (cu < 0 || cv < 0 || cu > C.W - 2 || cv > C.Lg - 2) continue;
+            int tl = cu + cv * C.W, tr = tl + 1, bl = tl + C.W, br = bl + 1;
+            int fa, fb, fc;
+            if (((cu + cv) & 1) == 0) { if (q == 0) { fa = tl; fb = tr; fc = br; } else { fa = tl; fb = br; fc = bl; } }
+            else { if (q == 0) { fa = tl; fb = tr; fc = bl; } else { fa = tr; fb = br; fc = bl; } }
+            if (fa == k || fb == k || fc == k) face_element<T>(C, S, fa, fb, fc);
+        }
     }
     // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
     T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
@@ -297,7 +315,7 @@ __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& 
 __device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
 
 template <class T>
-__global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double h) {
+__global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, double h) {
     constexpr bool tan = !std::is_same<T, double>::value;
     int k = tile_node(C);
     if (k < 0) return;
@@ -306,31 +324,32 @@ __global__ void __launch_bounds__(NBLK) k_assemble2(ClothDev C, AsmOut O, double
     int u = S.mu_, v = S.mv_;
     auto K = [&](int uu, int vv) { return uu + vv * C.W; };
     // shear_segs order per crossing: (left,bot), (left,top), (right,bot), (right,top)
-    for (int cu = u - 1; cu <= u + 1; ++cu)
-        for (int cv = v - 1; cv <= v + 1; ++cv) {
-            if (!interior(C, cu, cv)) continue;
-            if (cu != u && cv != v) continue;
-            int kc = K(cu, cv);
-            int nb1[2] = {K(cu - 1, cv), K(cu + 1, cv)};
-            int nb3[2] = {K(cu, cv + 1), K(cu, cv - 1)};
-            for (int a = 0; a < 2; ++a)
-                for (int bq = 0; bq < 2; ++bq) {
-                    int k1 = nb1[a], k3 = nb3[bq];
-                    if (kc != k && k1 != k && k3 != k) continue;
-                    if constexpr (tan) {
-                        shear_element<T>(C, S, kc, k1, k3, O.comp, O.dcomp);
-                    } else {
-                        // replay the segment's record for this node's role
-                        int sidx = 4 * crossing(C, cu, cv) + 2 * a + bq;
-                        int r = kc == k ? 0 : (k1 == k ? 1 : 2);
-                        const int kk[3] = {kc, k1, k3};
-                        d3 F;
-                        m33 B[3];
-                        shear_load(O.shear_rec, O.nseg, sidx, r, F, B);
-                        emit_shear_role<double>(S, kk, r, F, B);
-                    }
-                }
+#pragma unroll 1
+    for (int ci = 0; ci < 9; ++ci) {
+        int cu = u - 1 + ci / 3, cv = v - 1 + ci % 3;
+        if (!interior(C, cu, cv)) continue;
+        if (cu != u && cv != v) continue;
+        int kc = K(cu, cv);
+#pragma unroll 1
+        for (int sq = 0; sq < 4; ++sq) {
+            int a = sq >> 1, bq = sq & 1;
+            int k1 = a == 0 ? K(cu - 1, cv) : K(cu + 1, cv);
+            int k3 = bq == 0 ? K(cu, cv + 1) : K(cu, cv - 1);
+            if (kc != k && k1 != k && k3 != k) continue;
+            if constexpr (tan) {
+                shear_element<T>(C, S, kc, k1, k3, O.comp, O.dcomp);
+            } else {
+                // replay the segment's record for this node's role
+                int sidx = 4 * crossing(C, cu, cv) + 2 * a + bq;
+                int r = kc == k ? 0 : (k1 == k ? 1 : 2);
+                const int kk[3] = {kc, k1, k3};
+                d3 F;
+                m33 B[3];
+                shear_load(O.shear_rec, O.nseg, sidx, r, F, B);
+                emit_shear_role<double>(S, kk, r, F, B);
+            }
         }
+    }
     int pk = padidx(O.Wp, u, v);
     if (tan) {
         for (int c = 0; c < 3; ++c) {
@@ -1327,6 +1346,7 @@ struct yc_ctx {
     std::vector<double> tpar;               // ndir x YC_NPARAM
     double *dA, *dF, *db, *dis_slide, *dcomp, *At, *xsol, *bsave, *dsol, *trhs;
     double* shear_rec;                      // 3 roles x 30 x (4 Nc) shear records
+    double* bend_rec;                       // 3 roles x BEND_REC x n_bends bend records
 };
 
 namespace {
@@ -1363,6 +1383,7 @@ AsmOut make_out(yc_ctx* c) {
     O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
     O.detach = 0;
     O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
+    O.bend_rec = c->bend_rec;
     O.invdiag = c->invd; O.ext = c->has_ext ? c->ext : nullptr; O.flag = c->flag; O.Wp = c->Wp; O.P = c->P;
     return O;
 }
@@ -1538,6 +1559,9 @@ int do_assemble(yc_ctx* c, double h) {
     HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
     static_assert(NBLK == TILE_N, "assembly blocks are matrix tiles");
+    if constexpr (USE_BEND_REC) if (c->cd.n_bends > 0)
+        hipLaunchKernelGGL(k_bend_elem, dim3((c->cd.n_bends + 255) / 256), dim3(256), 0, c->stream, c->cd, c->bend_rec,
+                           c->cd.n_bends);
     hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     if (c->Nc > 0)
@@ -1804,6 +1828,10 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     c->cg1_kp = std::min((c->nT + BLK - 1) / BLK, CG1_MAXP);
     chk(dalloc(&c->cgslot, (size_t)2 * c->nT));
     chk(dalloc(&c->shear_rec, (size_t)90 * 4 * c->Nc + 1));
+    {
+        size_t nb = (size_t)c->Lg * (c->W - 2) + (size_t)c->W * (c->Lg - 2);
+        chk(dalloc(&c->bend_rec, (size_t)3 * BEND_REC * nb + 1));
+    }
     chk(dalloc(&c->cg_err, 1));
     chk(dalloc(&c->part1, (size_t)12 * c->cg1_kp * BLK));
     c->part.nS = c->nT;
@@ -1863,7 +1891,7 @@ void yc_destroy(yc_ctx* c) {
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
                     c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
                     c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs,
-                    c->shear_rec};
+                    c->shear_rec, c->bend_rec};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
