@@ -202,6 +202,10 @@ __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (
 // (455 us against 360 us for the node kernel), so the records are off.
 constexpr bool USE_BEND_REC = false;
 
+// LDS of the value assembly kernels: one tile's slot planes (81,408 bytes, above the
+// 64 KiB default, allowed per kernel at context creation)
+constexpr size_t ASM_LDS = (size_t)NSLOT * TILE_N * sizeof(double);
+
 // Node of thread t in the assembly kernels: one block per 8 x 8 matrix tile, so that a
 // wavefront's read-modify-writes of one slot are 512 contiguous bytes.
 __device__ __forceinline__ int tile_node(const ClothDev& C) {
@@ -222,6 +226,12 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble1(ClothDev C, AsmOut O, dou
     RowSinkT<T> S;
     S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
     int u = S.mu_, v = S.mv_;
+    // value run: the node's 159 slots accumulate in LDS (the tile's slot planes, same
+    // layout as in HBM) and go to HBM once at the end, instead of a global
+    // read-modify-write per contribution
+    extern __shared__ double lds_rows[];
+    double* gA = S.A;
+    if (!tan) S.A = lds_rows + threadIdx.x;
     if (S.A) for (int s = 0; s < NSLOT; ++s) S.A[(size_t)s * TILE_N] = 0.0;
     if (S.dA) for (int s = 0; s < NSLOT; ++s) S.dA[(size_t)s * TILE_N] = 0.0;
     // handle (src/PhysicsCloth.cpp:232-248)
@@ -278,6 +288,7 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble1(ClothDev C, AsmOut O, dou
     int pk = padidx(O.Wp, u, v);
     int e = S.inter ? crossing(C, u, v) : 0;
     if (!tan) {
+        for (int s = 0; s < NSLOT; ++s) gA[(size_t)s * TILE_N] = S.A[(size_t)s * TILE_N];
         for (int c = 0; c < 5; ++c) O.F[c * C.N + k] = val(S.F[c]);
         O.comp[k] = val(comp);
         O.is_slide[k] = val(sl[0]); O.is_slide[C.N + k] = val(sl[1]);
@@ -322,6 +333,12 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, dou
     RowSinkT<T> S;
     S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
     int u = S.mu_, v = S.mv_;
+    extern __shared__ double lds_rows[];
+    double* gA = S.A;
+    if (!tan) {   // the node's slots into LDS, back to HBM at the end
+        S.A = lds_rows + threadIdx.x;
+        for (int s = 0; s < NSLOT; ++s) S.A[(size_t)s * TILE_N] = gA[(size_t)s * TILE_N];
+    }
     auto K = [&](int uu, int vv) { return uu + vv * C.W; };
     // shear_segs order per crossing: (left,bot), (left,top), (right,bot), (right,top)
 #pragma unroll 1
@@ -374,8 +391,11 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, dou
         O.invdiag[c * O.P + pk] = inv;
         if (!isfinite(O.b[c * O.P + pk])) bad = true;
     }
-    for (int s = 0; s < NSLOT; ++s)
-        if (!isfinite(S.A[(size_t)s * TILE_N])) bad = true;
+    for (int s = 0; s < NSLOT; ++s) {
+        double a = S.A[(size_t)s * TILE_N];
+        if (!isfinite(a)) bad = true;
+        gA[(size_t)s * TILE_N] = a;
+    }
     if (bad) atomicOr(&O.flag[1], 1);
     if (S.bad) atomicOr(&O.flag[0], 1);
 }
@@ -1562,12 +1582,12 @@ int do_assemble(yc_ctx* c, double h) {
     if constexpr (USE_BEND_REC) if (c->cd.n_bends > 0)
         hipLaunchKernelGGL(k_bend_elem, dim3((c->cd.n_bends + 255) / 256), dim3(256), 0, c->stream, c->cd, c->bend_rec,
                            c->cd.n_bends);
-    hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
                            c->shear_rec, 4 * c->Nc);
-    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
                        c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
@@ -1850,6 +1870,11 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     if (!rc && hipHostMalloc((void**)&c->h_flag, 4 * sizeof(int)) != hipSuccess) rc = fail(YC_ERR_HIP, "pinned alloc");
     if (rc) { yc_destroy(c); return rc; }
     c->coop_ok = cgp_fits(c) ? 1 : 0;
+    if (hipFuncSetAttribute((const void*)k_assemble1<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess ||
+        hipFuncSetAttribute((const void*)k_assemble2<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess) {
+        yc_destroy(c);
+        return fail(YC_ERR_HIP, "assembly LDS size not supported");
+    }
     // constructor rest state (src/Cloth.cpp:52-99) and flags
     std::vector<double> x(3 * N), uv(2 * N);
     std::vector<unsigned char> fl(N, 0);
