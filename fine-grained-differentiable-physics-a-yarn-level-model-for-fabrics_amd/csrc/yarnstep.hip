@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -80,6 +81,10 @@ struct CGScal {
     // the iteration count reported at the stop
     double alpha[2], gamma[2];
     int fin_iters;
+    // persistent CG over many steps without host synchronisation: totals since the last
+    // reset (iterations, solves stopped at maxit, solves with a non-finite residual)
+    long long total_iters;
+    int nonconv, nonfinite;
 };
 
 // Partial sums of the CG reductions, one slot per block of the producing kernel.  A
@@ -978,11 +983,17 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
     }
     double thr = a.tol * a.tol * bb;
     thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
-    if (bb == 0.0 || a.maxit <= 0) {
+    if (bb == 0.0 || a.maxit <= 0 || !isfinite(bb) || !isfinite(rr0)) {
         if (lead) {
             a.sc->bb = bb; a.sc->rr = rr0; a.sc->thr = thr; a.sc->maxit = a.maxit;
             a.sc->zero_rhs = bb == 0.0; a.sc->fin_iters = 0; a.sc->itc[0] = a.sc->itc[1] = 0; a.sc->done = 1;
+            if (!isfinite(bb) || !isfinite(rr0)) a.sc->nonfinite = 1;
+            else if (bb != 0.0 && !(rr0 < thr)) a.sc->nonconv += 1;
         }
+        if (bb == 0.0)   // the reference solver returns x = 0 for a zero right-hand side
+#pragma unroll
+            for (int t = 0; t < CG1_NE; ++t)
+                if (e_in[t] && e_own[t]) a.x[e_i[t]] = 0.0;
         return;
     }
     // stage -1: w_0 = A D^-1 r_0 and the partials of iteration 0
@@ -1026,12 +1037,16 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
                 for (int m = 0; m < 6; ++m) sred[m] = acc[m];
             __syncthreads();
             const double gam = sred[0], d1 = sred[1], d2 = sred[2], d3 = sred[3], d4 = sred[4], rr = sred[5];
-            if (rr < thr || j >= a.maxit) {
+            const bool finite = isfinite(rr);
+            if (rr < thr || j >= a.maxit || !finite) {
                 if (lead) {
                     a.sc->bb = bb; a.sc->rr = rr; a.sc->thr = thr; a.sc->maxit = a.maxit; a.sc->zero_rhs = 0;
                     a.sc->fin_iters = rr < thr ? (j > 0 ? j - 1 : 0) : j;
                     a.sc->itc[0] = a.sc->itc[1] = a.sc->fin_iters;
                     a.sc->done = 1;
+                    a.sc->total_iters += a.sc->fin_iters;
+                    if (!finite) a.sc->nonfinite = 1;
+                    else if (!(rr < thr)) a.sc->nonconv += 1;
                 }
                 break;
             }
@@ -1575,8 +1590,9 @@ void from_padded(const yc_ctx* c, const std::vector<double>& pad, double* glob) 
         }
 }
 
-int do_assemble(yc_ctx* c, double h) {
-    HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+// async: no host synchronisation; the flags are sticky and read by the caller later
+int do_assemble(yc_ctx* c, double h, bool async = false) {
+    if (!async) HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
     static_assert(NBLK == TILE_N, "assembly blocks are matrix tiles");
     if constexpr (USE_BEND_REC) if (c->cd.n_bends > 0)
@@ -1591,6 +1607,11 @@ int do_assemble(yc_ctx* c, double h) {
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
                        c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
+    if (async) {
+        HIPCHK(hipGetLastError());
+        c->assembled = true;
+        return YC_OK;
+    }
     HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->assembled = true;
@@ -1612,7 +1633,8 @@ int finish_solve(yc_ctx* c, bool fused, int* iters, double* relres) {
     return YC_OK;
 }
 
-int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* iters, double* relres, bool x0_in_cx = false) {
+int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* iters, double* relres, bool x0_in_cx = false,
+             bool async = false) {
     if (!c->assembled) return fail(YC_ERR_ARG, "yc_solve before yc_assemble");
     int has_x0 = 0;
     if (x0_in_cx) {
@@ -1633,6 +1655,10 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
     if (persistent) {
         int rc = launch_cgp(c, tol, maxit);
         if (rc) return rc;
+        if (async) {   // results accumulate in sc; the caller synchronises once
+            c->solved = true;
+            return YC_OK;
+        }
         HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(c->h_flag + 2, c->cg_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -2159,8 +2185,47 @@ int yc_step(yc_ctx* c, double h, const double* x0, double tol, int maxit, yc_ste
     return rc;
 }
 
+}  // extern "C"
+
+namespace {
+
+int warm_guess(yc_ctx* c, yc_mlp* m);
+
+// Steps back to back with the persistent CG and no host synchronisation until the end:
+// assembly flags and the solves' outcomes accumulate on the device.
+int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters) {
+    g_err.clear();
+    HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync((char*)c->sc + offsetof(CGScal, total_iters), 0, sizeof(long long) + 2 * sizeof(int), c->stream));
+    for (int s = 0; s < steps; ++s) {
+        int rc = do_assemble(c, h, true);
+        if (!rc && m) rc = warm_guess(c, m);
+        if (!rc) rc = do_solve(c, nullptr, tol, maxit, nullptr, nullptr, m != nullptr, true);
+        if (!rc) rc = do_update(c, h);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_flag + 2, c->cg_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (total_iters) *total_iters = c->h_sc->total_iters;
+    if (c->h_flag[2]) {
+        (void)hipMemsetAsync(c->cg_err, 0, sizeof(int), c->stream);
+        return fail(YC_ERR_HIP, "persistent CG: a block waited past its spin limit");
+    }
+    if (c->h_flag[0]) return fail(YC_ERR_HIP, "assembly: stencil slot out of range");
+    if (c->h_flag[1] || c->h_sc->nonfinite) return fail(YC_ERR_NONFINITE, "NaN/Inf in A, b or a solution");
+    if (c->h_sc->nonconv) return fail(YC_ERR_NOT_CONVERGED, "a CG solve stopped at maxit above tolerance");
+    return YC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* total_iters) {
     if (!c) return fail(YC_ERR_ARG, "null context");
+    if (c->cg_variant == YC_CG_PERSISTENT && c->coop_ok) return run_async(c, nullptr, h, steps, tol, maxit, total_iters);
     long long tot = 0;
     int worst = YC_OK;
     for (int s = 0; s < steps; ++s) {
@@ -2607,6 +2672,7 @@ int yc_step_warm(yc_ctx* c, yc_mlp* m, double h, double tol, int maxit, yc_step_
 
 int yc_run_warm(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters) {
     if (!c || !m) return fail(YC_ERR_ARG, "null argument");
+    if (c->cg_variant == YC_CG_PERSISTENT && c->coop_ok) return run_async(c, m, h, steps, tol, maxit, total_iters);
     long long tot = 0;
     int worst = YC_OK;
     for (int s = 0; s < steps; ++s) {

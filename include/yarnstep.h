@@ -130,7 +130,10 @@ int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_s
 int yc_set_cg_variant(yc_ctx* ctx, int variant);
 
 /* Runs `steps` SimSteps back to back on the device with zero-guess CG; returns the
- * total CG iterations.  Host synchronises once per CG chunk only. */
+ * total CG iterations and the worst status.  With YC_CG_PERSISTENT the steps are queued
+ * without any host synchronisation (assembly flags and solve outcomes accumulate on the
+ * device), so a NaN/Inf or a solve stopped at maxit is reported after the last step;
+ * the other variants synchronise once per CG chunk and stop at the first NaN/Inf. */
 int yc_run(yc_ctx* ctx, double h, int steps, double tol, int maxit, long long* total_iters);
 
 /* Times `reps` launches of the CG loop's stencil product y = A p on the context's
