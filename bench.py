@@ -37,15 +37,19 @@ def cpu_baseline(cs, name, nsteps):
             "sample": "steps 2..%d of the %s workload, oracle single-thread FP64 CG to 1e-8" % (nsteps + 1, name)}
 
 
-def measured_traffic(kernel, workload):
+def measured_traffic(kernel, workload, iterations=None):
     """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE +
-    WRITE_SIZE runs, scaled by the copy calibration), or None when there is none."""
+    WRITE_SIZE runs, scaled by the copy calibration), or None when there is none.  A
+    solve kernel's summary gives bytes per CG iteration; `iterations` scales it to the
+    launch being reported."""
     path = os.path.join(ROOT, "profiles", "%s_%s_pmc.md" % (kernel, workload))
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
+        if line.startswith("traffic_bytes_per_iteration:") and iterations:
+            return float(line.split(":", 1)[1]) * iterations
     return None
 
 
@@ -114,10 +118,25 @@ def main():
         c_ref.close()
         ref_form = {"steps_per_s": args.steps / dt_ref, "cg_iters_per_step": it_ref / args.steps}
     cloth, dt_plain, it_plain = timed(False)
-    # dominant kernel: stencil SpMV of the CG loop, timed with HIP events on the context stream
+    # the stencil SpMV (the reference-form CG's product), timed with HIP events on the
+    # context stream
+    cloth.assemble(h)
     spmv_ms = cloth.time_spmv(200)
     spmv_bytes = cloth.spmv_bytes()
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    # the dominant kernel: one persistent-CG solve per step (k_cgp), timed with an event
+    # pair around each cooperative launch; its algorithmic bytes are the solve's
+    # iterations times one Jacobi-PCG iteration streamed from memory
+    dominant = None
+    if args.cg == "persistent":
+        cloth.assemble(h)
+        solve_ms, solve_iters = cloth.time_solve(tol=tol, reps=10)
+        solve_bytes = solve_iters * cloth.cg_iteration_bytes()
+        dominant = {"bound": "hbm", "achieved": solve_bytes / (solve_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "traffic": measured_traffic("k_cgp", args.workload, solve_iters),
+                    "kernel": "k_cgp (one persistent Jacobi-PCG solve)", "bytes_per_launch": solve_bytes,
+                    "avg_launch_ms": solve_ms, "iterations_per_launch": solve_iters}
+        dominant["frac"] = dominant["achieved"] / HBM_PEAK_GBS
     cloth.close()
     if net is not None:
         cloth, dt_warm, it_warm = timed(True)
@@ -154,10 +173,15 @@ def main():
             "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": mlp_tflops / BF16_PEAK_TFLOPS, "avg_launch_ms": mlp_ms,
                     "rows": spec["width"] * spec["length"]} if net else None},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_spmv", args.workload),
-                     "kernel": "k_spmv", "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms},
+        "roofline": dominant,
     }
+    spmv = {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic("k_spmv", args.workload),
+            "kernel": "k_spmv", "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms}
+    if dominant is None:
+        out["roofline"] = spmv
+    else:
+        out["config"]["spmv"] = spmv
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cs, args.workload, args.cpu_steps)
     print(json.dumps(out), flush=True)

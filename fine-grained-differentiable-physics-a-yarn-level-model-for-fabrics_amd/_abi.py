@@ -49,7 +49,8 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_mlp_param_count", "yc_mlp_create", "yc_mlp_destroy", "yc_mlp_set_params", "yc_mlp_get_params",
            "yc_mlp_set_norm", "yc_mlp_set_data", "yc_mlp_forward", "yc_mlp_grad", "yc_mlp_train", "yc_features",
            "yc_targets", "yc_step_warm", "yc_run_warm", "yc_mlp_time_forward", "yc_set_cg_variant",
-           "yc_tangent_begin", "yc_step_tangent", "yc_tangent_get", "yc_tangent_end"]
+           "yc_tangent_begin", "yc_step_tangent", "yc_tangent_get", "yc_tangent_end",
+           "yc_time_solve", "yc_cg_iteration_bytes"]
 
 
 def lib_path():
@@ -94,6 +95,9 @@ def load_library():
     L.yc_step_tangent.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]
     L.yc_tangent_get.argtypes = [vp, ctypes.c_int] + [dp] * 6
     L.yc_tangent_end.argtypes = [vp]
+    L.yc_time_solve.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, ip]
+    L.yc_cg_iteration_bytes.argtypes = [vp]
+    L.yc_cg_iteration_bytes.restype = ctypes.c_double
     L.yc_spmv_bytes.argtypes = [vp]
     L.yc_spmv_bytes.restype = ctypes.c_double
     fp = ctypes.POINTER(ctypes.c_float)

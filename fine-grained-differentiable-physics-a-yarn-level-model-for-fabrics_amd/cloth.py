@@ -234,6 +234,19 @@ class Cloth:
         _abi.check(_abi.load_library().yc_time_spmv(self._h, int(reps), ctypes.byref(ms)))
         return ms.value
 
+    def time_solve(self, tol=1e-8, maxit=None, reps=5):
+        """Mean duration (ms) of a persistent-CG solve of the assembled system, and its
+        iteration count."""
+        if maxit is None:
+            maxit = 2 * self.n
+        ms, it = ctypes.c_double(), ctypes.c_int()
+        _abi.check(_abi.load_library().yc_time_solve(self._h, float(tol), int(maxit), int(reps), ctypes.byref(ms),
+                                                     ctypes.byref(it)))
+        return ms.value, it.value
+
+    def cg_iteration_bytes(self):
+        return _abi.load_library().yc_cg_iteration_bytes(self._h)
+
     def spmv_bytes(self):
         return _abi.load_library().yc_spmv_bytes(self._h)
 

@@ -2366,6 +2366,41 @@ int yc_time_spmv(yc_ctx* c, int reps, double* avg_ms) {
     return st;
 }
 
+int yc_time_solve(yc_ctx* c, double tol, int maxit, int reps, double* avg_ms, int* iters) {
+    if (!c || reps <= 0 || !avg_ms) return fail(YC_ERR_ARG, "bad argument");
+    if (!c->assembled) return fail(YC_ERR_ARG, "yc_time_solve before yc_assemble");
+    if (c->cg_variant != YC_CG_PERSISTENT || !c->coop_ok) return fail(YC_ERR_ARG, "yc_time_solve times the persistent CG");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(YC_ERR_HIP, "hipEventCreate"); }
+    double tot = 0.0;
+    int rc = YC_OK;
+    for (int i = 0; i <= reps && !rc; ++i) {   // the first launch is untimed
+        hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 0);
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(YC_ERR_HIP, "hipEventRecord"); break; }
+        rc = launch_cgp(c, tol, maxit);
+        if (rc) break;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = fail(YC_ERR_HIP, "event"); break; }
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = fail(YC_ERR_HIP, "hipEventElapsedTime"); break; }
+        if (i > 0) tot += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost));
+    *avg_ms = tot / reps;
+    if (iters) *iters = c->h_sc->fin_iters;
+    return YC_OK;
+}
+
+double yc_cg_iteration_bytes(const yc_ctx* c) {
+    if (!c) return 0.0;
+    // one Jacobi-PCG iteration streamed from memory: A once, and 15 vector passes
+    // (p, y, x, r, z, D^-1 reads and writes) of 8 bytes per DOF
+    return yc_spmv_bytes(c) - 16.0 * c->n + 120.0 * c->n;
+}
+
 double yc_spmv_bytes(const yc_ctx* c) {
     if (!c) return 0.0;
     // structural nonzeros of A (LL 13-point stencil + LE/EL/EE along the yarns)

@@ -144,6 +144,13 @@ int yc_time_spmv(yc_ctx* ctx, int reps, double* avg_ms);
 /* Algorithmic HBM bytes of one product launch: 8 bytes per structural nonzero of A
  * plus 16 bytes per DOF (p read, A p written). */
 double yc_spmv_bytes(const yc_ctx* ctx);
+/* Times `reps` persistent-CG solves of the assembled system (YC_CG_PERSISTENT only) with
+ * an event pair around each cooperative launch; avg_ms the mean launch duration, iters
+ * the iteration count of the solve.  Overwrites the CG workspace and the solution. */
+int yc_time_solve(yc_ctx* ctx, double tol, int maxit, int reps, double* avg_ms, int* iters);
+/* Algorithmic bytes of one Jacobi-PCG iteration streamed from memory: 8 per structural
+ * nonzero of A plus 120 per DOF (15 vector passes). */
+double yc_cg_iteration_bytes(const yc_ctx* ctx);
 
 /* ---------------------------------------------------------------------------
  * Learned CG warm start.  No counterpart in the reference (whose only network,
