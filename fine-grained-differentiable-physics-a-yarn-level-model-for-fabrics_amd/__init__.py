@@ -12,3 +12,4 @@ from ._abi import (YcError, NotConverged, NonFinite, YC_OK, YC_ERR_NONFINITE,  #
                    YC_ERR_NOT_CONVERGED, YC_ERR_HIP, YC_ERR_ARG, lib_path, load_library)
 from .cloth import Cloth, PhysicsCloth, SimStep  # noqa: F401
 from . import grad  # noqa: F401
+from .batch import ClothBatch, autotune_batch, autotune_cloth  # noqa: F401

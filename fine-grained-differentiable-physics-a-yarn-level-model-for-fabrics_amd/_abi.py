@@ -38,6 +38,10 @@ class YcDesc(ctypes.Structure):
     ]
 
 
+class YcBatchOpts(ctypes.Structure):
+    _fields_ = [("graph", ctypes.c_int), ("group_order", ctypes.c_int), ("node_block", ctypes.c_int)]
+
+
 class YcStepInfo(ctypes.Structure):
     _fields_ = [("cg_iters", ctypes.c_int), ("cg_relres", ctypes.c_double), ("status", ctypes.c_int)]
 
@@ -50,7 +54,9 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_mlp_set_norm", "yc_mlp_set_data", "yc_mlp_forward", "yc_mlp_grad", "yc_mlp_train", "yc_features",
            "yc_targets", "yc_step_warm", "yc_run_warm", "yc_mlp_time_forward", "yc_set_cg_variant",
            "yc_tangent_begin", "yc_step_tangent", "yc_tangent_get", "yc_tangent_end",
-           "yc_time_solve", "yc_cg_iteration_bytes"]
+           "yc_time_solve", "yc_cg_iteration_bytes",
+           "yc_batch_create", "yc_batch_destroy", "yc_batch_set_opts", "yc_batch_get_opts", "yc_batch_run",
+           "yc_batch_info"]
 
 
 def lib_path():
@@ -118,6 +124,14 @@ def load_library():
     L.yc_step_warm.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]
     L.yc_run_warm.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                               ctypes.POINTER(ctypes.c_longlong)]
+    L.yc_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    L.yc_batch_destroy.argtypes = [vp]
+    L.yc_batch_destroy.restype = None
+    L.yc_batch_set_opts.argtypes = [vp, ctypes.POINTER(YcBatchOpts)]
+    L.yc_batch_get_opts.argtypes = [vp, ctypes.POINTER(YcBatchOpts)]
+    L.yc_batch_run.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_longlong), ip]
+    L.yc_batch_info.argtypes = [vp, ip, ip, ip, ip]
     _lib = L
     return L
 

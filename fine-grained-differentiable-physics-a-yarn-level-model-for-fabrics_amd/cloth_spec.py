@@ -152,6 +152,19 @@ def workload(name):
     raise KeyError(name)
 
 
+def batch_workload(n=64, seed=3, lo=10, hi=64):
+    """Cloths of uneven size for batched stepping and parameter learning: square sides
+    drawn uniformly from lo..hi, each a two-yarn blend with its own seed, weave
+    patterns cycling plain / twill / satin.  Returns [(spec, state)]."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(lo, hi + 1, size=n)
+    out = []
+    for i, W in enumerate(sides):
+        spec = heter_spec(int(W), ("plain", "twill", "satin")[i % 3], 1000 * seed + i)
+        out.append((spec, initial_state(spec)))
+    return out
+
+
 def ndof(spec):
     W, Lg = spec["width"], spec["length"]
     return 3 * W * Lg + 2 * (W - 2) * (Lg - 2)

@@ -183,8 +183,7 @@ __device__ __forceinline__ void shear_nodes(const ClothDev& C, int s, int (&k)[3
 // Element-major shear: one thread per segment computes it once and stores, for each of
 // its three nodes, the 30 doubles that node's rows receive (force 3 + three LL blocks
 // 27), as planes rec[(r * 30 + f) * nseg + s].
-__global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* comp, double* rec, int nseg) {
-    int s = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void shear_elem_seg(const ClothDev& C, const double* comp, double* rec, int nseg, int s) {
     if (s >= nseg) return;
     int k[3];
     shear_nodes(C, s, k);
@@ -197,6 +196,10 @@ __global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* co
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) base[(size_t)(3 + j * 9 + a * 3 + b) * nseg] = o.B[r][j].m[a][b];
     }
+}
+
+__global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* comp, double* rec, int nseg) {
+    shear_elem_seg(C, comp, rec, nseg, blockIdx.x * 256 + threadIdx.x);
 }
 
 // Reads the role-r record of segment s.

@@ -213,20 +213,24 @@ constexpr size_t ASM_LDS = (size_t)NSLOT * TILE_N * sizeof(double);
 
 // Node of thread t in the assembly kernels: one block per 8 x 8 matrix tile, so that a
 // wavefront's read-modify-writes of one slot are 512 contiguous bytes.
-__device__ __forceinline__ int tile_node(const ClothDev& C) {
-    int u = (blockIdx.x % C.ntx) * TILE + (threadIdx.x & (TILE - 1));
-    int v = (blockIdx.x / C.ntx) * TILE + (threadIdx.x / TILE);
+__device__ __forceinline__ int tile_node(const ClothDev& C, int tile) {
+    int u = (tile % C.ntx) * TILE + (threadIdx.x & (TILE - 1));
+    int v = (tile / C.ntx) * TILE + (threadIdx.x / TILE);
     return (u < C.W && v < C.Lg) ? u + v * C.W : -1;
 }
 
 // T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
 // src/TrainHelper.cpp:19-20).  T = dd: their directional derivatives along the
 // tangents held in C, into the d* outputs of O.
+// The body stays in the kernel itself.  Built as an inlined device function called by
+// a kernel (by reference or by value), this compiler produced wrong rows for the nodes
+// of tiles where some lanes skip an edge (boundary nodes: one edge's self block landed
+// two nodes away); the batched step therefore launches this kernel once per cloth.
 template <class T>
 __global__ void __launch_bounds__(NBLK, 1) k_assemble1(ClothDev C, AsmOut O, double h) {
     typedef v3<T> V;
     constexpr bool tan = !std::is_same<T, double>::value;
-    int k = tile_node(C);
+    int k = tile_node(C, blockIdx.x);
     if (k < 0) return;
     RowSinkT<T> S;
     S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
@@ -331,9 +335,9 @@ __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& 
 __device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
 
 template <class T>
-__global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, double h) {
+__device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& O, double h, int tile) {
     constexpr bool tan = !std::is_same<T, double>::value;
-    int k = tile_node(C);
+    int k = tile_node(C, tile);
     if (k < 0) return;
     RowSinkT<T> S;
     S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
@@ -403,6 +407,11 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, dou
     }
     if (bad) atomicOr(&O.flag[1], 1);
     if (S.bad) atomicOr(&O.flag[0], 1);
+}
+
+template <class T>
+__global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, double h) {
+    assemble2_tile<T>(C, O, h, blockIdx.x);
 }
 
 }  // namespace
@@ -517,8 +526,8 @@ __device__ __forceinline__ int vec_index(const VecArgs& a, int t, int& c) {
 
 // init: r = b - y (y = A x0 when has_x0, else x = 0, r = b); p = D^-1 r; partials of
 // r.r, r.z (into parity buffer 1, read by iteration 0) and b.b
-__global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, CGPart P, int has_x0) {
-    int t = blockIdx.x * BLK + threadIdx.x;
+__device__ __forceinline__ void cg_init_block(const VecArgs& a, const CGPart& P, int has_x0, int blk) {
+    int t = blk * BLK + threadIdx.x;
     double rr = 0.0, rz = 0.0, bb = 0.0;
     if (t < 5 * a.N) {
         int c;
@@ -534,8 +543,10 @@ __global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, CGPart P, int has_x0
     double s1 = block_sum(rr);
     double s2 = block_sum(rz);
     double s3 = block_sum(bb);
-    if (threadIdx.x == 0) { P.rr[blockIdx.x] = s1; P.rz[1][blockIdx.x] = s2; P.bb[blockIdx.x] = s3; }
+    if (threadIdx.x == 0) { P.rr[blk] = s1; P.rz[1][blk] = s2; P.bb[blk] = s3; }
 }
+
+__global__ void __launch_bounds__(BLK) k_cg_init(VecArgs a, CGPart P, int has_x0) { cg_init_block(a, P, has_x0, blockIdx.x); }
 
 // one block: stopping threshold and the initial decision (src/Solvers.cpp:22-24 semantics)
 __global__ void __launch_bounds__(BLK) k_cg_init2(CGPart P, CGScal* sc, double tol, int maxit) {
@@ -915,26 +926,26 @@ __device__ bool cgp_wait(const CgpArgs& a, const CgSlot* sl, unsigned long long 
 }
 
 // Publishes this block's partials and tag after all of its stores have completed.
-__device__ __forceinline__ void cgp_publish(const CgpArgs& a, CgSlot* sl, const double* acc, unsigned long long tag) {
+__device__ __forceinline__ void cgp_publish(const CgpArgs& a, CgSlot* sl, const double* acc, unsigned long long tag, int tile) {
     if (threadIdx.x == 0)
-        for (int m = 0; m < 6; ++m) st_dev(&sl[blockIdx.x].v[m], acc[m]);
+        for (int m = 0; m < 6; ++m) st_dev(&sl[tile].v[m], acc[m]);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_s_waitcnt(0);
-        __hip_atomic_store(&sl[blockIdx.x].tag, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&sl[tile].tag, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
+// The solve of one tile's block; `tile` numbers the block within its system.
 template <int KP>
-__global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
+__device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned long long epoch) {
     __shared__ double su[5][HALO_N];
     __shared__ double sp[5][TILE_N];
     __shared__ double sred[8];
-    const int tile = blockIdx.x;
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int tu0 = (tile % a.ntx) * TILE, tv0 = (tile / a.ntx) * TILE;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = tile == 0 && threadIdx.x == 0;
     // resident data: matrix rows, entry indices, D^-1, x and p of the owned entries
     const int nu = tu0 + (lane & 7), nv = tv0 + (lane >> 3);
     const bool node_in = nu < a.W && nv < a.Lg;
@@ -1002,7 +1013,7 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
         const int q = j & 1;
         const bool init = j < 0;
         const bool first = j == 0;
-        if (!init && !cgp_wait<KP>(a, a.slot[q], (a.epoch << 32) | (unsigned long long)j)) return;   // error
+        if (!init && !cgp_wait<KP>(a, a.slot[q], (epoch << 32) | (unsigned long long)j)) return;   // error
         // one round trip for the partials of every tile and the inputs of tile + halo
         // (r_j, w_j, s_{j-1}); loads whose values an init or first stage ignores are
         // still issued (their buffers exist), so that none of them sits in a branch
@@ -1126,13 +1137,16 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) {
                     st_dev(r_out + e_i[t], o_r[t]);
                 }
         }
-        cgp_publish(a, a.slot[init ? 0 : 1 - q], acc, (a.epoch << 32) | (unsigned long long)(j + 1));
+        cgp_publish(a, a.slot[init ? 0 : 1 - q], acc, (epoch << 32) | (unsigned long long)(j + 1), tile);
     }
     // the solution of the owned entries
 #pragma unroll
     for (int t = 0; t < CG1_NE; ++t)
         if (e_in[t] && e_own[t]) a.x[e_i[t]] = x_reg[t];
 }
+
+template <int KP>
+__global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) { cgp_tile<KP>(a, blockIdx.x, a.epoch); }
 
 // ---------------------------------------------------------------------------
 // Update (src/Cloth.cpp:111-148) and geometry refresh (src/Cloth.cpp:307-348)
@@ -1143,8 +1157,8 @@ struct StateDev {
     int W, Lg, N;
 };
 
-__global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h) {
-    int k = blockIdx.x * NBLK + threadIdx.x;
+__device__ __forceinline__ void update_node(const StateDev& s, const double* sol, const double* is_slide, int Wp, int P,
+                                            double h, int k) {
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
     int pk = padidx(Wp, u, v);
@@ -1164,6 +1178,10 @@ __global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, 
             s.uvbar[c * s.N + k] = s.uvbar[c * s.N + k] + is_slide[c * s.N + k] * nv * h;
         }
     }
+}
+
+__global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h) {
+    update_node(s, sol, is_slide, Wp, P, h, blockIdx.x * NBLK + threadIdx.x);
 }
 
 // Cloth::NodeNorm (src/Cloth.cpp:440-463): smallest-eigenvalue eigenvector of the 3x3
@@ -1224,8 +1242,7 @@ __device__ void node_frame(const StateDev& s, int k, NodeFrame& f) {
     f.sg = sg;
 }
 
-__global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
-    int k = blockIdx.x * NBLK + threadIdx.x;
+__device__ __forceinline__ void normals_node(const StateDev& s, int k) {
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
     if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
@@ -1239,6 +1256,8 @@ __global__ void __launch_bounds__(NBLK) k_normals(StateDev s) {
     double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = f.sg * (n[c] / nn);
 }
+
+__global__ void __launch_bounds__(NBLK) k_normals(StateDev s) { normals_node(s, blockIdx.x * NBLK + threadIdx.x); }
 
 // Tangent of the crossing normals along the position tangent dx: first-order
 // perturbation of the smallest eigenvector of the scatter matrix,
@@ -1373,6 +1392,7 @@ struct yc_ctx {
     CgSlot* cgslot;                         // 2 parities x nT slots of the persistent CG
     int* cg_err;
     unsigned long long epoch;               // solves run by the persistent CG
+    unsigned long long* bepoch;             // device counter of the batched solves' epochs
     int coop_ok;                            // the persistent CG fits on the device at once
     hipGraphExec_t graph1;
     // forward-mode tangents of the step (directional derivatives)
@@ -1879,6 +1899,11 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
         chk(dalloc(&c->bend_rec, (size_t)3 * BEND_REC * nb + 1));
     }
     chk(dalloc(&c->cg_err, 1));
+    chk(dalloc(&c->bepoch, 1));
+    if (!rc) {   // batched solves tag in their own range: 2^30 + n, host-launched solves count from 1
+        unsigned long long e0 = 1ULL << 30;
+        if (hipMemcpy(c->bepoch, &e0, sizeof(e0), hipMemcpyHostToDevice) != hipSuccess) rc = fail(YC_ERR_HIP, "upload");
+    }
     chk(dalloc(&c->part1, (size_t)12 * c->cg1_kp * BLK));
     c->part.nS = c->nT;
     c->part.nV = grid_for(5LL * c->N);
@@ -1942,7 +1967,7 @@ void yc_destroy(yc_ctx* c) {
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
                     c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
                     c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs,
-                    c->shear_rec, c->bend_rec};
+                    c->shear_rec, c->bend_rec, c->bepoch};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
@@ -2722,3 +2747,5 @@ int yc_run_warm(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit
 }
 
 }  // extern "C"
+
+#include "batch.hpp"

@@ -237,6 +237,36 @@ int yc_tangent_get(yc_ctx* ctx, int dir, double* dlpos, double* dlposfix, double
 /* Frees the tangent buffers. */
 int yc_tangent_end(yc_ctx* ctx);
 
+/* ---------------------------------------------------------------------------------
+ * Batches: many cloths of uneven size stepped together, each phase of a step one launch
+ * over all of them (the persistent CG one cooperative launch per group of cloths that
+ * fit on the device at once), replayed from a hipGraph where the runtime can capture
+ * the cooperative launch.  No counterpart in the reference, which steps one Cloth per
+ * PhysicsCloth (src/TrainHelper.cpp:15-97) and loops over cloths in its training
+ * drivers (src/TrainHeter.h:81-146).  Each cloth's arithmetic is yc_run's with the
+ * persistent CG, bit for bit.
+ * ------------------------------------------------------------------------------- */
+typedef struct yc_batch yc_batch;
+typedef struct {
+    int graph;        /* 1: replay each step from a hipGraph (direct launches if capture is unsupported) */
+    int group_order;  /* 0: cloths laid out in the given order; 1: by decreasing size */
+    int node_block;   /* threads per block of the per-node and per-segment kernels: 64, 128 or 256 */
+} yc_batch_opts;
+/* ctxs: n distinct contexts on one device; they stay owned by the caller and must
+ * outlive the batch.  Fails if one cloth's solve alone cannot be resident at once. */
+int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out);
+void yc_batch_destroy(yc_batch* b);
+int yc_batch_set_opts(yc_batch* b, const yc_batch_opts* opts);
+int yc_batch_get_opts(const yc_batch* b, yc_batch_opts* opts);
+/* `steps` SimSteps of every cloth (zero CG guess, tol/maxit as yc_step, maxit < 0 for
+ * each cloth's 2n, the reference solver's default; each cloth's current external force).  iters[i] (may be NULL): CG iterations of cloth i summed
+ * over the steps; status[i] (may be NULL): its worst status.  Returns the worst status
+ * over all cloths; yc_last_error names the cloth. */
+int yc_batch_run(yc_batch* b, double h, int steps, double tol, int maxit, long long* iters, int* status);
+/* Layout facts: cooperative groups, whether the last run replayed a graph, tiles of all
+ * cloths, tiles resident at once. */
+int yc_batch_info(const yc_batch* b, int* n_groups, int* graph_used, int* total_tiles, int* capacity);
+
 #ifdef __cplusplus
 }
 #endif
