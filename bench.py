@@ -37,6 +37,46 @@ def cpu_baseline(cs, name, nsteps):
             "sample": "steps 2..%d of the %s workload, oracle single-thread FP64 CG to 1e-8" % (nsteps + 1, name)}
 
 
+def batch_measure(ys, steps, warmup=2):
+    """64 cloths of uneven size (sides 10..64, two-yarn blends) stepped by shared
+    launches with the autotuned options, against the same cloths stepped one after
+    another on their own (persistent CG, queued steps)."""
+    cs = ys.cloth_spec
+    items = cs.batch_workload()
+    h, tol = items[0][0]["h"], 1e-8
+    cloths = []
+    for spec, state in items:
+        c = ys.Cloth(spec)
+        c.set_cg_variant("persistent")
+        c.set_ClothState(state)
+        cloths.append(c)
+    b = ys.ClothBatch(cloths)
+    best, _ = ys.autotune_batch(b, h, steps=2, tol=tol)
+    b.run(h, warmup, tol=tol)
+    t0 = time.perf_counter()
+    iters, status, rc = b.run(h, steps, tol=tol)
+    dt = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    for c, (spec, state) in zip(cloths, items):
+        c.set_ClothState(state)
+        c.run(h, warmup, tol=tol)
+    t0 = time.perf_counter()
+    for c in cloths:
+        c.run(h, steps, tol=tol)
+    dt_seq = time.perf_counter() - t0
+    for c in cloths:
+        c.close()
+    nodes = sum(s["width"] * s["length"] for s, _ in items)
+    return {"workload": "64 cloths, sides 10..64 (seed 3), two-yarn blends in plain/twill/satin, h=0.001, "
+                        "Jacobi-PCG to 1e-08, zero guess", "steps": steps,
+            "cloth_steps_per_s": len(items) * steps / dt, "node_steps_per_s": nodes * steps / dt,
+            "ms_per_step": 1e3 * dt / steps, "sequential_ms_per_step": 1e3 * dt_seq / steps,
+            "speedup_over_sequential": dt_seq / dt, "cg_iters_per_cloth_step": float(iters.sum()) / (len(items) * steps),
+            "status": int(rc), "options": best, "solve_groups": info["groups"], "graph_replay": info["graph_used"],
+            "tiles": info["tiles"], "resident_tiles": info["capacity"]}
+
+
 def measured_traffic(kernel, workload, iterations=None):
     """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE +
     WRITE_SIZE runs, scaled by the copy calibration), or None when there is none.  A
@@ -62,6 +102,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="time only the zero-guess CG path (profiling runs)")
+    ap.add_argument("--batch-steps", type=int, default=20, help="steps of the 64-cloth batch line (0: skip it)")
     ap.add_argument("--cg", default="persistent", choices=["persistent", "fused", "reference"],
                     help="CG form of the timed runs (the reference form is also timed once, zero guess)")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -182,6 +223,8 @@ def main():
         out["roofline"] = spmv
     else:
         out["config"]["spmv"] = spmv
+    if args.batch_steps > 0 and not args.plain_only:
+        out["config"]["batch"] = batch_measure(ys, args.batch_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cs, args.workload, args.cpu_steps)
     print(json.dumps(out), flush=True)

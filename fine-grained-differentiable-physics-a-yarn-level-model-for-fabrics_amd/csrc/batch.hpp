@@ -5,8 +5,9 @@
 // all of them: a block looks up its cloth in a block -> cloth map and runs the same body
 // as the single-cloth kernel on that cloth's buffers, so every cloth's arithmetic is the
 // single-cloth path's, bit for bit.  A step is
-//   k_b_epoch, k_assemble1 (per cloth), k_b_shear, k_b_assemble2, k_b_cg_init,
-//   k_b_cgp (one cooperative launch per group of cloths that fit on the device at once),
+//   k_b_epoch, k_b_assemble1, k_b_shear, k_b_assemble2, k_b_cg_init,
+//   k_b_cgp (one cooperative launch; its blocks walk groups of cloths that fit on the
+//   device at once, block b taking tile b of each group in turn),
 //   k_b_update, k_b_normals
 // with no host synchronisation; a run replays it from a hipGraph where the runtime
 // captures the cooperative launch, and launches it directly otherwise.  The reference
@@ -61,6 +62,17 @@ __global__ void __launch_bounds__(64) k_b_status(const BatchItem* __restrict__ i
     out[BSTAT * i + 5] = *b.cg.err;
 }
 
+// the first assembly kernel's body (assemble1_body.inc) on the block's cloth, from a
+// local copy of its descriptors as the single-cloth kernel takes them
+__global__ void __launch_bounds__(NBLK, 1) k_b_assemble1(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
+    typedef double T;
+    const BatchItem& bi = it[map[blockIdx.x]];
+    const ClothDev C = bi.C;
+    const AsmOut O = bi.O;
+    const int tile = blockIdx.x - bi.t0;
+#include "assemble1_body.inc"
+}
+
 __global__ void __launch_bounds__(NBLK, 1) k_b_assemble2(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
     const BatchItem& b = it[map[blockIdx.x]];
     assemble2_tile<double>(b.C, b.O, h, blockIdx.x - b.t0);
@@ -77,12 +89,24 @@ __global__ void __launch_bounds__(BLK) k_b_cg_init(const BatchItem* __restrict__
     cg_init_block(b.va, b.part, 0, blockIdx.x - b.v0);
 }
 
-// One group of cloths: blocks [off, off + gridDim.x) of the tile launch.
+// The solves of all cloths in one cooperative launch.  Group g holds tiles
+// [grp[2g], grp[2g] + grp[2g+1]) of the tile numbering; block b solves tile b of group 0,
+// then tile b of group 1, and so on.  A cloth's tiles are all in one group, so its
+// blocks are distinct and resident; a block that reaches a cloth early waits (bounded)
+// for peers still finishing an earlier group's cloth, which needs no block of a later
+// group, so every wait ends.  Blocks freed by a cloth that converged early go on to the
+// next group at once.
 template <int KP>
-__global__ void __launch_bounds__(BLK) k_b_cgp(const BatchItem* __restrict__ it, const int* __restrict__ map, int off) {
-    const int blk = blockIdx.x + off;
-    const BatchItem& b = it[map[blk]];
-    cgp_tile<KP>(b.cg, blk - b.t0, *b.epoch);
+__global__ void __launch_bounds__(BLK, 2) k_b_cgp(const BatchItem* __restrict__ it, const int* __restrict__ map,
+                                               const int* __restrict__ grp, int ngroups) {
+    for (int g = 0; g < ngroups; ++g) {
+        if ((int)blockIdx.x < grp[2 * g + 1]) {
+            const int blk = grp[2 * g] + blockIdx.x;
+            const BatchItem& b = it[map[blk]];
+            cgp_tile<KP>(b.cg, blk - b.t0, *b.epoch);
+        }
+        __syncthreads();   // the tile's LDS is reused by the next group's tile
+    }
 }
 
 template <int NB>
@@ -116,11 +140,12 @@ struct yc_batch {
     yc_batch_opts opts;
     int kp = 1, capacity = 0;
     int n_tile = 0, n_seg = 0, n_vec = 0, n_node = 0;
-    std::vector<std::pair<int, int>> groups;   // (first tile block, blocks) per cooperative launch
+    std::vector<std::pair<int, int>> groups;   // (first tile, tiles) of each group of the solve launch
+    int max_group = 0;
     std::vector<BatchItem> items;              // host copy, layout order
     BatchItem* d_items = nullptr;
-    int* d_maps = nullptr;                     // tile, segment, vector, node maps back to back
-    int *map_tile = nullptr, *map_seg = nullptr, *map_vec = nullptr, *map_node = nullptr;
+    int* d_maps = nullptr;                     // tile, segment, vector, node maps and the groups, back to back
+    int *map_tile = nullptr, *map_seg = nullptr, *map_vec = nullptr, *map_node = nullptr, *d_groups = nullptr;
     long long* d_status = nullptr;
     long long* h_status = nullptr;             // pinned
     hipGraphExec_t graph = nullptr;
@@ -184,6 +209,8 @@ int batch_layout(yc_batch* b) {
         mn.insert(mn.end(), (c->N + nb - 1) / nb, li);
     }
     if (gsize) b->groups.push_back({gstart, gsize});
+    b->max_group = 0;
+    for (const auto& g : b->groups) b->max_group = std::max(b->max_group, g.second);
     b->n_tile = (int)mt.size(); b->n_seg = (int)ms.size(); b->n_vec = (int)mv.size(); b->n_node = (int)mn.size();
     std::vector<int> maps;
     maps.reserve(mt.size() + ms.size() + mv.size() + mn.size());
@@ -191,6 +218,7 @@ int batch_layout(yc_batch* b) {
     maps.insert(maps.end(), ms.begin(), ms.end());
     maps.insert(maps.end(), mv.begin(), mv.end());
     maps.insert(maps.end(), mn.begin(), mn.end());
+    for (const auto& g : b->groups) { maps.push_back(g.first); maps.push_back(g.second); }
     HIPCHK(hipMalloc((void**)&b->d_items, sizeof(BatchItem) * n));
     HIPCHK(hipMalloc((void**)&b->d_maps, sizeof(int) * maps.size()));
     HIPCHK(hipMemcpy(b->d_maps, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice));
@@ -198,6 +226,7 @@ int batch_layout(yc_batch* b) {
     b->map_seg = b->map_tile + mt.size();
     b->map_vec = b->map_seg + ms.size();
     b->map_node = b->map_vec + mv.size();
+    b->d_groups = b->map_node + mn.size();
     b->laid_out = true;
     return YC_OK;
 }
@@ -220,7 +249,7 @@ int batch_upload_items(yc_batch* b, double tol, int maxit) {
         a.sc = c->sc; a.err = c->cg_err;
         a.epoch = 0;   // taken from the device counter
         a.tol = tol; a.maxit = maxit < 0 ? 2 * c->n : maxit; a.nbb = c->part.nV;
-        a.spin_limit = 1 << 18;
+        a.spin_limit = 1 << 22;   // a block may also wait for peers still solving an earlier group's cloth
         a.W = c->W; a.Lg = c->Lg; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx; a.nT = c->nT;
         it.st = make_state(c);
         it.epoch = c->bepoch;
@@ -258,20 +287,19 @@ int batch_step(yc_batch* b, double h) {
     const int n = (int)b->m.size();
     const BatchItem* it = b->d_items;
     hipLaunchKernelGGL(k_b_epoch, dim3((n + 63) / 64), dim3(64), 0, b->stream, it, n);
-    for (const BatchItem& bi : b->items)   // the first assembly kernel runs per cloth (see k_assemble1)
-        hipLaunchKernelGGL(k_assemble1<double>, dim3(bi.C.ntx * ((bi.C.Lg + TILE - 1) / TILE)), dim3(NBLK), ASM_LDS, b->stream,
-                           bi.C, bi.O, h);
+    hipLaunchKernelGGL(k_b_assemble1, dim3(b->n_tile), dim3(NBLK), ASM_LDS, b->stream, it, (const int*)b->map_tile, h);
     HIPCHK(hipGetLastError());
     int rc = batch_node_phase(b, h, 0);
     if (rc) return rc;
     hipLaunchKernelGGL(k_b_assemble2, dim3(b->n_tile), dim3(NBLK), ASM_LDS, b->stream, it, (const int*)b->map_tile, h);
     hipLaunchKernelGGL(k_b_cg_init, dim3(b->n_vec), dim3(BLK), 0, b->stream, it, (const int*)b->map_vec);
     HIPCHK(hipGetLastError());
-    for (const auto& g : b->groups) {
+    {
         const int* mp = b->map_tile;
-        int off = g.first;
-        void* args[] = {(void*)&it, (void*)&mp, (void*)&off};
-        HIPCHK(hipLaunchCooperativeKernel(b_cgp_fn(b->kp), dim3(g.second), dim3(BLK), args, 0, b->stream));
+        const int* gp = b->d_groups;
+        int ng = (int)b->groups.size();
+        void* args[] = {(void*)&it, (void*)&mp, (void*)&gp, (void*)&ng};
+        HIPCHK(hipLaunchCooperativeKernel(b_cgp_fn(b->kp), dim3(b->max_group), dim3(BLK), args, 0, b->stream));
     }
     return batch_node_phase(b, h, 1);
 }
@@ -327,7 +355,7 @@ int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out) {
     auto bail = [&](int rc) { yc_batch_destroy(b); (void)hipSetDevice(cur); return rc; };
     if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(b->device) != hipSuccess) return bail(fail(YC_ERR_HIP, "batch: device"));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(YC_ERR_HIP, "batch: stream"));
-    if (hipFuncSetAttribute((const void*)k_assemble1<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess ||
+    if (hipFuncSetAttribute((const void*)k_b_assemble1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess ||
         hipFuncSetAttribute((const void*)k_b_assemble2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess)
         return bail(fail(YC_ERR_HIP, "batch: assembly LDS size not supported"));
     if (hipMalloc((void**)&b->d_status, sizeof(long long) * BSTAT * n) != hipSuccess ||

@@ -226,102 +226,14 @@ __device__ __forceinline__ int tile_node(const ClothDev& C, int tile) {
 // a kernel (by reference or by value), this compiler produced wrong rows for the nodes
 // of tiles where some lanes skip an edge (boundary nodes: one edge's self block landed
 // two nodes away); the batched step therefore launches this kernel once per cloth.
+// The kernel body is shared with the batched kernel by textual inclusion: built as an
+// inlined device function called from a kernel (its descriptors by reference or by
+// value), this compiler produced wrong rows for boundary nodes (one edge's self block
+// landed two nodes away), while the body written inside the kernel is correct.
 template <class T>
 __global__ void __launch_bounds__(NBLK, 1) k_assemble1(ClothDev C, AsmOut O, double h) {
-    typedef v3<T> V;
-    constexpr bool tan = !std::is_same<T, double>::value;
-    int k = tile_node(C, blockIdx.x);
-    if (k < 0) return;
-    RowSinkT<T> S;
-    S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
-    int u = S.mu_, v = S.mv_;
-    // value run: the node's 159 slots accumulate in LDS (the tile's slot planes, same
-    // layout as in HBM) and go to HBM once at the end, instead of a global
-    // read-modify-write per contribution
-    extern __shared__ double lds_rows[];
-    double* gA = S.A;
-    if (!tan) S.A = lds_rows + threadIdx.x;
-    if (S.A) for (int s = 0; s < NSLOT; ++s) S.A[(size_t)s * TILE_N] = 0.0;
-    if (S.dA) for (int s = 0; s < NSLOT; ++s) S.dA[(size_t)s * TILE_N] = 0.0;
-    // handle (src/PhysicsCloth.cpp:232-248)
-    if (C.flags[k] & 2) {
-        V xf = plane3<T>(C.xfix, C.dxfix, C.N, k);
-        T hk = par<T>(C.handle_k, C.dhandle_k);
-        S.FL(k, (-hk) * (pos<T>(C, k) - xf));
-        S.LL(KIND_KX, k, k, I33T<T>(-hk));
-    }
-    // edges (list order: warp rows, then weft columns; src/Cloth.cpp:190-228).  One call
-    // site in a loop that is not unrolled: the inlined element code is large, and four or
-    // six copies of it would not fit the instruction cache.
-    int ne_warp = C.Lg * (C.W - 1);
-#pragma unroll 1
-    for (int ei = 0; ei < 4; ++ei) {
-        int eu, ev, ec, idx;
-        if (ei == 0) { if (u == 0) continue; eu = u - 1; ev = v; ec = 0; idx = v * (C.W - 1) + u - 1; }
-        else if (ei == 1) { if (u == C.W - 1) continue; eu = u; ev = v; ec = 0; idx = v * (C.W - 1) + u; }
-        else if (ei == 2) { if (v == 0) continue; eu = u; ev = v - 1; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v - 1; }
-        else { if (v == C.Lg - 1) continue; eu = u; ev = v; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v; }
-        edge_element<T>(C, S, eu, ev, ec, ec == 0 ? (v & 1) : (u & 1), omp_owner(C.n_edges, idx));
-    }
-    // bends centred on u-1, u, u+1 (warp) and v-1, v, v+1 (weft) (src/Cloth.cpp:230-270):
-    // a bend centred d after the node holds it as n2 (role 0), centred on it as n0
-    // (role 1), centred d before it as n1 (role 2)
-    int nb_warp = C.Lg * (C.W - 2);
-#pragma unroll 1
-    for (int bi6 = 0; bi6 < 6; ++bi6) {
-        int d = bi6 % 3 - 1, dir = bi6 / 3;
-        int uc = dir == 0 ? u + d : u, vc = dir == 0 ? v : v + d;
-        if (dir == 0 && (uc < 1 || uc > C.W - 2)) continue;
-        if (dir == 1 && (vc < 1 || vc > C.Lg - 2)) continue;
-        int bi = dir == 0 ? v * (C.W - 2) + uc - 1 : nb_warp + u * (C.Lg - 2) + vc - 1;
-        if constexpr (USE_BEND_REC && !tan) emit_bend_role(C, S, O.bend_rec, C.n_bends, bi, 1 - d, omp_owner(C.n_bends, bi));
-        else bend_element<T>(C, S, uc, vc, dir, dir == 0 ? (v & 1) : (u & 1), omp_owner(C.n_bends, bi));
-    }
-    // faces of the up to four cells around the node (src/Cloth.cpp:272-286)
-    if (C.wind_density != 0.0 || C.wind_drag != 0.0) {
-#pragma unroll 1
-        for (int fi = 0; fi < 8; ++fi) {
-            int cu = u - 1 + ((fi >> 2) & 1), cv = v - 1 + ((fi >> 1) & 1), q = fi & 1;
-            if (cu < 0 || cv < 0 || cu > C.W - 2 || cv > C.Lg - 2) continue;
-            int tl = cu + cv * C.W, tr = tl + 1, bl = tl + C.W, br = bl + 1;
-            int fa, fb, fc;
-            if (((cu + cv) & 1) == 0) { if (q == 0) { fa = tl; fb = tr; fc = br; } else { fa = tl; fb = br; fc = bl; } }
-            else { if (q == 0) { fa = tl; fb = tr; fc = bl; } else { fa = tr; fb = br; fc = bl; } }
-            if (fa == k || fb == k || fc == k) face_element<T>(C, S, fa, fb, fc);
-        }
-    }
-    // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
-    T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
-    if (S.inter) comp = friction<T>(C, S, sl);
-    // b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:20, 63)
-    int pk = padidx(O.Wp, u, v);
-    int e = S.inter ? crossing(C, u, v) : 0;
-    if (!tan) {
-        for (int s = 0; s < NSLOT; ++s) gA[(size_t)s * TILE_N] = S.A[(size_t)s * TILE_N];
-        for (int c = 0; c < 5; ++c) O.F[c * C.N + k] = val(S.F[c]);
-        O.comp[k] = val(comp);
-        O.is_slide[k] = val(sl[0]); O.is_slide[C.N + k] = val(sl[1]);
-        O.fepart[k] = val(S.FEpart[0]); O.fepart[C.N + k] = val(S.FEpart[1]);
-        for (int c = 0; c < 5; ++c) {
-            double bv = 0.0;
-            if (c < 3 || S.inter) {
-                double ext = 0.0;
-                if (O.ext) ext = c < 3 ? O.ext[3 * k + c] : O.ext[3 * C.N + 2 * e + (c - 3)];
-                bv = h * (val(S.F[c]) + ext - val(S.KvTv[c])) + val(S.Mv[c]);
-            }
-            O.b[c * O.P + pk] = bv;
-        }
-    } else {
-        for (int c = 0; c < 5; ++c) O.dF[c * C.N + k] = tan_of(S.F[c]);
-        O.dcomp[k] = tan_of(comp);
-        O.dis_slide[k] = tan_of(sl[0]); O.dis_slide[C.N + k] = tan_of(sl[1]);
-        for (int c = 0; c < 5; ++c) {
-            double bv = 0.0;   // f_ext is a constant input
-            if (c < 3 || S.inter) bv = h * (tan_of(S.F[c]) - tan_of(S.KvTv[c])) + tan_of(S.Mv[c]);
-            O.db[c * O.P + pk] = bv;
-        }
-    }
-    if (S.bad) atomicOr(&O.flag[0], 1);
+    const int tile = blockIdx.x;
+#include "assemble1_body.inc"
 }
 
 // Shear segments touching the node (4 own, 2 each as warp neighbour of the left and
