@@ -13,3 +13,4 @@ from ._abi import (YcError, NotConverged, NonFinite, YC_OK, YC_ERR_NONFINITE,  #
 from .cloth import Cloth, PhysicsCloth, SimStep  # noqa: F401
 from . import grad  # noqa: F401
 from .batch import ClothBatch, autotune_batch, autotune_cloth  # noqa: F401
+from . import learn  # noqa: F401
