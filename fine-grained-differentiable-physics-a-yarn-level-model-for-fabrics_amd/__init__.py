@@ -14,3 +14,4 @@ from .cloth import Cloth, PhysicsCloth, SimStep  # noqa: F401
 from . import grad  # noqa: F401
 from .batch import ClothBatch, autotune_batch, autotune_cloth  # noqa: F401
 from . import learn  # noqa: F401
+from . import h5io  # noqa: F401
