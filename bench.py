@@ -77,6 +77,40 @@ def batch_measure(ys, steps, warmup=2):
             "tiles": info["tiles"], "resident_tiles": info["capacity"]}
 
 
+def large_measure(ys, steps):
+    """The 256 x 256 homogeneous plain cloth with a seeded +z pressure (about 327k DOF):
+    plain-CG steps/s (the fused form: the persistent one needs every tile resident),
+    and the assembly and the stencil product against the HBM peak."""
+    cs = ys.cloth_spec
+    spec, state, ext, _ = cs.workload("homo256")
+    h, tol = spec["h"], 1e-8
+    c = ys.Cloth(spec)
+    c.set_cg_variant("fused")
+    c.set_ClothState(state)
+    c.set_ext_force(ext)
+    c.run(h, 2, tol=tol)
+    t0 = time.perf_counter()
+    iters, rc = c.run(h, steps, tol=tol)
+    dt = time.perf_counter() - t0
+    c.assemble(h)
+    asm_ms = c.time_assemble(h, 20)
+    spmv_ms = c.time_spmv(200)
+    spmv_bytes = c.spmv_bytes()
+    W, n = spec["width"], c.n
+    nnz = (spmv_bytes - 16 * n) / 8
+    asm_bytes = 8 * nnz + 16 * n + 8 * 16 * W * W
+    c.close()
+    return {"workload": "homo256: 256x256 crossings, plain weave, cotton, seeded +z pressure, h=0.001, "
+                        "Jacobi-PCG (fused form) to 1e-08", "steps": steps, "steps_per_s": steps / dt,
+            "ms_per_step": 1e3 * dt / steps, "cg_iters_per_step": iters / steps, "status": int(rc),
+            "spmv": {"bound": "hbm", "achieved": spmv_bytes / (spmv_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms},
+            "assembly": {"bound": "hbm", "achieved": asm_bytes / (asm_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": asm_bytes / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "bytes_per_assembly": asm_bytes, "avg_ms": asm_ms}}
+
+
 def measured_traffic(kernel, workload, iterations=None):
     """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE +
     WRITE_SIZE runs, scaled by the copy calibration), or None when there is none.  A
@@ -103,6 +137,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="time only the zero-guess CG path (profiling runs)")
     ap.add_argument("--batch-steps", type=int, default=20, help="steps of the 64-cloth batch line (0: skip it)")
+    ap.add_argument("--large-steps", type=int, default=10, help="steps of the 256x256 line (0: skip it)")
     ap.add_argument("--cg", default="persistent", choices=["persistent", "fused", "reference"],
                     help="CG form of the timed runs (the reference form is also timed once, zero guess)")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -225,6 +260,8 @@ def main():
         out["config"]["spmv"] = spmv
     if args.batch_steps > 0 and not args.plain_only:
         out["config"]["batch"] = batch_measure(ys, args.batch_steps)
+    if args.large_steps > 0 and not args.plain_only:
+        out["config"]["homo256"] = large_measure(ys, args.large_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cs, args.workload, args.cpu_steps)
     print(json.dumps(out), flush=True)

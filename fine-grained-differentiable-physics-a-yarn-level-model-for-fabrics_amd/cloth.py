@@ -244,6 +244,12 @@ class Cloth:
                                                      ctypes.byref(it)))
         return ms.value, it.value
 
+    def time_assemble(self, h, reps=20):
+        """Mean duration (ms) of one assembly of A and b (HIP events, no status read-back)."""
+        ms = ctypes.c_double()
+        _abi.check(_abi.load_library().yc_time_assemble(self._h, float(h), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def cg_iteration_bytes(self):
         return _abi.load_library().yc_cg_iteration_bytes(self._h)
 

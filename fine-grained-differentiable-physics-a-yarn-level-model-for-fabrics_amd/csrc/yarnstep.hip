@@ -2303,6 +2303,18 @@ int yc_time_spmv(yc_ctx* c, int reps, double* avg_ms) {
     return st;
 }
 
+int yc_time_assemble(yc_ctx* c, double h, int reps, double* avg_ms) {
+    if (!c || reps <= 0 || !avg_ms) return fail(YC_ERR_ARG, "bad argument");
+    int rc = YC_OK;
+    int st = time_launches(c->stream, reps, avg_ms, [&] {
+        if (!rc) rc = do_assemble(c, h, true);
+    });
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return st;
+}
+
 int yc_time_solve(yc_ctx* c, double tol, int maxit, int reps, double* avg_ms, int* iters) {
     if (!c || reps <= 0 || !avg_ms) return fail(YC_ERR_ARG, "bad argument");
     if (!c->assembled) return fail(YC_ERR_ARG, "yc_time_solve before yc_assemble");

@@ -151,6 +151,10 @@ int yc_time_solve(yc_ctx* ctx, double tol, int maxit, int reps, double* avg_ms, 
 /* Algorithmic bytes of one Jacobi-PCG iteration streamed from memory: 8 per structural
  * nonzero of A plus 120 per DOF (15 vector passes). */
 double yc_cg_iteration_bytes(const yc_ctx* ctx);
+/* Times `reps` assemblies of A and b (the kernels of yc_assemble, without its status
+ * read-back) on the context's stream with one HIP event pair; avg_ms the mean per
+ * assembly.  Measurement hook with no counterpart in the reference. */
+int yc_time_assemble(yc_ctx* ctx, double h, int reps, double* avg_ms);
 
 /* ---------------------------------------------------------------------------
  * Learned CG warm start.  No counterpart in the reference (whose only network,

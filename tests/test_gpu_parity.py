@@ -108,3 +108,29 @@ def test_trajectory_100_steps(ys, cs):
             worst = max(worst, e)
             say("  step %d rel %.2e" % (s + 1, e))
     assert worst < TOL_TRAJ
+
+
+def test_large_cloth_first_step(ys, cs):
+    """The 256 x 256 cloth with its seeded pressure (327k DOF): A and b against the oracle
+    within 1e-10 (sparse Frobenius norms), then a fused-form solve whose true residual
+    |b - A x| / |b|, computed with the oracle's A, meets the tolerance."""
+    spec, state, ext, _ = cs.workload("homo256")
+    g, r = _pair(ys, spec, state)
+    h = spec["h"]
+    g.set_cg_variant("fused")
+    g.set_ext_force(ext)
+    g.assemble(h)
+    r.assemble(h, ext)
+    Ar = r.csr(0).tocsr()
+    D = (g.csr() - Ar).tocsr()
+    eA = np.sqrt((D.data ** 2).sum()) / np.sqrt((Ar.data ** 2).sum())
+    br = r.vec("b")
+    eb = rel(g.vec("b"), br)
+    say("  256x256 |dA|=%.2e |db|=%.2e nnz %d" % (eA, eb, Ar.nnz))
+    assert eA < TOL_STEP and eb < TOL_STEP
+    it, rr, ok = g.solve(tol=1e-8)
+    x = g.vec("x")
+    res = np.linalg.norm(br - Ar @ x) / np.linalg.norm(br)
+    say("  fused CG iters %d recursive relres %.2e true relres %.2e" % (it, rr, res))
+    assert ok and rr < 1e-8 and res < 1.5e-8
+    g.close()
