@@ -837,10 +837,49 @@ __device__ bool cgp_wait(const CgpArgs& a, const CgSlot* sl, unsigned long long 
     return !bad;
 }
 
-// Publishes this block's partials and tag after all of its stores have completed.
-__device__ __forceinline__ void cgp_publish(const CgpArgs& a, CgSlot* sl, const double* acc, unsigned long long tag, int tile) {
+// Block sums of six values that every thread receives: wavefront sums (DPP) into
+// sh[m][wave], one barrier, then every thread adds the four wavefront sums in the same
+// order (the order thread 0 of block_sum_n uses, so the sums are the same bits).  sh
+// must not be read by a slower thread from an earlier call: the caller separates two
+// uses of one sh by at least one other barrier.
+__device__ __forceinline__ void block_allsum6(double (&v)[6], double (*sh)[BLK / 64]) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) v[m] = wave_sum_dpp(v[m]);
+    if ((threadIdx.x & 63) == 63)
+#pragma unroll
+        for (int m = 0; m < 6; ++m) sh[m][threadIdx.x >> 6] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < BLK / 64; ++i) t += sh[m][i];
+        v[m] = t;
+    }
+}
+
+// Block-sums this block's six partials and has thread 0 store them into the block's
+// slot (the tag follows in cgp_tag, once every store of the stage has completed).
+__device__ __forceinline__ void cgp_partials(CgSlot* sl, double (&acc)[6], double (*sh)[BLK / 64], int tile) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) acc[m] = wave_sum_dpp(acc[m]);
+    if ((threadIdx.x & 63) == 63)
+#pragma unroll
+        for (int m = 0; m < 6; ++m) sh[m][threadIdx.x >> 6] = acc[m];
+    __syncthreads();
     if (threadIdx.x == 0)
-        for (int m = 0; m < 6; ++m) st_dev(&sl[tile].v[m], acc[m]);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < BLK / 64; ++i) t += sh[m][i];
+            st_dev(&sl[tile].v[m], t);
+        }
+}
+
+// Publishes the stage: every thread waits for its own stores (partials and vectors), one
+// barrier, then thread 0 stores the tag in the block's slot.
+__device__ __forceinline__ void cgp_tag(CgSlot* sl, unsigned long long tag, int tile) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -855,6 +894,7 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
     __shared__ double su[5][HALO_N];
     __shared__ double sp[5][TILE_N];
     __shared__ double sred[8];
+    __shared__ double shA[6][BLK / 64], shB[6][BLK / 64];   // the two block sums of a stage
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int tu0 = (tile % a.ntx) * TILE, tv0 = (tile / a.ntx) * TILE;
     const bool lead = tile == 0 && threadIdx.x == 0;
@@ -955,11 +995,8 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                 if (threadIdx.x + k * BLK < a.nT)
 #pragma unroll
                     for (int m = 0; m < 6; ++m) acc[m] += pv[k][m];
-            block_sum_n(acc);
-            if (threadIdx.x == 0)
-                for (int m = 0; m < 6; ++m) sred[m] = acc[m];
-            __syncthreads();
-            const double gam = sred[0], d1 = sred[1], d2 = sred[2], d3 = sred[3], d4 = sred[4], rr = sred[5];
+            block_allsum6(acc, shA);
+            const double gam = acc[0], d1 = acc[1], d2 = acc[2], d3 = acc[3], d4 = acc[4], rr = acc[5];
             const bool finite = isfinite(rr);
             if (rr < thr || j >= a.maxit || !finite) {
                 if (lead) {
@@ -1028,7 +1065,8 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                 acc[3] += y0 * sp[3][lane] + y1 * sp[4][lane];
             }
         }
-        block_sum_n(acc);
+        CgSlot* slot_out = a.slot[init ? 0 : 1 - q];
+        cgp_partials(slot_out, acc, shB, tile);
         // outputs: w_{j+1} (all), r_{j+1} and s_j (iterations only)
         double* w_out = a.w[init ? 0 : 1 - q];
         if (node_in) {
@@ -1049,7 +1087,7 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                     st_dev(r_out + e_i[t], o_r[t]);
                 }
         }
-        cgp_publish(a, a.slot[init ? 0 : 1 - q], acc, (epoch << 32) | (unsigned long long)(j + 1), tile);
+        cgp_tag(slot_out, (epoch << 32) | (unsigned long long)(j + 1), tile);
     }
     // the solution of the owned entries
 #pragma unroll
