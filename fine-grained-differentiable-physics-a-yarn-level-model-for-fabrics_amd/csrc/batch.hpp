@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(64) k_b_status(const BatchItem* __restrict__ i
 
 // the first assembly kernel's body (assemble1_body.inc) on the block's cloth, from a
 // local copy of its descriptors as the single-cloth kernel takes them
-__global__ void __launch_bounds__(NBLK, 1) k_b_assemble1(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
+__global__ void __launch_bounds__(ASM1_THREADS, 1) k_b_assemble1(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
     typedef double T;
     const BatchItem& bi = it[map[blockIdx.x]];
     const ClothDev C = bi.C;
@@ -287,7 +287,7 @@ int batch_step(yc_batch* b, double h) {
     const int n = (int)b->m.size();
     const BatchItem* it = b->d_items;
     hipLaunchKernelGGL(k_b_epoch, dim3((n + 63) / 64), dim3(64), 0, b->stream, it, n);
-    hipLaunchKernelGGL(k_b_assemble1, dim3(b->n_tile), dim3(NBLK), ASM_LDS, b->stream, it, (const int*)b->map_tile, h);
+    hipLaunchKernelGGL(k_b_assemble1, dim3(b->n_tile), dim3(ASM1_THREADS), ASM1_LDS, b->stream, it, (const int*)b->map_tile, h);
     HIPCHK(hipGetLastError());
     int rc = batch_node_phase(b, h, 0);
     if (rc) return rc;
@@ -355,7 +355,7 @@ int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out) {
     auto bail = [&](int rc) { yc_batch_destroy(b); (void)hipSetDevice(cur); return rc; };
     if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(b->device) != hipSuccess) return bail(fail(YC_ERR_HIP, "batch: device"));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(YC_ERR_HIP, "batch: stream"));
-    if (hipFuncSetAttribute((const void*)k_b_assemble1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess ||
+    if (hipFuncSetAttribute((const void*)k_b_assemble1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM1_LDS) != hipSuccess ||
         hipFuncSetAttribute((const void*)k_b_assemble2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess)
         return bail(fail(YC_ERR_HIP, "batch: assembly LDS size not supported"));
     if (hipMalloc((void**)&b->d_status, sizeof(long long) * BSTAT * n) != hipSuccess ||

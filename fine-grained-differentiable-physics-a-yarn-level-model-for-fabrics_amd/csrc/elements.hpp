@@ -170,6 +170,7 @@ struct RowSinkT {
     int me, mu_, mv_;
     bool inter;
     double* A;          // slot 0 of node "me"; slots TILE_N apart
+    double* Aself;      // if set: where the L-L self block (9 planes) goes instead of A
     double* dA;         // tangent run: slot 0 of node "me" in the tangent matrix, or null
     bool detach;
     double coef[3];     // +1, -h^2, -h
@@ -183,7 +184,7 @@ struct RowSinkT {
     YS_DEV void init(const ClothDev* c, int k, double* a, double h, double* da = nullptr, bool det = false) {
         C = c; me = k; mu_ = k % c->W; mv_ = k / c->W; inter = interior(*c, mu_, mv_);
         size_t base = a_base(c->ntx, mu_, mv_);
-        A = a ? a + base : nullptr; dA = da ? da + base : nullptr; detach = det;
+        A = a ? a + base : nullptr; dA = da ? da + base : nullptr; detach = det; Aself = nullptr;
         coef[0] = 1.0; coef[1] = -(h * h); coef[2] = -h;
         for (int i = 0; i < 5; ++i) { F[i] = T(0.0); Mv[i] = T(0.0); KvTv[i] = T(0.0); }
         KxEE[0] = KxEE[1] = KvEE[0] = KvEE[1] = FEpart[0] = FEpart[1] = T(0.0);
@@ -199,6 +200,10 @@ struct RowSinkT {
     YS_DEV void addA(int rc, int kcol, int cc, T x) {
         int s = slot(rc, kcol, cc);
         if (s < 0) return;
+        if (Aself && rc < 3 && cc < 3 && s == rc * 45 + cc) {   // the L-L self block
+            Aself[(size_t)(rc * 3 + cc) * TILE_N] += val(x);
+            return;
+        }
         if (A) A[(size_t)s * TILE_N] += val(x);
         if (dA) dA[(size_t)s * TILE_N] += tan_of(x);
     }
@@ -260,6 +265,24 @@ struct RowSinkT {
             }
         }
         if (kind == KIND_KV && kD == me) add5(KvTv, 3 + c, blk(v) * velE(kF, c));
+    }
+    // the accumulators as NACC planes `stride` apart (value run): store, and add in
+    static constexpr int NACC = 31;
+    YS_DEV void store_acc(double* p, int stride) const {
+        int i = 0;
+        for (int c = 0; c < 5; ++c) { p[(size_t)(i++) * stride] = val(F[c]); p[(size_t)(i++) * stride] = val(Mv[c]); p[(size_t)(i++) * stride] = val(KvTv[c]); }
+        for (int c = 0; c < 2; ++c) { p[(size_t)(i++) * stride] = val(KxEE[c]); p[(size_t)(i++) * stride] = val(KvEE[c]); p[(size_t)(i++) * stride] = val(FEpart[c]); }
+        p[(size_t)(i++) * stride] = val(SBC_L.x); p[(size_t)(i++) * stride] = val(SBC_L.y); p[(size_t)(i++) * stride] = val(SBC_L.z);
+        for (int c = 0; c < 2; ++c) { p[(size_t)(i++) * stride] = val(SBC_LE[c].x); p[(size_t)(i++) * stride] = val(SBC_LE[c].y); p[(size_t)(i++) * stride] = val(SBC_LE[c].z); }
+        p[(size_t)i * stride] = (double)bad;
+    }
+    YS_DEV void add_acc(const double* p, int stride) {
+        int i = 0;
+        for (int c = 0; c < 5; ++c) { F[c] += p[(size_t)(i++) * stride]; Mv[c] += p[(size_t)(i++) * stride]; KvTv[c] += p[(size_t)(i++) * stride]; }
+        for (int c = 0; c < 2; ++c) { KxEE[c] += p[(size_t)(i++) * stride]; KvEE[c] += p[(size_t)(i++) * stride]; FEpart[c] += p[(size_t)(i++) * stride]; }
+        SBC_L.x += p[(size_t)(i++) * stride]; SBC_L.y += p[(size_t)(i++) * stride]; SBC_L.z += p[(size_t)(i++) * stride];
+        for (int c = 0; c < 2; ++c) { SBC_LE[c].x += p[(size_t)(i++) * stride]; SBC_LE[c].y += p[(size_t)(i++) * stride]; SBC_LE[c].z += p[(size_t)(i++) * stride]; }
+        if (p[(size_t)i * stride] != 0.0) bad = 1;
     }
     YS_DEV void FL(int k, v3<T> f) { if (k == me) { F[0] += f.x; F[1] += f.y; F[2] += f.z; } }
     // E force on the crossing of node k; elem_owner is the static owner thread of the

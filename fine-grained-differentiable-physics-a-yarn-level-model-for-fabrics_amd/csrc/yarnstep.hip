@@ -210,14 +210,19 @@ constexpr bool USE_BEND_REC = false;
 // LDS of the value assembly kernels: one tile's slot planes (81,408 bytes, above the
 // 64 KiB default, allowed per kernel at context creation)
 constexpr size_t ASM_LDS = (size_t)NSLOT * TILE_N * sizeof(double);
+// the first assembly kernel's value run: the slot planes, then the second half's L-L self
+// block (9 planes) and accumulators (RowSinkT::NACC planes) -- 101,888 bytes
+constexpr size_t ASM1_LDS = (size_t)(NSLOT + 9 + RowSinkT<double>::NACC) * TILE_N * sizeof(double);
+constexpr int ASM1_THREADS = 2 * TILE_N;
 
 // Node of thread t in the assembly kernels: one block per 8 x 8 matrix tile, so that a
 // wavefront's read-modify-writes of one slot are 512 contiguous bytes.
-__device__ __forceinline__ int tile_node(const ClothDev& C, int tile) {
-    int u = (tile % C.ntx) * TILE + (threadIdx.x & (TILE - 1));
-    int v = (tile / C.ntx) * TILE + (threadIdx.x / TILE);
+__device__ __forceinline__ int tile_node(const ClothDev& C, int tile, int lane) {
+    int u = (tile % C.ntx) * TILE + (lane & (TILE - 1));
+    int v = (tile / C.ntx) * TILE + (lane / TILE);
     return (u < C.W && v < C.Lg) ? u + v * C.W : -1;
 }
+__device__ __forceinline__ int tile_node(const ClothDev& C, int tile) { return tile_node(C, tile, threadIdx.x); }
 
 // T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
 // src/TrainHelper.cpp:19-20).  T = dd: their directional derivatives along the
@@ -231,7 +236,7 @@ __device__ __forceinline__ int tile_node(const ClothDev& C, int tile) {
 // value), this compiler produced wrong rows for boundary nodes (one edge's self block
 // landed two nodes away), while the body written inside the kernel is correct.
 template <class T>
-__global__ void __launch_bounds__(NBLK, 1) k_assemble1(ClothDev C, AsmOut O, double h) {
+__global__ void __launch_bounds__(ASM1_THREADS, 1) k_assemble1(ClothDev C, AsmOut O, double h) {
     const int tile = blockIdx.x;
 #include "assemble1_body.inc"
 }
@@ -1568,7 +1573,7 @@ int do_assemble(yc_ctx* c, double h, bool async = false) {
     if constexpr (USE_BEND_REC) if (c->cd.n_bends > 0)
         hipLaunchKernelGGL(k_bend_elem, dim3((c->cd.n_bends + 255) / 256), dim3(256), 0, c->stream, c->cd, c->bend_rec,
                            c->cd.n_bends);
-    hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(ASM1_THREADS), ASM1_LDS, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
@@ -1871,7 +1876,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     if (!rc && hipHostMalloc((void**)&c->h_flag, 4 * sizeof(int)) != hipSuccess) rc = fail(YC_ERR_HIP, "pinned alloc");
     if (rc) { yc_destroy(c); return rc; }
     c->coop_ok = cgp_fits(c) ? 1 : 0;
-    if (hipFuncSetAttribute((const void*)k_assemble1<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess ||
+    if (hipFuncSetAttribute((const void*)k_assemble1<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM1_LDS) != hipSuccess ||
         hipFuncSetAttribute((const void*)k_assemble2<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess) {
         yc_destroy(c);
         return fail(YC_ERR_HIP, "assembly LDS size not supported");
