@@ -82,7 +82,7 @@ def initial_state(spec):
     s.update({k: np.zeros((N, 2)) for k in ("EPos", "EPosBar", "EVel")})
     for j in range(Lg):
         for i in range(W):
-            k = i + j * Lg  # get/set_ClothState index u + v*length
+            k = i + j * W  # state index u + v*width (= the reference's u + v*length when square)
             if spec["pose"] == "upright":
                 s["LPos"][k] = (i * L, 0.0, hgt - j * L)
             else:
@@ -113,7 +113,7 @@ def pressure_force(spec, state, p):
     x = np.zeros((N, 3))
     for v in range(Lg):
         for u in range(W):
-            x[u + v * W] = state["LPos"][u + v * Lg]
+            x[u + v * W] = state["LPos"][u + v * W]
     f = np.zeros(n)
     for i in range(W - 1):
         for j in range(Lg - 1):

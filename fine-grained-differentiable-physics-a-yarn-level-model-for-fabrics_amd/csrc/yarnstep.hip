@@ -1946,7 +1946,10 @@ int yc_set_state(yc_ctx* c, const double* lpos, const double* lposfix, const dou
     std::vector<double> a(3 * N), b2(3 * N), cv(3 * N), d(2 * N), e(2 * N), f(2 * N);
     for (int v = 0; v < c->Lg; ++v)
         for (int u = 0; u < c->W; ++u) {
-            int k = u + v * c->W, s = u + v * c->Lg;   // state index u + v*length (src/Cloth.cpp:426)
+            // state index u + v*length as the reference (src/Cloth.cpp:426), which addresses the
+            // nodes one to one only when width == length; u + v*width equals it there and is
+            // the trajectory loader's order (src/Loader.cpp:160) for non-square cloths
+            int k = u + v * c->W, s = k;
             for (int q = 0; q < 3; ++q) { a[q * N + k] = lpos[3 * s + q]; b2[q * N + k] = lposfix[3 * s + q]; cv[q * N + k] = lvel[3 * s + q]; }
             for (int q = 0; q < 2; ++q) { d[q * N + k] = epos[2 * s + q]; e[q * N + k] = eposbar[2 * s + q]; f[q * N + k] = evel[2 * s + q]; }
         }
@@ -1976,7 +1979,7 @@ int yc_get_state(yc_ctx* c, double* lpos, double* lposfix, double* lvel, double*
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int v = 0; v < c->Lg; ++v)
         for (int u = 0; u < c->W; ++u) {
-            int k = u + v * c->W, s = u + v * c->Lg;
+            int k = u + v * c->W, s = k;   // state index: see yc_set_state
             for (int q = 0; q < 3; ++q) {
                 if (lpos) lpos[3 * s + q] = a[q * N + k];
                 if (lposfix) lposfix[3 * s + q] = b2[q * N + k];
@@ -2251,7 +2254,7 @@ int yc_tangent_begin(yc_ctx* c, int ndir, const double* dparam, const double* ds
             const double *epos = src + 9 * N, *eposbar = src + 11 * N, *evel = src + 13 * N;
             for (int v = 0; v < c->Lg; ++v)
                 for (int u = 0; u < c->W; ++u) {
-                    int k = u + v * c->W, st = u + v * c->Lg;
+                    int k = u + v * c->W, st = k;   // state index: see yc_set_state
                     for (int q = 0; q < 3; ++q) {
                         t[0 * N + q * N + k] = lpos[3 * st + q];
                         t[3 * N + q * N + k] = lvel[3 * st + q];
@@ -2281,7 +2284,7 @@ int yc_tangent_get(yc_ctx* c, int dir, double* dlpos, double* dlposfix, double* 
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int v = 0; v < c->Lg; ++v)
         for (int u = 0; u < c->W; ++u) {
-            int k = u + v * c->W, st = u + v * c->Lg;
+            int k = u + v * c->W, st = k;   // state index: see yc_set_state
             for (int q = 0; q < 3; ++q) {
                 if (dlpos) dlpos[3 * st + q] = t[0 * N + q * N + k];
                 if (dlvel) dlvel[3 * st + q] = t[3 * N + q * N + k];

@@ -15,7 +15,9 @@
  * Index conventions are the reference's:
  *   node k = u + v*width, crossing e = (u-1) + (v-1)*(width-2),
  *   global vector [L: 3k+{x,y,z}] ++ [E: 3N + 2e+{u,v}]      (src/MatrixMp.cpp:120-122)
- *   state arrays use index u + v*length as get/set_ClothState do (src/Cloth.cpp:362,426).
+ *   state arrays use index u + v*width: get/set_ClothState's u + v*length
+ *   (src/Cloth.cpp:362,426) for the square cloths it addresses one to one, and the
+ *   trajectory loader's order (src/Loader.cpp:160) for non-square ones.
  */
 #ifndef YARNSTEP_H
 #define YARNSTEP_H

@@ -1190,11 +1190,14 @@ int yo_ndof(const yo_cloth* c) { return c->n; }
 
 int yo_owner(int n_iter, int i) { return owner(n_iter, i); }
 
-// get_ClothState (src/Cloth.cpp:350-371): state index u + v*length (Q9)
+// get_ClothState (src/Cloth.cpp:350-371).  The reference's state index is u + v*length
+// (Q9), which only addresses a square cloth's nodes one to one (it collides or runs past
+// the arrays when width != length); this index, u + v*width, equals it for square cloths
+// and is the trajectory loader's order (src/Loader.cpp:160) for the others.
 void yo_get_state(const yo_cloth* c, double* lpos, double* lposfix, double* lvel,
                   double* epos, double* eposbar, double* evel) {
     for (int v2 = 0; v2 < c->Lg; ++v2) for (int u = 0; u < c->W; ++u) {
-        int k = c->kidx(u, v2), s = u + v2 * c->Lg;
+        int k = c->kidx(u, v2), s = u + v2 * c->W;
         for (int a = 0; a < 3; ++a) {
             if (lpos) lpos[3 * s + a] = c->x[k][a];
             if (lposfix) lposfix[3 * s + a] = c->xfix[k][a];
@@ -1212,7 +1215,7 @@ void yo_get_state(const yo_cloth* c, double* lpos, double* lposfix, double* lvel
 void yo_set_state(yo_cloth* c, const double* lpos, const double* lposfix, const double* lvel,
                   const double* epos, const double* eposbar, const double* evel) {
     for (int v2 = 0; v2 < c->Lg; ++v2) for (int u = 0; u < c->W; ++u) {
-        int k = c->kidx(u, v2), s = u + v2 * c->Lg;
+        int k = c->kidx(u, v2), s = u + v2 * c->W;
         for (int a = 0; a < 3; ++a) {
             c->x[k][a] = lpos[3 * s + a];
             c->xfix[k][a] = lposfix[3 * s + a];

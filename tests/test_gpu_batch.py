@@ -57,6 +57,9 @@ def _small_items(cs):
     spec = cs.homo_spec(9, "twill")
     spec["wind_vel"] = [0.0, 4.0, 0.0]
     items.append((spec, cs.initial_state(spec)))
+    spec = cs.heter_spec(10, "satin", 17)   # a non-square cloth: 10 x 17 crossings
+    spec["length"] = 17
+    items.append((spec, cs.initial_state(spec)))
     return items
 
 

@@ -134,3 +134,35 @@ def test_large_cloth_first_step(ys, cs):
     say("  fused CG iters %d recursive relres %.2e true relres %.2e" % (it, rr, res))
     assert ok and rr < 1e-8 and res < 1.5e-8
     g.close()
+
+
+@pytest.mark.parametrize("W,Lg,pattern", [(12, 7, "twill"), (7, 12, "satin")])
+def test_non_square_cloth(ys, cs, W, Lg, pattern):
+    """Cloths with width != length (the tiles of A are then not square in number):
+    assembly against the oracle at 1e-10 on an excited state, and a 30-step trajectory
+    of the persistent form within 1e-6."""
+    spec = cs.heter_spec(W, pattern, W * 100 + Lg)
+    spec["length"] = Lg
+    spec["handles"] = [(0, 0), (W - 1, 0)]
+    st = excited_state(cs, spec, 7)
+    g, r = _pair(ys, spec, st)
+    h = spec["h"]
+    g.assemble(h)
+    r.assemble(h)
+    eA = np.linalg.norm((g.csr() - r.csr(0)).toarray()) / np.linalg.norm(r.csr(0).toarray())
+    eb = rel(g.vec("b"), r.vec("b"))
+    say("  %dx%d |dA|=%.2e |db|=%.2e" % (W, Lg, eA, eb))
+    assert eA < TOL_STEP and eb < TOL_STEP
+    g.close()
+    st0 = cs.initial_state(spec)
+    g, r = _pair(ys, spec, st0)
+    g.set_cg_variant("persistent")
+    for _ in range(30):
+        _, _, rc = g.step(h, tol=1e-12, maxit=20000)
+        st_, _, _ = r.step(h, None, tol=1e-12, maxit=20000)
+        assert rc == 0 and st_ == 0
+    sg, sr = g.get_ClothState(), r.get_state()
+    e = max(rel(sg[k], sr[k]) for k in ("LPos", "LVel", "EPos", "EVel", "EPosBar"))
+    say("  %dx%d 30-step rel %.2e" % (W, Lg, e))
+    assert e < 1e-6
+    g.close()

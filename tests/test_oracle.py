@@ -395,3 +395,33 @@ def test_update_rule(cs):
     k = 1 + 1 * 5
     ud = x[3 * N:3 * N + 2]
     np.testing.assert_allclose(g["EPosBar"][k], st["EPosBar"][k] + sl[0] * ud * 1e-3, rtol=0, atol=1e-18)
+
+
+@pytest.mark.parametrize("W,Lg", [(7, 5), (5, 7)])
+def test_non_square_rest_state_and_state_order(cs, W, Lg):
+    """Non-square cloths: at the rest pose the only L force is gravity (as for square
+    ones), and set/get state use index u + v*width (the reference's u + v*length for
+    the square cloths it supports) -- a round trip returns every node's own values."""
+    s = _spec(cs, W, wind_density=0.0, wind_drag=0.0)
+    s["length"] = Lg
+    st = cs.initial_state(s)
+    N = W * Lg
+    assert st["LPos"].shape == (N, 3)
+    for v in range(Lg):
+        for u in range(W):
+            assert np.allclose(st["LPos"][u + v * W], (u * s["L"], 0.0, s["hight"] - v * s["L"]))
+    c = _forces(s, st)
+    F = c.vec("F")[: 3 * N].reshape(-1, 3)
+    rho, L, g = s["yarns"][0]["rho"], s["L"], s["G"][2]
+    nedge = np.array([(u > 0) + (u < W - 1) + (v > 0) + (v < Lg - 1) for v in range(Lg) for u in range(W)])
+    np.testing.assert_allclose(F[:, 2], -0.5 * rho * g * L * nedge, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(F[:, :2], 0.0, atol=1e-11)
+    rng = np.random.default_rng(W * 10 + Lg)
+    mark = {k: rng.normal(size=v.shape) for k, v in st.items()}
+    c.set_state(mark)
+    back = c.get_state()
+    for k in ("LPos", "LPosFix", "LVel"):
+        assert np.array_equal(back[k], mark[k])
+    inter = np.array([0 < u < W - 1 and 0 < v < Lg - 1 for v in range(Lg) for u in range(W)])
+    for k in ("EPos", "EPosBar", "EVel"):
+        assert np.array_equal(back[k][inter], mark[k][inter])
