@@ -15,3 +15,4 @@ from . import grad  # noqa: F401
 from .batch import ClothBatch, autotune_batch, autotune_cloth  # noqa: F401
 from . import learn  # noqa: F401
 from . import h5io  # noqa: F401
+from . import render  # noqa: F401
