@@ -56,7 +56,7 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_tangent_begin", "yc_step_tangent", "yc_tangent_get", "yc_tangent_end",
            "yc_time_solve", "yc_cg_iteration_bytes",
            "yc_batch_create", "yc_batch_destroy", "yc_batch_set_opts", "yc_batch_get_opts", "yc_batch_run",
-           "yc_batch_info", "yc_time_assemble"]
+           "yc_batch_info", "yc_time_assemble", "yc_tangent_set_ext"]
 
 
 def lib_path():
@@ -101,6 +101,7 @@ def load_library():
     L.yc_step_tangent.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]
     L.yc_tangent_get.argtypes = [vp, ctypes.c_int] + [dp] * 6
     L.yc_tangent_end.argtypes = [vp]
+    L.yc_tangent_set_ext.argtypes = [vp, dp]
     L.yc_time_solve.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, ip]
     L.yc_time_assemble.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp]
     L.yc_cg_iteration_bytes.argtypes = [vp]

@@ -210,6 +210,15 @@ class Cloth:
         _abi.check(rc)
         return info.cg_iters, info.cg_relres, rc
 
+    def tangent_set_ext(self, dext):
+        """Tangents of the external force, (ndir, n) in the global layout, for the following
+        step_tangent calls; None clears them."""
+        if dext is None:
+            _abi.check(_abi.load_library().yc_tangent_set_ext(self._h, _NULL))
+            return
+        a = _arr(dext, (self.ndir, self.n))
+        _abi.check(_abi.load_library().yc_tangent_set_ext(self._h, _dp(a)))
+
     def tangent_get(self, d):
         N = self.N
         s = {k: np.zeros((N, 3)) for k in ("LPos", "LPosFix", "LVel")}

@@ -468,7 +468,10 @@ int yc_batch_run(yc_batch* b, double h, int steps, double tol, int maxit, long l
                          : st && st[3] ? "assembly: stencil slot out of range"
                          : worst == YC_ERR_NONFINITE ? "NaN/Inf in A, b or a solution"
                          : "a CG solve stopped at maxit above tolerance";
-        if (st && st[5]) (void)hipMemset(b->m[worst_i]->cg_err, 0, sizeof(int));
+        if (st && st[5]) {   // on the batch's stream, ahead of its next launches
+            (void)hipMemsetAsync(b->m[worst_i]->cg_err, 0, sizeof(int), b->stream);
+            (void)hipStreamSynchronize(b->stream);
+        }
         return fail(worst, "batch cloth " + std::to_string(worst_i) + ": " + what);
     }
     return YC_OK;

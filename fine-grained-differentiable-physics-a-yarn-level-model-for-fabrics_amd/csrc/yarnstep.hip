@@ -187,6 +187,7 @@ struct AsmOut {
     double* fepart;     // 2 planes of N
     double* invdiag;    // padded 5-plane vector
     const double* ext;  // global-layout n vector or null
+    const double* dext; // tangent run: the direction's tangent of ext (global layout) or null
     int* flag;          // [0] structural error, [1] non-finite
     int Wp, P;          // padded width, plane stride
     // tangent run: tangents of A (exact mode, else null), F, b, is_slide and comp, and
@@ -1355,6 +1356,7 @@ struct yc_ctx {
     double* tstate;                         // ndir x TAN_PLANES planes of N
     std::vector<double> tpar;               // ndir x YC_NPARAM
     double *dA, *dF, *db, *dis_slide, *dcomp, *At, *xsol, *bsave, *dsol, *trhs;
+    double* tdext;                          // ndir x n tangents of the external force, or null
     double* shear_rec;                      // 3 roles x 30 x (4 Nc) shear records
     double* bend_rec;                       // 3 roles x BEND_REC x n_bends bend records
 };
@@ -1391,6 +1393,7 @@ AsmOut make_out(yc_ctx* c) {
     AsmOut O;
     O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
     O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
+    O.dext = nullptr;
     O.detach = 0;
     O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
     O.bend_rec = c->bend_rec;
@@ -1682,7 +1685,11 @@ int do_update(yc_ctx* c, double h) {
 template <class T>
 int dalloc(T** p, size_t count) {
     HIPCHK(hipMalloc((void**)p, count * sizeof(T) + 64));
+    // hipMemset of device memory returns before the fill is done, on the null stream,
+    // which does not order against the contexts' non-blocking streams: wait for it, or a
+    // following upload on a context stream can land first and be zeroed
     HIPCHK(hipMemset(*p, 0, count * sizeof(T) + 64));
+    HIPCHK(hipStreamSynchronize(nullptr));
     return YC_OK;
 }
 
@@ -1783,6 +1790,7 @@ int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int
     O.dF = c->dF; O.db = c->db; O.dis_slide = c->dis_slide; O.dcomp = c->dcomp;
     O.detach = mode == YC_GRAD_REF;
     O.b = c->bsave;   // the tangent kernels write db only; b is not touched
+    O.dext = c->tdext ? c->tdext + (size_t)d * c->n : nullptr;
     hipLaunchKernelGGL(k_assemble1<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
     hipLaunchKernelGGL(k_assemble2<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
     HIPCHK(hipGetLastError());
@@ -1921,7 +1929,7 @@ void yc_destroy(yc_ctx* c) {
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
                     c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
-                    c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs,
+                    c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs, c->tdext,
                     c->shear_rec, c->bend_rec, c->bepoch};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -2225,7 +2233,8 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
 int yc_tangent_end(yc_ctx* c) {
     if (!c) return fail(YC_ERR_ARG, "null context");
     (void)hipStreamSynchronize(c->stream);
-    double** bufs[] = {&c->tstate, &c->dA, &c->dF, &c->db, &c->dis_slide, &c->dcomp, &c->At, &c->xsol, &c->bsave, &c->dsol, &c->trhs};
+    double** bufs[] = {&c->tstate, &c->dA, &c->dF, &c->db, &c->dis_slide, &c->dcomp, &c->At, &c->xsol, &c->bsave, &c->dsol, &c->trhs,
+                       &c->tdext};
     for (double** p : bufs) { if (*p) (void)hipFree(*p); *p = nullptr; }
     c->ndir = 0;
     c->tpar.clear();
@@ -2271,6 +2280,23 @@ int yc_tangent_begin(yc_ctx* c, int ndir, const double* dparam, const double* ds
             HIPCHK(hipStreamSynchronize(c->stream));
         }
     }
+    return YC_OK;
+}
+
+int yc_tangent_set_ext(yc_ctx* c, const double* dext) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (c->ndir <= 0) return fail(YC_ERR_ARG, "yc_tangent_set_ext before yc_tangent_begin");
+    if (!dext) {
+        if (c->tdext) { HIPCHK(hipStreamSynchronize(c->stream)); (void)hipFree(c->tdext); c->tdext = nullptr; }
+        return YC_OK;
+    }
+    const size_t cnt = (size_t)c->ndir * c->n;
+    if (!c->tdext) {
+        int rc = dalloc(&c->tdext, cnt);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c->tdext, dext, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return YC_OK;
 }
 

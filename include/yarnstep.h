@@ -237,6 +237,12 @@ int yc_tangent_begin(yc_ctx* ctx, int ndir, const double* dparam, const double* 
 /* SimStep (as yc_step, zero CG guess) and the tangents of all directions through it;
  * the tangent solves use tol and maxit too. */
 int yc_step_tangent(yc_ctx* ctx, double h, int mode, double tol, int maxit, yc_step_info* info);
+/* Tangents of the external force for the following yc_step_tangent calls: dext holds
+ * ndir x n doubles in the global layout (the yc_set_ext_force layout), NULL clears them.
+ * The force enters b = h (F + f_ext - K_v^T v) + M v, so its tangent adds h dext to the
+ * tangent of b in both gradient modes: the path of the reference's control parameters
+ * (src/YarnSimNet.h:87-246, SimStep with ext_force, src/TrainHelper.cpp:56-97). */
+int yc_tangent_set_ext(yc_ctx* ctx, const double* dext);
 /* State tangent of direction dir, in the yc_get_state layout; any pointer may be NULL. */
 int yc_tangent_get(yc_ctx* ctx, int dir, double* dlpos, double* dlposfix, double* dlvel,
                    double* depos, double* deposbar, double* devel);

@@ -158,7 +158,7 @@ def _glob_vel(spec, st):
     for v in range(Lg):
         for u in range(W):
             k = u + v * W
-            s = u + v * Lg
+            s = k   # state index u + v*width
             out[3 * k:3 * k + 3] = st["LVel"][s]
             if 0 < u < W - 1 and 0 < v < Lg - 1:
                 e = (u - 1) + (v - 1) * (W - 2)
@@ -321,3 +321,42 @@ def test_autograd_rollout_gradient_matches_fd(ys, cs):
     an = float(g @ dirn)
     say("  loss %.4e  directional derivative: autograd %.8e  oracle fd %.8e  rel %.2e" % (float(loss), an, fd, abs(an - fd) / abs(fd)))
     assert abs(an - fd) <= 1e-5 * abs(fd)
+
+
+@pytest.mark.parametrize("steps", [1, 3])
+def test_ext_force_tangent_matches_fd(ys, cs, steps):
+    """The tangent of the trajectory along a direction of the external force (the path of
+    the reference's control parameters, which enter SimStep through ext_force) against
+    central finite differences of oracle rollouts under the perturbed force."""
+    spec = _spec(cs)
+    h = spec["h"]
+    state = natural_state(cs, spec)
+    g = ys.Cloth(spec)
+    n = g.n
+    f0 = cs.pressure_force(spec, state, 1.0)
+    dext = np.random.default_rng(9).normal(0.0, 3e-7, n)
+    g.set_ClothState(state)
+    g.set_ext_force(f0)
+    g.tangent_begin(np.zeros((1, len(g.PARAMS))))
+    g.tangent_set_ext(dext[None, :])
+    for _ in range(steps):
+        g.step_tangent(h, "exact", tol=1e-14, maxit=20000)
+    tan = flat(g.tangent_get(0))
+    g.tangent_end()
+    g.close()
+
+    def roll(e):
+        r = OracleCloth(spec)
+        r.set_state(state)
+        for _ in range(steps):
+            st, _, _ = r.step(h, f0 + e * dext, tol=1e-14, maxit=20000)
+            assert st == 0
+        return flat(r.get_state())
+
+    def cd(e):
+        return (roll(e) - roll(-e)) / (2 * e)
+    fd = (4.0 * cd(0.25) - cd(0.5)) / 3.0
+    err = np.linalg.norm(tan - fd)
+    say("  %d steps, d/dext |fd| %.3e |tan| %.3e |tan-fd| %.2e rel %.2e" % (steps, np.linalg.norm(fd), np.linalg.norm(tan), err,
+                                                                       err / np.linalg.norm(fd)))
+    assert err <= (1e-6 if steps == 1 else 2e-5) * np.linalg.norm(fd)
