@@ -16,3 +16,4 @@ from .batch import ClothBatch, autotune_batch, autotune_cloth  # noqa: F401
 from . import learn  # noqa: F401
 from . import h5io  # noqa: F401
 from . import render  # noqa: F401
+from . import control  # noqa: F401

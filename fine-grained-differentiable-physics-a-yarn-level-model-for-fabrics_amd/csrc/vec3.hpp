@@ -47,7 +47,12 @@ YS_DEV bool operator!=(dd a, double b) { return a.v != b; }
 YS_DEV dd sqrt(dd a) { double s = ::sqrt(a.v); return dd(s, a.d / (2.0 * s)); }
 YS_DEV dd sin(dd a) { return dd(::sin(a.v), ::cos(a.v) * a.d); }
 YS_DEV dd cos(dd a) { return dd(::cos(a.v), -::sin(a.v) * a.d); }
-YS_DEV dd acos(dd a) { return dd(::acos(a.v), -a.d / ::sqrt(1.0 - a.v * a.v)); }
+// At |a| = 1 (a straight yarn: the bend angle is 0) acos has no derivative; the angle's
+// central directional derivative there is 0, which keeps the tangents finite
+YS_DEV dd acos(dd a) {
+    double s = ::sqrt(1.0 - a.v * a.v);
+    return dd(::acos(a.v), s > 0.0 ? -a.d / s : 0.0);
+}
 YS_DEV dd tanh(dd a) { double t = ::tanh(a.v); return dd(t, (1.0 - t * t) * a.d); }
 YS_DEV dd exp(dd a) { double e = ::exp(a.v); return dd(e, e * a.d); }
 YS_DEV dd fabs(dd a) { return a.v < 0.0 ? -a : a; }

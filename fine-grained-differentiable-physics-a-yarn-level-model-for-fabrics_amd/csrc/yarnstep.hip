@@ -1763,6 +1763,8 @@ int solve_on(yc_ctx* c, const double* Aptr, double tol, int maxit, int* iters) {
         if (iters) *iters = std::max(c->h_sc->itc[0], c->h_sc->itc[1]);
     }
     if (c->h_sc->zero_rhs) HIPCHK(hipMemsetAsync(c->cx, 0, (size_t)5 * c->P * sizeof(double), c->stream));
+    if (!std::isfinite(c->h_sc->bb) || !std::isfinite(c->h_sc->rr))
+        return fail(YC_ERR_NONFINITE, "tangent: NaN/Inf in the right-hand side or the solution");
     if (c->h_sc->rr >= c->h_sc->thr && !c->h_sc->zero_rhs) return fail(YC_ERR_NOT_CONVERGED, "tangent CG stopped at maxit above tolerance");
     return YC_OK;
 }

@@ -360,3 +360,42 @@ def test_ext_force_tangent_matches_fd(ys, cs, steps):
     say("  %d steps, d/dext |fd| %.3e |tan| %.3e |tan-fd| %.2e rel %.2e" % (steps, np.linalg.norm(fd), np.linalg.norm(tan), err,
                                                                        err / np.linalg.norm(fd)))
     assert err <= (1e-6 if steps == 1 else 2e-5) * np.linalg.norm(fd)
+
+
+@pytest.mark.parametrize("mode", ["exact", "ref"])
+def test_tangent_from_rest_pose_is_finite(ys, cs, mode):
+    """From the constructor's rest pose every yarn is straight (bend angle 0, where acos
+    has no derivative).  The tangents stay finite there, and over two steps the exact
+    ones match central differences of the oracle.  The step is not smooth at straight
+    yarns (the bend Jacobians are zero at angle 0 and tend to a finite limit, computed
+    with 1/sin cancellations, as the angle goes to 0), so the difference quotients
+    themselves scatter: between steps of 1e-5 and 2e-5 of each parameter they move by
+    up to 6e-5 of their size, and the check allows 3e-4."""
+    spec = cs.homo_spec(7)
+    h = spec["h"]
+    state = cs.initial_state(spec)
+    g = ys.Cloth(spec)
+    names = ("rho0", "Y0", "B0", "S")
+    ks = [g.PARAMS.index(n) for n in names]
+    scale = np.array([abs(param_value(spec, k)) for k in ks])
+    dp = np.zeros((len(ks), len(g.PARAMS)))
+    for i, k in enumerate(ks):
+        dp[i, k] = scale[i]
+    g.set_ClothState(state)
+    g.tangent_begin(dp)
+    for _ in range(2):
+        g.step_tangent(h, mode, tol=1e-14, maxit=20000)
+    tans = [flat(g.tangent_get(i)) for i in range(len(ks))]
+    g.tangent_end()
+    g.close()
+    for i, k in enumerate(ks):
+        assert np.isfinite(tans[i]).all() and np.linalg.norm(tans[i]) > 0, names[i]
+        if mode != "exact":
+            continue
+        e = 1e-5
+        fd = (oracle_rollout(spec_with(spec, k, e * scale[i]), state, 2, h)
+              - oracle_rollout(spec_with(spec, k, -e * scale[i]), state, 2, h)) / (2 * e)
+        err = np.linalg.norm(tans[i] - fd)
+        say("  rest pose, 2 steps, d/d%-4s |fd| %.3e |tan-fd| %.2e rel %.2e" % (names[i], np.linalg.norm(fd), err,
+                                                                          err / np.linalg.norm(fd)))
+        assert err <= 3e-4 * np.linalg.norm(fd), names[i]
