@@ -2,9 +2,11 @@
 
 Workload: the 128x128 two-yarn twill cloth with seeded wind (cloth_spec.workload
 "twill128"), h = 1 ms, FP64, the reference's step (assemble, solve, update) per step.
-Two runs from the same initial state: plain zero-guess Jacobi-PCG, and CG started
-from the learned warm start; both stop at the same 1e-8 relative residual.  `value`
-is the warm-started steps/s; the plain rate and the ratio are in `config`.  Inputs
+Two runs from the same initial state: plain zero-guess PCG, and PCG started from the
+learned warm start; both stop at the same 1e-8 relative residual (persistent form,
+2x2-block-Jacobi preconditioner by default).  `value` is the warm-started steps/s; the
+plain rate, the ratio and the Jacobi-preconditioned rates (the reference solver's
+preconditioner, persistent and reference forms) are in `config`.  Inputs
 live in HBM before the timed region.  Prints one JSON line.
 """
 import argparse
@@ -140,6 +142,9 @@ def main():
     ap.add_argument("--large-steps", type=int, default=10, help="steps of the 256x256 line (0: skip it)")
     ap.add_argument("--cg", default="persistent", choices=["persistent", "fused", "reference"],
                     help="CG form of the timed runs (the reference form is also timed once, zero guess)")
+    ap.add_argument("--precond", default="block2", choices=["block2", "jacobi"],
+                    help="preconditioner of the persistent-CG runs (the Jacobi one, the reference solver's, is "
+                         "also timed once, zero guess)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run 5 launches of a 256 MiB -> 256 MiB FP64 negation, the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counter runs")
@@ -170,13 +175,16 @@ def main():
         X, T, M = ws.generate(ws.training_cloths(6), steps=40, skip=5)
         net, _ = ws.fit(X, T, M, epochs=16)
 
-    def timed(warm, variant=None):
+    def timed(warm, variant=None, precond=None):
         cloth = ys.Cloth(spec)
         try:
             cloth.set_cg_variant(variant or args.cg)
         except ys.YcError:   # the persistent form needs every tile resident at once
             args.cg = "fused"
+            args.precond = "jacobi"
             cloth.set_cg_variant(args.cg)
+        if (variant or args.cg) == "persistent":
+            cloth.set_preconditioner(precond or args.precond)
         cloth.set_ClothState(state)
         cloth.set_ext_force(ext)
         run = (lambda n: cloth.run_warm(net, h, n, tol=tol)) if warm else (lambda n: cloth.run(h, n, tol=tol))
@@ -193,6 +201,11 @@ def main():
         c_ref, dt_ref, it_ref = timed(False, "reference")
         c_ref.close()
         ref_form = {"steps_per_s": args.steps / dt_ref, "cg_iters_per_step": it_ref / args.steps}
+    jacobi = None
+    if args.cg == "persistent" and args.precond != "jacobi" and not args.plain_only:
+        c_j, dt_j, it_j = timed(False, "persistent", "jacobi")
+        c_j.close()
+        jacobi = {"steps_per_s": args.steps / dt_j, "cg_iters_per_step": it_j / args.steps}
     cloth, dt_plain, it_plain = timed(False)
     # the stencil SpMV (the reference-form CG's product), timed with HIP events on the
     # context stream
@@ -202,15 +215,18 @@ def main():
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     # the dominant kernel: one persistent-CG solve per step (k_cgp), timed with an event
     # pair around each cooperative launch; its algorithmic bytes are the solve's
-    # iterations times one Jacobi-PCG iteration streamed from memory
+    # iterations times one PCG iteration streamed from memory
     dominant = None
     if args.cg == "persistent":
         cloth.assemble(h)
         solve_ms, solve_iters = cloth.time_solve(tol=tol, reps=10)
         solve_bytes = solve_iters * cloth.cg_iteration_bytes()
+        bj = args.precond == "block2"
         dominant = {"bound": "hbm", "achieved": solve_bytes / (solve_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "traffic": measured_traffic("k_cgp", args.workload, solve_iters),
-                    "kernel": "k_cgp (one persistent Jacobi-PCG solve)", "bytes_per_launch": solve_bytes,
+                    "unit": "GB/s",
+                    "traffic": measured_traffic("k_cgp_bj" if bj else "k_cgp", args.workload, solve_iters),
+                    "kernel": "k_cgp (one persistent %s-PCG solve)" % ("2x2-block-Jacobi" if bj else "Jacobi"),
+                    "bytes_per_launch": solve_bytes,
                     "avg_launch_ms": solve_ms, "iterations_per_launch": solve_iters}
         dominant["frac"] = dominant["achieved"] / HBM_PEAK_GBS
     cloth.close()
@@ -238,14 +254,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "%s: %dx%d crossings, %s weave, %d yarns, seeded wind, h=%g, Jacobi-PCG to %g relative residual" % (
+        "config": {"workload": "%s: %dx%d crossings, %s weave, %d yarns, seeded wind, h=%g, PCG to %g relative residual" % (
             args.workload, spec["width"], spec["length"], spec["pattern"], len(spec["yarns"]), h, tol),
-            "solver": "learned warm start (32-128-128-128-5 BF16 MLP) + Jacobi-PCG (%s form)" % args.cg,
+            "solver": "learned warm start (32-128-128-128-5 BF16 MLP) + %s-PCG (%s form)" % (
+                "2x2-block-Jacobi" if args.precond == "block2" else "Jacobi", args.cg),
             "cg_iters_per_step": it_warm / args.steps,
             "plain_cg_steps_per_s": plain_steps_per_s, "plain_cg_ms_per_step": 1e3 * dt_plain / args.steps,
             "plain_cg_iters_per_step": it_plain / args.steps,
             "warm_over_plain": steps_per_s / plain_steps_per_s,
             "plain_cg_reference_form": ref_form,
+            "plain_cg_jacobi_persistent": jacobi,
             "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": mlp_tflops / BF16_PEAK_TFLOPS, "avg_launch_ms": mlp_ms,
                     "rows": spec["width"] * spec["length"]} if net else None},

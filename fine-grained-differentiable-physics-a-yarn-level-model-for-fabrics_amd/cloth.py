@@ -238,6 +238,13 @@ class Cloth:
     def set_cg_variant(self, name):
         _abi.check(_abi.load_library().yc_set_cg_variant(self._h, self.CG_VARIANTS[name]))
 
+    # preconditioners: "jacobi" (the reference solver's diagonal one) or "block2" (exact
+    # inverses of A on 2 x 2 groups of nodes; persistent CG only)
+    PRECONDITIONERS = {"jacobi": 0, "block2": 1}
+
+    def set_preconditioner(self, name):
+        _abi.check(_abi.load_library().yc_set_preconditioner(self._h, self.PRECONDITIONERS[name]))
+
     def time_spmv(self, reps=200):
         ms = ctypes.c_double()
         _abi.check(_abi.load_library().yc_time_spmv(self._h, int(reps), ctypes.byref(ms)))

@@ -243,6 +243,7 @@ int batch_upload_items(yc_batch* b, double tol, int maxit) {
         it.part = c->part;
         CgpArgs& a = it.cg;
         a.A = c->A; a.invd = c->invd; a.bbpart = c->part.bb; a.rrpart = c->part.rr;
+        a.brow = nullptr;   // the batch solves with the Jacobi preconditioner
         a.x = c->cx;
         a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1; a.s[0] = c->cs0; a.s[1] = c->cs1;
         a.slot[0] = c->cgslot; a.slot[1] = c->cgslot + c->nT;
@@ -342,6 +343,7 @@ int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out) {
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i]) return fail(YC_ERR_ARG, "batch: null context");
         if (ctxs[i]->device != ctxs[0]->device) return fail(YC_ERR_ARG, "batch: contexts on different devices");
+        if (ctxs[i]->precond != YC_PRECOND_JACOBI) return fail(YC_ERR_ARG, "batch: the batch solves with the Jacobi preconditioner");
         for (int j = 0; j < i; ++j)
             if (ctxs[j] == ctxs[i]) return fail(YC_ERR_ARG, "batch: a context appears twice");
     }

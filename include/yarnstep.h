@@ -131,6 +131,19 @@ int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_s
 #define YC_CG_PERSISTENT 2
 int yc_set_cg_variant(yc_ctx* ctx, int variant);
 
+/* Preconditioner of the CG solves.
+ * YC_PRECOND_JACOBI (default): the reference solver's diagonal preconditioner
+ *   (src/Solvers.cpp:3-31, Eigen's DiagonalPreconditioner).
+ * YC_PRECOND_BLOCK2: block Jacobi over 2 x 2 groups of nodes (the exact inverse of A on
+ *   each group's up to 20 DOFs, recomputed from A before every solve).  No counterpart
+ *   in the reference: the same stopping rule on the unpreconditioned |r|^2, so the
+ *   solution meets the same tolerance, in about half the iterations.  Persistent CG
+ *   only: a solve with another variant, or on a cloth whose persistent CG does not fit,
+ *   fails with YC_ERR_ARG. */
+#define YC_PRECOND_JACOBI 0
+#define YC_PRECOND_BLOCK2 1
+int yc_set_preconditioner(yc_ctx* ctx, int kind);
+
 /* Runs `steps` SimSteps back to back on the device with zero-guess CG; returns the
  * total CG iterations and the worst status.  With YC_CG_PERSISTENT the steps are queued
  * without any host synchronisation (assembly flags and solve outcomes accumulate on the
@@ -151,7 +164,8 @@ double yc_spmv_bytes(const yc_ctx* ctx);
  * the iteration count of the solve.  Overwrites the CG workspace and the solution. */
 int yc_time_solve(yc_ctx* ctx, double tol, int maxit, int reps, double* avg_ms, int* iters);
 /* Algorithmic bytes of one Jacobi-PCG iteration streamed from memory: 8 per structural
- * nonzero of A plus 120 per DOF (15 vector passes). */
+ * nonzero of A plus 120 per DOF (15 vector passes); with YC_PRECOND_BLOCK2 the 20
+ * coefficients of each DOF's preconditioner row replace its diagonal entry (272 per DOF). */
 double yc_cg_iteration_bytes(const yc_ctx* ctx);
 /* Times `reps` assemblies of A and b (the kernels of yc_assemble, without its status
  * read-back) on the context's stream with one HIP event pair; avg_ms the mean per
