@@ -945,20 +945,37 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
         x_reg[t] = a.x[e_own[t] ? e_i[t] : 0];
         p_reg[t] = 0.0;
     }
-    // block-Jacobi: the 20 coefficients of each entry's row over its group (node q =
-    // (du) + 2 (dv) from the group's origin, component c: coefficient q * 5 + c) and the
-    // group origin in halo coordinates
-    double brow[CG1_NE][BJ ? 20 : 1];
-    int g_ln0[CG1_NE];
+    // block-Jacobi apply, by group: the halo's 6 x 6 groups of 20 DOFs, 7 threads per
+    // group (3 rows each, 2 for the last), so that a thread reads its group's 20 inputs
+    // once.  brow[o]: the 20 coefficients of the thread's o-th row (over node q = du + 2 dv
+    // from the group's origin and component c: coefficient q * 5 + c), zero for nodes
+    // outside the grid; bj_c / bj_ln: where its result goes in su.
+    constexpr int BJ_T = 7, BJ_GROUPS = (HALO_W / 2) * (HALO_W / 2);
+    static_assert(BJ_T * BJ_GROUPS <= BLK && 3 * BJ_T >= 20 && HALO_W % 2 == 0, "block-Jacobi thread map");
+    double brow[3][BJ ? 20 : 1];
+    int bj_c[3], bj_ln[3], bj_ln0 = 0;
+    bool bj_ok[3];
     double u_reg[CG1_NE];   // u_j = M^-1 r_j of the owned entries, from the previous stage
 #pragma unroll
-    for (int t = 0; t < CG1_NE; ++t) {
-        int lx = e_ln[t] % HALO_W, ly = e_ln[t] / HALO_W;
-        g_ln0[t] = (lx & ~1) + (ly & ~1) * HALO_W;
-        u_reg[t] = 0.0;
-        if constexpr (BJ) {
+    for (int t = 0; t < CG1_NE; ++t) u_reg[t] = 0.0;
+    {
+        const int gi = threadIdx.x / BJ_T, jt = threadIdx.x - BJ_T * gi;
+        const int gx0 = 2 * (gi % (HALO_W / 2)), gy0 = 2 * (gi / (HALO_W / 2));
+        bj_ln0 = gi < BJ_GROUPS ? gx0 + gy0 * HALO_W : 0;   // spare threads read group 0
 #pragma unroll
-            for (int m = 0; m < 20; ++m) brow[t][m] = a.brow[(size_t)e_i[t] * 20 + m];
+        for (int o = 0; o < 3; ++o) {
+            const int oi = 3 * jt + o, qn = oi / 5, c = oi - 5 * qn;
+            const int lx = gx0 + (qn & 1), ly = gy0 + (qn >> 1);
+            const int u = tu0 - 2 + lx, v = tv0 - 2 + ly;
+            bj_ok[o] = gi < BJ_GROUPS && oi < 20;
+            bj_c[o] = c;
+            bj_ln[o] = lx + ly * HALO_W;
+            const bool in = bj_ok[o] && u >= 0 && u < a.W && v >= 0 && v < a.Lg;
+            if constexpr (BJ) {
+                const double* src = a.brow + (size_t)(in ? c * a.P + padidx(a.Wp, u, v) : 0) * 20;
+#pragma unroll
+                for (int m = 0; m < 20; ++m) brow[o][m] = in ? src[m] : 0.0;
+            }
         }
     }
     // stopping threshold from the b.b and r_0.r_0 partials of k_cg_init (every block
@@ -1052,9 +1069,11 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
         }
         double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // r.u, u.w, u.s, p.w, p.s, r.r
         double o_s[CG1_NE], o_r[CG1_NE];
-        double b_un[CG1_NE];
+        double k_rn[CG1_NE], k_sj[CG1_NE];
         if constexpr (BJ) {
-            // r_{j+1} on tile + halo into LDS, then u_{j+1} = M^-1 r_{j+1} per entry
+            // r_{j+1} on tile + halo into LDS, then u_{j+1} = M^-1 r_{j+1} into su by group
+            // (every su entry of tile + halo is written: 36 groups x 20 rows); the owned
+            // entries' dot products with u follow the barrier after the main pass
 #pragma unroll
             for (int t = 0; t < CG1_NE; ++t)
                 if (threadIdx.x + t * BLK < 5 * HALO_N) {
@@ -1062,15 +1081,18 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                     sr[e_c[t]][e_ln[t]] = e_r[t] - alpha * sj;
                 }
             __syncthreads();
+            double in[20];
 #pragma unroll
-            for (int t = 0; t < CG1_NE; ++t) {
+            for (int qn = 0; qn < 4; ++qn)
+#pragma unroll
+                for (int cc = 0; cc < 5; ++cc)
+                    in[qn * 5 + cc] = sr[cc][bj_ln0 + (qn & 1) + (qn >> 1) * HALO_W];
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
                 double uu = 0.0;
 #pragma unroll
-                for (int qn = 0; qn < 4; ++qn)
-#pragma unroll
-                    for (int cc = 0; cc < 5; ++cc)
-                        uu += brow[t][qn * 5 + cc] * sr[cc][g_ln0[t] + (qn & 1) + (qn >> 1) * HALO_W];
-                b_un[t] = e_in[t] ? uu : 0.0;
+                for (int m = 0; m < 20; ++m) uu += brow[o][m] * in[m];
+                if (bj_ok[o]) su[bj_c[o]][bj_ln[o]] = uu;
             }
         }
 #pragma unroll
@@ -1079,8 +1101,10 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
             if (threadIdx.x + t * BLK >= 5 * HALO_N) continue;
             double sj = e_w[t] + beta * e_s[t];
             double rn = e_r[t] - alpha * sj;
-            double un = BJ ? b_un[t] : e_id[t] * rn;
+            double un = BJ ? 0.0 : e_id[t] * rn;
             double pj = 0.0;
+            k_rn[t] = rn;
+            k_sj[t] = sj;
             if (e_in[t] && e_own[t]) {
                 if (!init) {
                     pj = (BJ ? u_reg[t] : e_id[t] * e_r[t]) + beta * p_reg[t];
@@ -1089,19 +1113,30 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                     o_s[t] = sj;
                     o_r[t] = rn;
                 }
-                acc[0] += rn * un;
-                acc[2] += un * sj;
+                if (!BJ) {
+                    acc[0] += rn * un;
+                    acc[2] += un * sj;
+                }
                 acc[4] += pj * sj;
                 acc[5] += rn * rn;
-                if (BJ) u_reg[t] = un;
             }
-            su[e_c[t]][e_ln[t]] = un;
+            if (!BJ) su[e_c[t]][e_ln[t]] = un;
             if (e_own[t]) {
                 int lx = e_ln[t] % HALO_W, ly = e_ln[t] / HALO_W;
                 sp[e_c[t]][(lx - 2) + (ly - 2) * TILE] = pj;
             }
         }
         __syncthreads();
+        if constexpr (BJ) {
+#pragma unroll
+            for (int t = 0; t < CG1_NE; ++t)
+                if (threadIdx.x + t * BLK < 5 * HALO_N && e_in[t] && e_own[t]) {
+                    double un = su[e_c[t]][e_ln[t]];
+                    acc[0] += k_rn[t] * un;
+                    acc[2] += un * k_sj[t];
+                    u_reg[t] = un;
+                }
+        }
         double y0 = 0.0, y1 = 0.0;
         if (node_in) {
             auto xin = [&](int c, int du, int dv) { return su[c][l0 + du + dv * HALO_W]; };
