@@ -1186,57 +1186,65 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) { cgp_tile<KP, BJ>(a, bl
 // 2x2-node block-Jacobi preconditioner: for the group of nodes (2gu + i, 2gv + j), i, j in
 // {0, 1}, the inverse of A restricted to the group's DOFs (up to 20: 3 L and, at interior
 // crossings, 2 E per node), by Gauss-Jordan without pivoting (a principal block of the
-// step's symmetric positive definite system), one 64-thread block per group.  Row
-// (component c, node) goes to brow[(c P + padded node) 20 + q' 5 + c'], zero for DOFs that
-// do not exist.
+// step's symmetric positive definite system).  One wavefront inverts three groups: lane
+// 20 g + j holds column j of group g in registers; at pivot k lane k's column goes through
+// LDS to the group's other lanes.  Row (component c, node) goes to
+// brow[(c P + padded node) 20 + q' 5 + c'], zero for DOFs that do not exist.
+constexpr int BJF_GROUPS = 3;   // groups per 64-lane block
 __global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx,
-                                                   int ngx) {
-    __shared__ double M[20][21];
-    const int gu = (blockIdx.x % ngx) * 2, gv = (blockIdx.x / ngx) * 2;
+                                                   int ngx, int ngroups) {
+    __shared__ double colk[BJF_GROUPS][20];
+    const int gl = threadIdx.x / 20, j = threadIdx.x - 20 * gl;
+    const int grp = blockIdx.x * BJF_GROUPS + gl;
+    const bool live = gl < BJF_GROUPS && grp < ngroups;
+    const int gu = live ? (grp % ngx) * 2 : 0, gv = live ? (grp / ngx) * 2 : 0;
     auto exists = [&](int q, int c) {
         int u = gu + (q & 1), v = gv + (q >> 1);
         if (u >= W || v >= Lg) return false;
         return c < 3 || (u > 0 && u < W - 1 && v > 0 && v < Lg - 1);
     };
-    for (int idx = threadIdx.x; idx < 400; idx += 64) {
-        int i = idx / 20, j = idx - 20 * (idx / 20);
-        int qi = i / 5, ci = i - 5 * qi, qj = j / 5, cj = j - 5 * qj;
+    const int qj = j / 5, cj = j - 5 * qj;
+    const bool jex = live && exists(qj, cj);
+    double col[20];
+#pragma unroll
+    for (int i = 0; i < 20; ++i) {
+        const int qi = i / 5, ci = i - 5 * qi;
         double val = i == j ? 1.0 : 0.0;
-        if (exists(qi, ci) && exists(qj, cj)) {
+        if (jex && exists(qi, ci)) {
             int s = slot_of(ci, (qj & 1) - (qi & 1), (qj >> 1) - (qi >> 1), cj);
             val = s >= 0 ? A[a_base(ntx, gu + (qi & 1), gv + (qi >> 1)) + (size_t)s * TILE_N] : 0.0;
         }
-        M[i][j] = val;
+        col[i] = val;
     }
-    __syncthreads();
+#pragma unroll
     for (int k = 0; k < 20; ++k) {
-        const double ip = 1.0 / M[k][k];
-        double nv[7];
+        if (gl < BJF_GROUPS && j == k)
 #pragma unroll
-        for (int m = 0; m < 7; ++m) {
-            int idx = threadIdx.x + 64 * m;
-            nv[m] = 0.0;
-            if (idx >= 400) continue;
-            int i = idx / 20, j = idx - 20 * (idx / 20);
-            if (i == k && j == k) nv[m] = ip;
-            else if (i == k) nv[m] = M[k][j] * ip;
-            else if (j == k) nv[m] = -M[i][k] * ip;
-            else nv[m] = M[i][j] - M[i][k] * (M[k][j] * ip);
-        }
+            for (int i = 0; i < 20; ++i) colk[gl][i] = col[i];
         __syncthreads();
+        if (gl < BJF_GROUPS) {
+            double ck[20];
 #pragma unroll
-        for (int m = 0; m < 7; ++m) {
-            int idx = threadIdx.x + 64 * m;
-            if (idx < 400) M[idx / 20][idx - 20 * (idx / 20)] = nv[m];
+            for (int i = 0; i < 20; ++i) ck[i] = colk[gl][i];
+            const double ip = 1.0 / ck[k];
+            if (j == k) {
+#pragma unroll
+                for (int i = 0; i < 20; ++i) col[i] = i == k ? ip : -ck[i] * ip;
+            } else {
+                const double f = col[k] * ip;
+#pragma unroll
+                for (int i = 0; i < 20; ++i) col[i] = i == k ? f : col[i] - ck[i] * f;
+            }
         }
         __syncthreads();
     }
-    for (int idx = threadIdx.x; idx < 400; idx += 64) {
-        int i = idx / 20, j = idx - 20 * (idx / 20);
-        int qi = i / 5, ci = i - 5 * qi, qj = j / 5, cj = j - 5 * qj;
-        int u = gu + (qi & 1), v = gv + (qi >> 1);
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) {
+        const int qi = i / 5, ci = i - 5 * qi;
+        const int u = gu + (qi & 1), v = gv + (qi >> 1);
         if (u >= W || v >= Lg) continue;
-        brow[((size_t)ci * P + padidx(Wp, u, v)) * 20 + j] = exists(qi, ci) && exists(qj, cj) ? M[i][j] : 0.0;
+        brow[((size_t)ci * P + padidx(Wp, u, v)) * 20 + j] = jex && exists(qi, ci) ? col[i] : 0.0;
     }
 }
 
@@ -1641,8 +1649,9 @@ const void* cgp_kernel(int kp) {
 
 // The block-Jacobi rows of the matrix Aptr (k_bj_factor), before a block-Jacobi solve.
 int bj_factor(yc_ctx* c, const double* Aptr) {
-    const int ngx = (c->W + 1) / 2, ngy = (c->Lg + 1) / 2;
-    hipLaunchKernelGGL(k_bj_factor, dim3(ngx * ngy), dim3(64), 0, c->stream, Aptr, c->bj, c->W, c->Lg, c->Wp, c->P, c->ntx, ngx);
+    const int ngx = (c->W + 1) / 2, ngy = (c->Lg + 1) / 2, ng = ngx * ngy;
+    hipLaunchKernelGGL(k_bj_factor, dim3((ng + BJF_GROUPS - 1) / BJF_GROUPS), dim3(64), 0, c->stream, Aptr, c->bj, c->W, c->Lg,
+                       c->Wp, c->P, c->ntx, ngx, ng);
     HIPCHK(hipGetLastError());
     return YC_OK;
 }
