@@ -64,6 +64,27 @@ class Cloth:
             [_arr(s[k], (N, 2)) for k in ("EPos", "EPosBar", "EVel")]
         _abi.check(_abi.load_library().yc_set_state(self._h, *[_dp(x) for x in a]))
 
+    # --- state in device memory (torch tensors on the context's device) ---
+    def set_ClothState_device(self, s, stream=None):
+        """s: the get_ClothState keys as contiguous float64 CUDA tensors ((N, 3) / (N, 2));
+        the copy is ordered on `stream` (default: torch's current stream)."""
+        import torch
+        t = [s[k] for k in self.STATE_KEYS]
+        for k, x in zip(self.STATE_KEYS, t):
+            if not (x.is_cuda and x.dtype == torch.float64 and x.is_contiguous()) or x.numel() != self.N * (3 if k[0] == "L" else 2):
+                raise ValueError("%s: expected a contiguous float64 CUDA tensor of %d x %d" % (k, self.N, 3 if k[0] == "L" else 2))
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        _abi.check(_abi.load_library().yc_set_state_device(self._h, *[x.data_ptr() for x in t], st))
+
+    def get_ClothState_device(self, stream=None):
+        """The state as float64 CUDA tensors, written in `stream` order (default: torch's
+        current stream)."""
+        import torch
+        st_ = stream or torch.cuda.current_stream()
+        out = {k: torch.empty((self.N, 3 if k[0] == "L" else 2), dtype=torch.float64, device="cuda") for k in self.STATE_KEYS}
+        _abi.check(_abi.load_library().yc_get_state_device(self._h, *[out[k].data_ptr() for k in self.STATE_KEYS], st_.cuda_stream))
+        return out
+
     def set_ext_force(self, f):
         if f is None:
             _abi.check(_abi.load_library().yc_set_ext_force(self._h, _NULL))

@@ -1497,6 +1497,7 @@ struct yc_ctx {
     int coop_bj_ok;                         // ... with the block-Jacobi preconditioner
     int precond;                            // YC_PRECOND_*
     double* bj;                             // block-Jacobi rows, 5 P x 20 (when selected)
+    hipEvent_t io_event;                    // orders device state transfers with a caller's stream
     hipGraphExec_t graph1;
     // forward-mode tangents of the step (directional derivatives)
     int ndir;
@@ -1576,6 +1577,41 @@ bool pattern_flip(int pattern, int i, int j) {
     if (pattern == 1) { int rv = j % 4; int ru = (i + rv) % 4; return ru < 2; }
     int rv = (j - 1) % 4; int q = rv / 2; int r = rv % 2; int iu = q * 3 + r * 2; int ru = (i - 1 + iu) % 4;
     return ru == 0;
+}
+
+// State between a caller's node-major device arrays ((N, 3) and (N, 2), state index
+// u + v * width) and the context's component planes, one thread per node.
+struct StateIO {
+    const double* in[6];   // lpos, lposfix, lvel, epos, eposbar, evel (set) or null
+    double* out[6];        // the same (get) or null
+};
+
+__global__ void __launch_bounds__(NBLK) k_state_in(StateDev s, StateIO io) {
+    int k = blockIdx.x * NBLK + threadIdx.x;
+    if (k >= s.N) return;
+    for (int q = 0; q < 3; ++q) {
+        s.x[q * s.N + k] = io.in[0][3 * k + q];
+        s.xfix[q * s.N + k] = io.in[1][3 * k + q];
+        s.v[q * s.N + k] = io.in[2][3 * k + q];
+    }
+    for (int q = 0; q < 2; ++q) {
+        s.uv[q * s.N + k] = io.in[3][2 * k + q];
+        s.uvbar[q * s.N + k] = io.in[4][2 * k + q];
+        s.uvd[q * s.N + k] = io.in[5][2 * k + q];
+    }
+}
+
+__global__ void __launch_bounds__(NBLK) k_state_out(StateDev s, StateIO io) {
+    int k = blockIdx.x * NBLK + threadIdx.x;
+    if (k >= s.N) return;
+    const double* src3[3] = {s.x, s.xfix, s.v};
+    const double* src2[3] = {s.uv, s.uvbar, s.uvd};
+    for (int a = 0; a < 3; ++a)
+        if (io.out[a])
+            for (int q = 0; q < 3; ++q) io.out[a][3 * k + q] = src3[a][q * s.N + k];
+    for (int a = 0; a < 3; ++a)
+        if (io.out[3 + a])
+            for (int q = 0; q < 2; ++q) io.out[3 + a][2 * k + q] = src2[a][q * s.N + k];
 }
 
 int refresh_geometry(yc_ctx* c) {
@@ -2110,6 +2146,7 @@ void yc_destroy(yc_ctx* c) {
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
+    if (c->io_event) (void)hipEventDestroy(c->io_event);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2148,6 +2185,44 @@ int yc_set_state(yc_ctx* c, const double* lpos, const double* lposfix, const dou
     HIPCHK(hipStreamSynchronize(c->stream));
     c->assembled = c->solved = false;
     return YC_OK;
+}
+
+// Orders the context's stream after the caller's (before == true) or the caller's after
+// the context's.
+static int join_streams(yc_ctx* c, hipStream_t caller, bool before) {
+    if (!c->io_event && hipEventCreateWithFlags(&c->io_event, hipEventDisableTiming) != hipSuccess)
+        return fail(YC_ERR_HIP, "hipEventCreateWithFlags");
+    hipStream_t from = before ? caller : c->stream, to = before ? c->stream : caller;
+    HIPCHK(hipEventRecord(c->io_event, from));
+    HIPCHK(hipStreamWaitEvent(to, c->io_event, 0));
+    return YC_OK;
+}
+
+int yc_set_state_device(yc_ctx* c, const double* lpos, const double* lposfix, const double* lvel, const double* epos,
+                        const double* eposbar, const double* evel, void* stream) {
+    if (!c || !lpos || !lposfix || !lvel || !epos || !eposbar || !evel) return fail(YC_ERR_ARG, "null argument");
+    hipStream_t cs = (hipStream_t)stream;
+    int rc = join_streams(c, cs, true);
+    if (rc) return rc;
+    StateIO io = {{lpos, lposfix, lvel, epos, eposbar, evel}, {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
+    hipLaunchKernelGGL(k_state_in, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), io);
+    HIPCHK(hipGetLastError());
+    rc = refresh_geometry(c);
+    if (rc) return rc;
+    c->assembled = c->solved = false;
+    return join_streams(c, cs, false);
+}
+
+int yc_get_state_device(yc_ctx* c, double* lpos, double* lposfix, double* lvel, double* epos, double* eposbar,
+                        double* evel, void* stream) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    hipStream_t cs = (hipStream_t)stream;
+    int rc = join_streams(c, cs, true);
+    if (rc) return rc;
+    StateIO io = {{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}, {lpos, lposfix, lvel, epos, eposbar, evel}};
+    hipLaunchKernelGGL(k_state_out, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), io);
+    HIPCHK(hipGetLastError());
+    return join_streams(c, cs, false);
 }
 
 int yc_get_state(yc_ctx* c, double* lpos, double* lposfix, double* lvel, double* epos, double* eposbar, double* evel) {

@@ -80,6 +80,20 @@ int yc_set_state(yc_ctx* ctx, const double* lpos, const double* lposfix, const d
 int yc_get_state(yc_ctx* ctx, double* lpos, double* lposfix, double* lvel,
                  double* epos, double* eposbar, double* evel);
 
+/* yc_set_state / yc_get_state with device arrays (on the context's device, same layouts):
+ * the state moves between the caller's HBM buffers and the context without a host copy.
+ * `stream` is the caller's hipStream_t (NULL: the null stream): the transfer starts after
+ * the work queued on it so far, and work queued on it afterwards starts after the
+ * transfer, so the caller may reuse or read its buffers in stream order; neither call
+ * waits on the host.  No counterpart in the reference (its state is CPU tensors); for
+ * callers that keep the state in device memory, such as a LibTorch or PyTorch training
+ * loop on the GPU.  Every pointer of yc_set_state_device is required; any of
+ * yc_get_state_device may be NULL. */
+int yc_set_state_device(yc_ctx* ctx, const double* lpos, const double* lposfix, const double* lvel,
+                        const double* epos, const double* eposbar, const double* evel, void* stream);
+int yc_get_state_device(yc_ctx* ctx, double* lpos, double* lposfix, double* lvel,
+                        double* epos, double* eposbar, double* evel, void* stream);
+
 /* External generalized force added to b (the ext_force of
  * SimStep(cloth, phys, h, ext_force), src/TrainHelper.cpp:56-63).  n doubles in the
  * global layout, copied to the device; NULL clears it. */

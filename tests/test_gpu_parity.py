@@ -166,3 +166,39 @@ def test_non_square_cloth(ys, cs, W, Lg, pattern):
     say("  %dx%d 30-step rel %.2e" % (W, Lg, e))
     assert e < 1e-6
     g.close()
+
+
+def test_device_state_round_trip(ys, cs):
+    """yc_set_state_device / yc_get_state_device with torch tensors on the device: the
+    same state as the host path, in the caller's stream order, and a step from a device
+    state equals a step from the same host state."""
+    import torch
+    spec = cs.heter_spec(12, "twill", 3)
+    spec["length"] = 9   # non-square: the state index is u + v * width
+    c0 = ys.Cloth(spec)   # a moving state: three steps from rest
+    c0.set_ClothState(cs.initial_state(spec))
+    c0.run(spec["h"], 3, tol=1e-10)
+    st = c0.get_ClothState()
+    c0.close()
+    a, b = ys.Cloth(spec), ys.Cloth(spec)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        dev = {k: torch.from_numpy(np.ascontiguousarray(v)).to("cuda", non_blocking=False) for k, v in st.items()}
+        a.set_ClothState_device(dev, stream=s)
+        for k in dev:
+            dev[k].zero_()    # queued after the transfer on the same stream
+        back = a.get_ClothState_device(stream=s)
+    s.synchronize()
+    b.set_ClothState(st)
+    host = a.get_ClothState()
+    for k in st:
+        assert np.array_equal(host[k], np.asarray(st[k])), k
+        assert np.array_equal(back[k].cpu().numpy(), np.asarray(st[k])), k
+    for c in (a, b):
+        it, rr, rc = c.step(spec["h"], tol=1e-10)
+        assert rc == 0
+    sa, sb = a.get_ClothState(), b.get_ClothState()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
