@@ -57,7 +57,7 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_time_solve", "yc_cg_iteration_bytes",
            "yc_batch_create", "yc_batch_destroy", "yc_batch_set_opts", "yc_batch_get_opts", "yc_batch_run",
            "yc_batch_info", "yc_time_assemble", "yc_tangent_set_ext", "yc_set_preconditioner",
-           "yc_set_state_device", "yc_get_state_device"]
+           "yc_set_state_device", "yc_get_state_device", "yc_export_precond"]
 
 
 def lib_path():
@@ -100,6 +100,7 @@ def load_library():
     L.yc_set_cg_variant.argtypes = [vp, ctypes.c_int]
     L.yc_set_preconditioner.argtypes = [vp, ctypes.c_int]
     L.yc_set_state_device.argtypes = [vp] + [ctypes.c_void_p] * 7
+    L.yc_export_precond.argtypes = [vp, dp]
     L.yc_get_state_device.argtypes = [vp] + [ctypes.c_void_p] * 7
     L.yc_tangent_begin.argtypes = [vp, ctypes.c_int, dp, dp]
     L.yc_step_tangent.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]

@@ -266,6 +266,12 @@ class Cloth:
     def set_preconditioner(self, name):
         _abi.check(_abi.load_library().yc_set_preconditioner(self._h, self.PRECONDITIONERS[name]))
 
+    def precond_rows(self):
+        """The block-Jacobi rows of the assembled A, (n, 20) (see yc_export_precond)."""
+        out = np.zeros((self.n, 20))
+        _abi.check(_abi.load_library().yc_export_precond(self._h, _dp(out)))
+        return out
+
     def time_spmv(self, reps=200):
         ms = ctypes.c_double()
         _abi.check(_abi.load_library().yc_time_spmv(self._h, int(reps), ctypes.byref(ms)))

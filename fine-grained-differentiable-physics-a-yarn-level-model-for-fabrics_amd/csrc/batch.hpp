@@ -286,6 +286,7 @@ int batch_node_phase(yc_batch* b, double h, int phase) {
 // One step of every cloth, queued on the batch stream.
 int batch_step(yc_batch* b, double h) {
     const int n = (int)b->m.size();
+    for (yc_ctx* c : b->m) c->bj_of = nullptr;   // the batch rewrites every A
     const BatchItem* it = b->d_items;
     hipLaunchKernelGGL(k_b_epoch, dim3((n + 63) / 64), dim3(64), 0, b->stream, it, n);
     hipLaunchKernelGGL(k_b_assemble1, dim3(b->n_tile), dim3(ASM1_THREADS), ASM1_LDS, b->stream, it, (const int*)b->map_tile, h);

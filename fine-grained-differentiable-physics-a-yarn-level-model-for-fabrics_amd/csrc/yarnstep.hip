@@ -188,6 +188,7 @@ struct AsmOut {
     double* invdiag;    // padded 5-plane vector
     const double* ext;  // global-layout n vector or null
     const double* dext; // tangent run: the direction's tangent of ext (global layout) or null
+    double* bj;         // value run: block-Jacobi rows to compute from the final rows, or null
     int* flag;          // [0] structural error, [1] non-finite
     int Wp, P;          // padded width, plane stride
     // tangent run: tangents of A (exact mode, else null), F, b, is_slide and comp, and
@@ -228,10 +229,6 @@ __device__ __forceinline__ int tile_node(const ClothDev& C, int tile) { return t
 // T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
 // src/TrainHelper.cpp:19-20).  T = dd: their directional derivatives along the
 // tangents held in C, into the d* outputs of O.
-// The body stays in the kernel itself.  Built as an inlined device function called by
-// a kernel (by reference or by value), this compiler produced wrong rows for the nodes
-// of tiles where some lanes skip an edge (boundary nodes: one edge's self block landed
-// two nodes away); the batched step therefore launches this kernel once per cloth.
 // The kernel body is shared with the batched kernel by textual inclusion: built as an
 // inlined device function called from a kernel (its descriptors by reference or by
 // value), this compiler produced wrong rows for boundary nodes (one edge's self block
@@ -253,10 +250,8 @@ __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& 
 __device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
 
 template <class T>
-__device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& O, double h, int tile) {
+__device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& O, double h, int k) {
     constexpr bool tan = !std::is_same<T, double>::value;
-    int k = tile_node(C, tile);
-    if (k < 0) return;
     RowSinkT<T> S;
     S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
     int u = S.mu_, v = S.mv_;
@@ -325,6 +320,20 @@ __device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& 
     }
     if (bad) atomicOr(&O.flag[1], 1);
     if (S.bad) atomicOr(&O.flag[0], 1);
+}
+
+__device__ void bj_factor_tile(const ClothDev& C, const AsmOut& O, int tile);
+
+template <class T>
+__device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& O, double h, int tile) {
+    int k = tile_node(C, tile);
+    if (k >= 0) assemble2_node<T>(C, O, h, k);
+    if constexpr (std::is_same<T, double>::value) {
+        if (O.bj) {   // the tile's 2 x 2 groups from its final rows, still in LDS
+            __syncthreads();
+            bj_factor_tile(C, O, tile);
+        }
+    }
 }
 
 template <class T>
@@ -1191,28 +1200,29 @@ __global__ void __launch_bounds__(BLK) k_cgp(CgpArgs a) { cgp_tile<KP, BJ>(a, bl
 // LDS to the group's other lanes.  Row (component c, node) goes to
 // brow[(c P + padded node) 20 + q' 5 + c'], zero for DOFs that do not exist.
 constexpr int BJF_GROUPS = 3;   // groups per 64-lane block
-__global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx,
-                                                   int ngx, int ngroups) {
-    __shared__ double colk[BJF_GROUPS][20];
-    const int gl = threadIdx.x / 20, j = threadIdx.x - 20 * gl;
-    const int grp = blockIdx.x * BJF_GROUPS + gl;
-    const bool live = gl < BJF_GROUPS && grp < ngroups;
-    const int gu = live ? (grp % ngx) * 2 : 0, gv = live ? (grp / ngx) * 2 : 0;
-    auto exists = [&](int q, int c) {
-        int u = gu + (q & 1), v = gv + (q >> 1);
-        if (u >= W || v >= Lg) return false;
-        return c < 3 || (u > 0 && u < W - 1 && v > 0 && v < Lg - 1);
-    };
+
+__device__ __forceinline__ bool bj_exists(int W, int Lg, int gu, int gv, int q, int c) {
+    int u = gu + (q & 1), v = gv + (q >> 1);
+    if (u >= W || v >= Lg) return false;
+    return c < 3 || (u > 0 && u < W - 1 && v > 0 && v < Lg - 1);
+}
+
+// Inverts the group at (gu, gv) held by lanes 20 gl + j of a 64-thread block and stores its
+// rows; every lane calls it (it passes 40 barriers).  load(qi, ci, s): the entry of node qi's
+// row component ci in slot s.
+template <class Load>
+__device__ __forceinline__ void bj_invert_store(int W, int Lg, int Wp, int P, double* brow, bool live, int gu, int gv,
+                                                int gl, int j, double (*colk)[20], Load load) {
     const int qj = j / 5, cj = j - 5 * qj;
-    const bool jex = live && exists(qj, cj);
+    const bool jex = live && bj_exists(W, Lg, gu, gv, qj, cj);
     double col[20];
 #pragma unroll
     for (int i = 0; i < 20; ++i) {
         const int qi = i / 5, ci = i - 5 * qi;
         double val = i == j ? 1.0 : 0.0;
-        if (jex && exists(qi, ci)) {
+        if (jex && bj_exists(W, Lg, gu, gv, qi, ci)) {
             int s = slot_of(ci, (qj & 1) - (qi & 1), (qj >> 1) - (qi >> 1), cj);
-            val = s >= 0 ? A[a_base(ntx, gu + (qi & 1), gv + (qi >> 1)) + (size_t)s * TILE_N] : 0.0;
+            val = s >= 0 ? load(qi, ci, s) : 0.0;
         }
         col[i] = val;
     }
@@ -1244,7 +1254,38 @@ __global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow,
         const int qi = i / 5, ci = i - 5 * qi;
         const int u = gu + (qi & 1), v = gv + (qi >> 1);
         if (u >= W || v >= Lg) continue;
-        brow[((size_t)ci * P + padidx(Wp, u, v)) * 20 + j] = jex && exists(qi, ci) ? col[i] : 0.0;
+        brow[((size_t)ci * P + padidx(Wp, u, v)) * 20 + j] = jex && bj_exists(W, Lg, gu, gv, qi, ci) ? col[i] : 0.0;
+    }
+}
+
+// Standalone form, from A in HBM: three groups per 64-thread block.
+__global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx,
+                                                   int ngx, int ngroups) {
+    __shared__ double colk[BJF_GROUPS][20];
+    const int gl = threadIdx.x / 20, j = threadIdx.x - 20 * gl;
+    const int grp = blockIdx.x * BJF_GROUPS + gl;
+    const bool live = gl < BJF_GROUPS && grp < ngroups;
+    const int gu = live ? (grp % ngx) * 2 : 0, gv = live ? (grp / ngx) * 2 : 0;
+    bj_invert_store(W, Lg, Wp, P, brow, live, gu, gv, gl, j, colk, [&](int qi, int, int s) {
+        return A[a_base(ntx, gu + (qi & 1), gv + (qi >> 1)) + (size_t)s * TILE_N];
+    });
+}
+
+// Fused form, at the end of the second assembly kernel: the tile's 16 groups (groups never
+// straddle 8 x 8 tiles) from its final rows in LDS, three at a time.
+__device__ void bj_factor_tile(const ClothDev& C, const AsmOut& O, int tile) {
+    extern __shared__ double lds_rows[];   // slot s of node lane l at s * TILE_N + l
+    __shared__ double colk[BJF_GROUPS][20];
+    const int tu0 = (tile % C.ntx) * TILE, tv0 = (tile / C.ntx) * TILE;
+    const int gl = threadIdx.x / 20, j = threadIdx.x - 20 * gl;
+    constexpr int NG = (TILE / 2) * (TILE / 2);
+    for (int round = 0; round < (NG + BJF_GROUPS - 1) / BJF_GROUPS; ++round) {
+        const int g = round * BJF_GROUPS + gl;
+        const int gx = 2 * (g % (TILE / 2)), gy = 2 * (g / (TILE / 2));
+        const bool live = gl < BJF_GROUPS && g < NG && tu0 + gx < C.W && tv0 + gy < C.Lg;
+        bj_invert_store(C.W, C.Lg, O.Wp, O.P, O.bj, live, tu0 + gx, tv0 + gy, gl, j, colk, [&](int qi, int, int s) {
+            return lds_rows[(size_t)s * TILE_N + (gx + (qi & 1)) + (gy + (qi >> 1)) * TILE];
+        });
     }
 }
 
@@ -1497,6 +1538,7 @@ struct yc_ctx {
     int coop_bj_ok;                         // ... with the block-Jacobi preconditioner
     int precond;                            // YC_PRECOND_*
     double* bj;                             // block-Jacobi rows, 5 P x 20 (when selected)
+    const double* bj_of;                    // the matrix (A or At) bj was computed from, or null
     hipEvent_t io_event;                    // orders device state transfers with a caller's stream
     hipGraphExec_t graph1;
     // forward-mode tangents of the step (directional derivatives)
@@ -1542,6 +1584,7 @@ AsmOut make_out(yc_ctx* c) {
     O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
     O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
     O.dext = nullptr;
+    O.bj = nullptr;
     O.detach = 0;
     O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
     O.bend_rec = c->bend_rec;
@@ -1683,23 +1726,26 @@ const void* cgp_kernel(int kp) {
     }
 }
 
-// The block-Jacobi rows of the matrix Aptr (k_bj_factor), before a block-Jacobi solve.
+// The block-Jacobi rows of the matrix Aptr (k_bj_factor), unless they are already those
+// of Aptr: c->bj_of names the matrix the rows were computed from (the assembly computes
+// them for A; everything that rewrites A or At without them clears it).
 int bj_factor(yc_ctx* c, const double* Aptr) {
+    if (c->bj_of == Aptr) return YC_OK;
     const int ngx = (c->W + 1) / 2, ngy = (c->Lg + 1) / 2, ng = ngx * ngy;
     hipLaunchKernelGGL(k_bj_factor, dim3((ng + BJF_GROUPS - 1) / BJF_GROUPS), dim3(64), 0, c->stream, Aptr, c->bj, c->W, c->Lg,
                        c->Wp, c->P, c->ntx, ngx, ng);
     HIPCHK(hipGetLastError());
+    c->bj_of = Aptr;
     return YC_OK;
 }
 
-// factor: with the block-Jacobi preconditioner, compute its rows from the matrix first
-// (a caller timing repeated solves of one matrix factors once and passes false).
-int launch_cgp(yc_ctx* c, double tol, int maxit, const double* Aptr = nullptr, bool factor = true) {
+// With the block-Jacobi preconditioner the rows of the matrix solved are made current first.
+int launch_cgp(yc_ctx* c, double tol, int maxit, const double* Aptr = nullptr) {
     CgpArgs a;
     a.A = Aptr ? Aptr : c->A; a.invd = c->invd; a.bbpart = c->part.bb; a.rrpart = c->part.rr;
     const bool bj = c->precond == YC_PRECOND_BLOCK2;
     a.brow = bj ? c->bj : nullptr;
-    if (bj && factor) {
+    if (bj) {
         int rc = bj_factor(c, a.A);
         if (rc) return rc;
     }
@@ -1786,7 +1832,13 @@ int do_assemble(yc_ctx* c, double h, bool async = false) {
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
                            c->shear_rec, 4 * c->Nc);
-    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, make_out(c), h);
+    {
+        // with the block-Jacobi preconditioner the second kernel also computes its rows
+        AsmOut O2 = make_out(c);
+        O2.bj = c->precond == YC_PRECOND_BLOCK2 ? c->bj : nullptr;
+        hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, O2, h);
+        c->bj_of = O2.bj ? c->A : nullptr;
+    }
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
                        c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
@@ -2590,8 +2642,10 @@ int yc_step_tangent(yc_ctx* c, double h, int mode, double tol, int maxit, yc_ste
     const size_t PV = (size_t)5 * c->P;
     HIPCHK(hipMemcpyAsync(c->xsol, c->cx, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->bsave, c->b, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if (mode == YC_GRAD_REF)
+    if (mode == YC_GRAD_REF) {
+        if (c->bj_of == c->At) c->bj_of = nullptr;   // At is rewritten
         hipLaunchKernelGGL(k_transpose, dim3(c->nT), dim3(NBLK), 0, c->stream, (const double*)c->A, c->At, c->W, c->Lg, c->ntx);
+    }
     HIPCHK(hipGetLastError());
     for (int d = 0; d < c->ndir && !rc; ++d) rc = tangent_dir(c, d, mode, h, tol, maxit, nullptr);
     // restore the step's own right-hand side and solution, then update
@@ -2627,6 +2681,31 @@ int yc_set_preconditioner(yc_ctx* c, int kind) {
         }
     }
     c->precond = kind;
+    c->bj_of = nullptr;
+    return YC_OK;
+}
+
+int yc_export_precond(yc_ctx* c, double* rows) {
+    if (!c || !rows) return fail(YC_ERR_ARG, "null argument");
+    if (c->precond != YC_PRECOND_BLOCK2) return fail(YC_ERR_ARG, "the context does not use the block-Jacobi preconditioner");
+    if (!c->assembled) return fail(YC_ERR_ARG, "yc_export_precond before yc_assemble");
+    int rc = bj_factor(c, c->A);
+    if (rc) return rc;
+    std::vector<double> pad((size_t)5 * c->P * 20);
+    HIPCHK(hipMemcpyAsync(pad.data(), c->bj, pad.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int v = 0; v < c->Lg; ++v)
+        for (int u = 0; u < c->W; ++u) {
+            const int k = u + v * c->W, pk = (u + 2) + (v + 2) * c->Wp;
+            for (int q = 0; q < 3; ++q)
+                std::memcpy(rows + (size_t)(3 * k + q) * 20, pad.data() + ((size_t)q * c->P + pk) * 20, 20 * sizeof(double));
+            if (u > 0 && u < c->W - 1 && v > 0 && v < c->Lg - 1) {
+                const int e = (u - 1) + (v - 1) * (c->W - 2);
+                for (int q = 0; q < 2; ++q)
+                    std::memcpy(rows + ((size_t)3 * c->N + 2 * e + q) * 20, pad.data() + ((size_t)(3 + q) * c->P + pk) * 20,
+                                20 * sizeof(double));
+            }
+        }
     return YC_OK;
 }
 
@@ -2660,12 +2739,13 @@ int yc_time_solve(yc_ctx* c, double tol, int maxit, int reps, double* avg_ms, in
     HIPCHK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(YC_ERR_HIP, "hipEventCreate"); }
     double tot = 0.0;
-    // the block-Jacobi rows of the one matrix, once: the event pairs time the solve kernel
+    // the block-Jacobi rows of the one matrix are made current before the first launch
+    // (untimed): the event pairs time the solve kernel
     int rc = c->precond == YC_PRECOND_BLOCK2 ? bj_factor(c, c->A) : YC_OK;
     for (int i = 0; i <= reps && !rc; ++i) {   // the first launch is untimed
         hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 0);
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(YC_ERR_HIP, "hipEventRecord"); break; }
-        rc = launch_cgp(c, tol, maxit, nullptr, false);
+        rc = launch_cgp(c, tol, maxit);
         if (rc) break;
         if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = fail(YC_ERR_HIP, "event"); break; }
         float ms = 0.0f;

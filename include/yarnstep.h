@@ -157,6 +157,11 @@ int yc_set_cg_variant(yc_ctx* ctx, int variant);
 #define YC_PRECOND_JACOBI 0
 #define YC_PRECOND_BLOCK2 1
 int yc_set_preconditioner(yc_ctx* ctx, int kind);
+/* Inspection hook: the block-Jacobi rows of the assembled A, n x 20 doubles, DOF i (global
+ * layout) at rows[20 i]: the coefficients of its row of M^-1 over its 2 x 2 group, node
+ * q = du + 2 dv from the group's origin (even u, v) and component c at 5 q + c, zero where
+ * the group has no such DOF.  YC_PRECOND_BLOCK2 contexts only, after yc_assemble. */
+int yc_export_precond(yc_ctx* ctx, double* rows);
 
 /* Runs `steps` SimSteps back to back on the device with zero-guess CG; returns the
  * total CG iterations and the worst status.  With YC_CG_PERSISTENT the steps are queued

@@ -183,3 +183,51 @@ def test_block_jacobi_needs_the_persistent_cg(ys, cs):
     b = ys.ClothBatch([g])
     b.close()
     g.close()
+
+
+@pytest.mark.parametrize("idx", [0, 2])
+def test_block_jacobi_rows_are_the_group_inverses(ys, cs, idx):
+    """The preconditioner rows against numpy inverses of the assembled A's 2 x 2-group
+    blocks, for the rows the assembly computes and for the standalone kernel's (the same
+    bits)."""
+    name, spec = _specs(cs)[idx]
+    W, Lg = spec["width"], spec["length"]
+    N = W * Lg
+    r0 = OracleCloth(spec)
+    r0.set_state(cs.initial_state(spec))
+    for _ in range(3):
+        r0.step(spec["h"], None, tol=1e-12)
+    g = _block(ys, spec)
+    g.set_ClothState(r0.get_state())
+    g.assemble(spec["h"])
+    fused = g.precond_rows()
+    g.set_preconditioner("block2")       # forgets the rows: the next ones come from k_bj_factor
+    alone = g.precond_rows()
+    assert np.array_equal(fused, alone)
+    A = g.csr().toarray()
+    worst = 0.0
+    for gv in range(0, Lg, 2):
+        for gu in range(0, W, 2):
+            idx_, slots = [], []
+            for q in range(4):
+                u, v = gu + (q & 1), gv + (q >> 1)
+                if u >= W or v >= Lg:
+                    continue
+                k = u + v * W
+                for c in range(5):
+                    if c < 3:
+                        idx_.append(3 * k + c)
+                    elif 0 < u < W - 1 and 0 < v < Lg - 1:
+                        idx_.append(3 * N + 2 * ((u - 1) + (v - 1) * (W - 2)) + c - 3)
+                    else:
+                        continue
+                    slots.append(5 * q + c)
+            Binv = np.linalg.inv(A[np.ix_(idx_, idx_)])
+            got = fused[idx_][:, slots]
+            worst = max(worst, np.abs(got - Binv).max() / np.abs(Binv).max())
+            # coefficients of DOFs the group does not have are zero
+            other = [m for m in range(20) if m not in slots]
+            assert not fused[idx_][:, other].any()
+    say("  %s: max |rows - inv| / max|inv| over groups %.2e" % (name, worst))
+    assert worst < 1e-12
+    g.close()
