@@ -75,6 +75,15 @@ def load_library():
     p = lib_path()
     if not os.path.exists(p):
         raise YcError(YC_ERR_HIP, "libyarnstep.so is not built (run __graft_entry__.build())")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname libamdhip64.so.7,
+    # requested by torch as "libamdhip64.so").  Loaded after this library, torch would map a
+    # second runtime beside the system one this library requests; loaded first, its runtime
+    # is the one this library binds to (same soname).  Two runtimes in one process have
+    # failed to find the device in torch after many contexts had come and gone.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(p)
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
