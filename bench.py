@@ -15,6 +15,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -113,6 +115,33 @@ def large_measure(ys, steps):
                          "bytes_per_assembly": asm_bytes, "avg_ms": asm_ms}}
 
 
+def tangent_measure(ys, spec, state, steps, precond):
+    """The differentiation path on the same workload: SimSteps with the forward-mode
+    tangents of the eight learned material parameters (the reference's gradient, and the
+    exact one), as a parameter-learning rollout runs them."""
+    names = ("rho0", "rho1", "Y0", "Y1", "B0", "B1", "S", "mu")
+    out = {"workload": "the same cloth and state, %d steps, tangents of %s" % (steps, ", ".join(names)),
+           "directions": len(names)}
+    for mode in ("ref", "exact"):
+        c = ys.Cloth(spec)
+        c.set_cg_variant("persistent")
+        c.set_preconditioner(precond)
+        c.set_ClothState(state)
+        dp = np.zeros((len(names), len(c.PARAMS)))
+        for i, n in enumerate(names):
+            dp[i, c.PARAMS.index(n)] = 1.0
+        c.tangent_begin(dp)
+        c.step_tangent(spec["h"], mode, tol=1e-8)   # untimed
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            c.step_tangent(spec["h"], mode, tol=1e-8)
+        dt = time.perf_counter() - t0
+        c.tangent_end()
+        c.close()
+        out[mode] = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps}
+    return out
+
+
 def measured_traffic(kernel, workload, iterations=None):
     """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE +
     WRITE_SIZE runs, scaled by the copy calibration), or None when there is none.  A
@@ -140,6 +169,8 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="time only the zero-guess CG path (profiling runs)")
     ap.add_argument("--batch-steps", type=int, default=20, help="steps of the 64-cloth batch line (0: skip it)")
     ap.add_argument("--large-steps", type=int, default=10, help="steps of the 256x256 line (0: skip it)")
+    ap.add_argument("--tangent-steps", type=int, default=10,
+                    help="steps of the differentiation line (8 parameter tangents; 0: skip it)")
     ap.add_argument("--cg", default="persistent", choices=["persistent", "fused", "reference"],
                     help="CG form of the timed runs (the reference form is also timed once, zero guess)")
     ap.add_argument("--precond", default="block2", choices=["block2", "jacobi"],
@@ -280,6 +311,8 @@ def main():
         out["config"]["batch"] = batch_measure(ys, args.batch_steps)
     if args.large_steps > 0 and not args.plain_only:
         out["config"]["homo256"] = large_measure(ys, args.large_steps)
+    if args.tangent_steps > 0 and not args.plain_only and args.cg == "persistent":
+        out["config"]["tangents"] = tangent_measure(ys, spec, state, args.tangent_steps, args.precond)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cs, args.workload, args.cpu_steps)
     print(json.dumps(out), flush=True)

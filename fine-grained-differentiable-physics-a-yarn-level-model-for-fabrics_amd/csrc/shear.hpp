@@ -158,17 +158,6 @@ __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& 
     for (int j = 0; j < 3; ++j) S.LL(KIND_KX, k[r], k[j], B[j]);
 }
 
-template <class T, class Sink>
-__device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp_arr,
-                              const double* dcomp_arr) {
-    ShearOut<T> o;
-    shear_compute<T>(C, k0, k1, k3, comp_arr, dcomp_arr, o);
-    const int k[3] = {k0, k1, k3};
-    S.FL(k0, o.F[0]); S.FL(k1, o.F[1]); S.FL(k3, o.F[2]);
-    for (int r = 0; r < 3; ++r)
-        for (int j = 0; j < 3; ++j) S.LL(KIND_KX, k[r], k[j], o.B[r][j]);
-}
-
 // Segment s = 4 e + q of interior crossing e, q in the reference's shear_segs order
 // (left,bot), (left,top), (right,bot), (right,top) (src/Cloth.cpp:170-181): its nodes.
 __device__ __forceinline__ void shear_nodes(const ClothDev& C, int s, int (&k)[3]) {
@@ -200,6 +189,26 @@ __device__ __forceinline__ void shear_elem_seg(const ClothDev& C, const double* 
 
 __global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* comp, double* rec, int nseg) {
     shear_elem_seg(C, comp, rec, nseg, blockIdx.x * 256 + threadIdx.x);
+}
+
+// Tangent run: the tangents of the same 30 doubles per role, one thread per segment,
+// along the direction held in C (its state and parameter tangents and the compression
+// tangents dcomp), in the value records' layout.
+__global__ void __launch_bounds__(256) k_shear_elem_tan(ClothDev C, const double* comp, const double* dcomp, double* rec,
+                                                        int nseg) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= nseg) return;
+    int k[3];
+    shear_nodes(C, s, k);
+    ShearOut<dd> o;
+    shear_compute<dd>(C, k[0], k[1], k[2], comp, dcomp, o);
+    for (int r = 0; r < 3; ++r) {
+        double* base = rec + (size_t)r * 30 * nseg + s;
+        base[0] = o.F[r].x.d; base[(size_t)nseg] = o.F[r].y.d; base[(size_t)2 * nseg] = o.F[r].z.d;
+        for (int j = 0; j < 3; ++j)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) base[(size_t)(3 + j * 9 + a * 3 + b) * nseg] = o.B[r][j].m[a][b].d;
+    }
 }
 
 // Reads the role-r record of segment s.

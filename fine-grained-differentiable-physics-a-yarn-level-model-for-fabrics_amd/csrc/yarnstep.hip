@@ -240,11 +240,9 @@ __global__ void __launch_bounds__(ASM1_THREADS, 1) k_assemble1(ClothDev C, AsmOu
 }
 
 // Shear segments touching the node (4 own, 2 each as warp neighbour of the left and
-// right crossings, 2 each as weft neighbour of the top and bottom crossings), then the
-// Jacobi diagonal and the NaN/Inf screen of the node's rows.
-template <class T, class Sink>
-__device__ void shear_element(const ClothDev& C, Sink& S, int k0, int k1, int k3, const double* comp,
-                              const double* dcomp);
+// right crossings, 2 each as weft neighbour of the top and bottom crossings), replayed from
+// the element-major records, then the Jacobi diagonal and the NaN/Inf screen of the node's
+// rows.
 template <class T, class Sink>
 __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& F, const mat3<T> (&B)[3]);
 __device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
@@ -275,16 +273,23 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
             int k1 = a == 0 ? K(cu - 1, cv) : K(cu + 1, cv);
             int k3 = bq == 0 ? K(cu, cv + 1) : K(cu, cv - 1);
             if (kc != k && k1 != k && k3 != k) continue;
+            // replay the segment's record for this node's role: its values (value run) or
+            // their tangents (tangent run, k_shear_elem_tan; the tangent outputs take only
+            // the tangent parts of the shear's forces and K_x blocks, so the values are 0)
+            int sidx = 4 * crossing(C, cu, cv) + 2 * a + bq;
+            int r = kc == k ? 0 : (k1 == k ? 1 : 2);
+            const int kk[3] = {kc, k1, k3};
+            d3 F;
+            m33 B[3];
+            shear_load(O.shear_rec, O.nseg, sidx, r, F, B);
             if constexpr (tan) {
-                shear_element<T>(C, S, kc, k1, k3, O.comp, O.dcomp);
+                v3<T> Ft = V3<T>(T(0.0, F.x), T(0.0, F.y), T(0.0, F.z));
+                mat3<T> Bt[3];
+                for (int j = 0; j < 3; ++j)
+                    for (int aa = 0; aa < 3; ++aa)
+                        for (int bb = 0; bb < 3; ++bb) Bt[j].m[aa][bb] = T(0.0, B[j].m[aa][bb]);
+                emit_shear_role<T>(S, kk, r, Ft, Bt);
             } else {
-                // replay the segment's record for this node's role
-                int sidx = 4 * crossing(C, cu, cv) + 2 * a + bq;
-                int r = kc == k ? 0 : (k1 == k ? 1 : 2);
-                const int kk[3] = {kc, k1, k3};
-                d3 F;
-                m33 B[3];
-                shear_load(O.shear_rec, O.nseg, sidx, r, F, B);
                 emit_shear_role<double>(S, kk, r, F, B);
             }
         }
@@ -2055,6 +2060,10 @@ int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int
     O.b = c->bsave;   // the tangent kernels write db only; b is not touched
     O.dext = c->tdext ? c->tdext + (size_t)d * c->n : nullptr;
     hipLaunchKernelGGL(k_assemble1<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
+    // the shear segments' tangents, once per segment, into the shear records (the step's
+    // own records are no longer needed: its assembly is done)
+    hipLaunchKernelGGL(k_shear_elem_tan, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, C, (const double*)c->comp,
+                       (const double*)c->dcomp, c->shear_rec, 4 * c->Nc);
     hipLaunchKernelGGL(k_assemble2<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
     HIPCHK(hipGetLastError());
     if (mode == YC_GRAD_EXACT) {
