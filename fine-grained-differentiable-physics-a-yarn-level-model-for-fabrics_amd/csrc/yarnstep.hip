@@ -239,6 +239,18 @@ __global__ void __launch_bounds__(ASM1_THREADS, 1) k_assemble1(ClothDev C, AsmOu
 #include "assemble1_body.inc"
 }
 
+// The tangent run of every direction in one launch (blockIdx.y: the direction, with its
+// own descriptors): one direction's tiles use a quarter of the SIMDs at one wavefront
+// each, so the directions go together.
+__global__ void __launch_bounds__(NBLK, 1) k_assemble1_dirs(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
+                                                           double h) {
+    typedef dd T;
+    const ClothDev C = Cs[blockIdx.y];
+    const AsmOut O = Os[blockIdx.y];
+    const int tile = blockIdx.x;
+#include "assemble1_body.inc"
+}
+
 // Shear segments touching the node (4 own, 2 each as warp neighbour of the left and
 // right crossings, 2 each as weft neighbour of the top and bottom crossings), replayed from
 // the element-major records, then the Jacobi diagonal and the NaN/Inf screen of the node's
@@ -1552,6 +1564,10 @@ struct yc_ctx {
     std::vector<double> tpar;               // ndir x YC_NPARAM
     double *dA, *dF, *db, *dis_slide, *dcomp, *At, *xsol, *bsave, *dsol, *trhs;
     double* tdext;                          // ndir x n tangents of the external force, or null
+    ClothDev* tdev;                         // ndir tangent descriptors (device), for k_assemble1_dirs
+    AsmOut* tout;                           // ndir tangent outputs (device)
+    std::vector<ClothDev> h_tdev;           // their host copies (kept alive for the upload)
+    std::vector<AsmOut> h_tout;
     double* shear_rec;                      // 3 roles x 30 x (4 Nc) shear records
     double* bend_rec;                       // 3 roles x BEND_REC x n_bends bend records
 };
@@ -2040,44 +2056,71 @@ __global__ void k_axpy_neg(const double* a, const double* y, double* out, int n)
     if (i < n) out[i] = a[i] - y[i];
 }
 
-// Tangent of one step along direction d, after the step's own assembly and solve (the
-// solution is in xsol, the right-hand side in bsave).  exact (mode 1): the derivative of
-// the step as computed, A x = b differentiated as A dx = db - dA x.  ref (mode 0): the
-// reference's gradient: A, M and K_v held constant, the non-jamming shear angle held
-// constant, and the linear solve transposed (A^T dx = db), so that the transpose of this
-// tangent map is the reference's backward solve with A (src/Solvers.cpp:51-53).
+// The outputs of direction d's tangent assembly (per-direction buffers, so that the first
+// kernel runs for every direction at once).
+AsmOut make_tan_out(yc_ctx* c, int d, int mode) {
+    const size_t N = c->N, PV = (size_t)5 * c->P;
+    AsmOut O = make_out(c);
+    O.dA = mode == YC_GRAD_EXACT ? c->dA + (size_t)d * c->a_size : nullptr;
+    O.dF = c->dF + (size_t)d * 5 * N; O.db = c->db + (size_t)d * PV;
+    O.dis_slide = c->dis_slide + (size_t)d * 2 * N; O.dcomp = c->dcomp + (size_t)d * N;
+    O.detach = mode == YC_GRAD_REF;
+    O.b = c->bsave;   // the tangent kernels write db only; b is not touched
+    O.dext = c->tdext ? c->tdext + (size_t)d * c->n : nullptr;
+    return O;
+}
+
+// First part of every direction's tangent: normal tangents, then the first assembly
+// kernel of all directions in one launch.
+int tangent_assemble_all(yc_ctx* c, int mode, double h) {
+    StateDev sd = make_state(c);
+    for (int d = 0; d < c->ndir; ++d)
+        hipLaunchKernelGGL(k_normals_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd,
+                           (const double*)tan_state(c, d).dx, tan_nrm(c, d));
+    c->h_tdev.resize(c->ndir);
+    c->h_tout.resize(c->ndir);
+    for (int d = 0; d < c->ndir; ++d) {
+        c->h_tdev[d] = make_tan_dev(c, d, mode == YC_GRAD_REF);
+        c->h_tout[d] = make_tan_out(c, d, mode);
+    }
+    HIPCHK(hipMemcpyAsync(c->tdev, c->h_tdev.data(), sizeof(ClothDev) * c->ndir, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tout, c->h_tout.data(), sizeof(AsmOut) * c->ndir, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_assemble1_dirs, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream, (const ClothDev*)c->tdev,
+                       (const AsmOut*)c->tout, h);
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+// Rest of direction d's tangent, after tangent_assemble_all (the solution is in xsol, the
+// right-hand side in bsave).  exact (mode 1): the derivative of the step as computed,
+// A x = b differentiated as A dx = db - dA x.  ref (mode 0): the reference's gradient: A,
+// M and K_v held constant, the non-jamming shear angle held constant, and the linear solve
+// transposed (A^T dx = db), so that the transpose of this tangent map is the reference's
+// backward solve with A (src/Solvers.cpp:51-53).
 int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int* iters) {
     const size_t PV = (size_t)5 * c->P;
     StateDev sd = make_state(c);
     TanState ts = tan_state(c, d);
-    hipLaunchKernelGGL(k_normals_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd, (const double*)ts.dx,
-                       tan_nrm(c, d));
-    ClothDev C = make_tan_dev(c, d, mode == YC_GRAD_REF);
-    AsmOut O = make_out(c);
-    O.dA = mode == YC_GRAD_EXACT ? c->dA : nullptr;
-    O.dF = c->dF; O.db = c->db; O.dis_slide = c->dis_slide; O.dcomp = c->dcomp;
-    O.detach = mode == YC_GRAD_REF;
-    O.b = c->bsave;   // the tangent kernels write db only; b is not touched
-    O.dext = c->tdext ? c->tdext + (size_t)d * c->n : nullptr;
-    hipLaunchKernelGGL(k_assemble1<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
+    const ClothDev& C = c->h_tdev[d];
+    const AsmOut& O = c->h_tout[d];
     // the shear segments' tangents, once per segment, into the shear records (the step's
     // own records are no longer needed: its assembly is done)
     hipLaunchKernelGGL(k_shear_elem_tan, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, C, (const double*)c->comp,
-                       (const double*)c->dcomp, c->shear_rec, 4 * c->Nc);
+                       (const double*)O.dcomp, c->shear_rec, 4 * c->Nc);
     hipLaunchKernelGGL(k_assemble2<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
     HIPCHK(hipGetLastError());
     if (mode == YC_GRAD_EXACT) {
-        hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->xsol, c->trhs, c->dA), (double*)nullptr,
+        hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->xsol, c->trhs, O.dA), (double*)nullptr,
                            (const CGScal*)c->sc, 0);
-        hipLaunchKernelGGL(k_axpy_neg, dim3(grid_for((long long)PV)), dim3(BLK), 0, c->stream, (const double*)c->db,
+        hipLaunchKernelGGL(k_axpy_neg, dim3(grid_for((long long)PV)), dim3(BLK), 0, c->stream, (const double*)O.db,
                            (const double*)c->trhs, c->b, (int)PV);
     } else {
-        HIPCHK(hipMemcpyAsync(c->b, c->db, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->b, O.db, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
     int rc = solve_on(c, mode == YC_GRAD_EXACT ? c->A : c->At, tol, maxit, iters);
     if (rc) return rc;
     hipLaunchKernelGGL(k_update_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd, ts, (const double*)c->xsol,
-                       (const double*)c->cx, (const double*)c->is_slide, (const double*)c->dis_slide, c->Wp, c->P, h);
+                       (const double*)c->cx, (const double*)c->is_slide, (const double*)O.dis_slide, c->Wp, c->P, h);
     HIPCHK(hipGetLastError());
     return YC_OK;
 }
@@ -2207,6 +2250,8 @@ void yc_destroy(yc_ctx* c) {
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
+    if (c->tdev) (void)hipFree(c->tdev);
+    if (c->tout) (void)hipFree(c->tout);
     if (c->io_event) (void)hipEventDestroy(c->io_event);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2548,6 +2593,10 @@ int yc_tangent_end(yc_ctx* c) {
     double** bufs[] = {&c->tstate, &c->dA, &c->dF, &c->db, &c->dis_slide, &c->dcomp, &c->At, &c->xsol, &c->bsave, &c->dsol, &c->trhs,
                        &c->tdext};
     for (double** p : bufs) { if (*p) (void)hipFree(*p); *p = nullptr; }
+    if (c->tdev) (void)hipFree(c->tdev);
+    if (c->tout) (void)hipFree(c->tout);
+    c->tdev = nullptr;
+    c->tout = nullptr;
     c->ndir = 0;
     c->tpar.clear();
     return YC_OK;
@@ -2561,8 +2610,12 @@ int yc_tangent_begin(yc_ctx* c, int ndir, const double* dparam, const double* ds
     int rc = YC_OK;
     auto chk = [&](int r) { if (r && !rc) rc = r; };
     chk(dalloc(&c->tstate, (size_t)ndir * TAN_PLANES * N));
-    chk(dalloc(&c->dA, c->a_size)); chk(dalloc(&c->At, c->a_size));
-    chk(dalloc(&c->dF, 5 * N)); chk(dalloc(&c->db, PV)); chk(dalloc(&c->dis_slide, 2 * N)); chk(dalloc(&c->dcomp, N));
+    // per-direction outputs of the tangent assembly (all directions assemble in one launch)
+    chk(dalloc(&c->dA, (size_t)ndir * c->a_size)); chk(dalloc(&c->At, c->a_size));
+    chk(dalloc(&c->dF, (size_t)ndir * 5 * N)); chk(dalloc(&c->db, (size_t)ndir * PV));
+    chk(dalloc(&c->dis_slide, (size_t)ndir * 2 * N)); chk(dalloc(&c->dcomp, (size_t)ndir * N));
+    if (!rc && hipMalloc((void**)&c->tdev, sizeof(ClothDev) * ndir) != hipSuccess) rc = fail(YC_ERR_HIP, "hipMalloc");
+    if (!rc && hipMalloc((void**)&c->tout, sizeof(AsmOut) * ndir) != hipSuccess) rc = fail(YC_ERR_HIP, "hipMalloc");
     chk(dalloc(&c->xsol, PV)); chk(dalloc(&c->bsave, PV)); chk(dalloc(&c->dsol, PV)); chk(dalloc(&c->trhs, PV));
     if (rc) { yc_tangent_end(c); return rc; }
     c->ndir = ndir;
@@ -2656,6 +2709,7 @@ int yc_step_tangent(yc_ctx* c, double h, int mode, double tol, int maxit, yc_ste
         hipLaunchKernelGGL(k_transpose, dim3(c->nT), dim3(NBLK), 0, c->stream, (const double*)c->A, c->At, c->W, c->Lg, c->ntx);
     }
     HIPCHK(hipGetLastError());
+    rc = tangent_assemble_all(c, mode, h);
     for (int d = 0; d < c->ndir && !rc; ++d) rc = tangent_dir(c, d, mode, h, tol, maxit, nullptr);
     // restore the step's own right-hand side and solution, then update
     HIPCHK(hipMemcpyAsync(c->b, c->bsave, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
