@@ -188,7 +188,6 @@ struct AsmOut {
     double* invdiag;    // padded 5-plane vector
     const double* ext;  // global-layout n vector or null
     const double* dext; // tangent run: the direction's tangent of ext (global layout) or null
-    double* bj;         // value run: block-Jacobi rows to compute from the final rows, or null
     int* flag;          // [0] structural error, [1] non-finite
     int Wp, P;          // padded width, plane stride
     // tangent run: tangents of A (exact mode, else null), F, b, is_slide and comp, and
@@ -339,18 +338,10 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
     if (S.bad) atomicOr(&O.flag[0], 1);
 }
 
-__device__ void bj_factor_tile(const ClothDev& C, const AsmOut& O, int tile);
-
 template <class T>
 __device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& O, double h, int tile) {
     int k = tile_node(C, tile);
     if (k >= 0) assemble2_node<T>(C, O, h, k);
-    if constexpr (std::is_same<T, double>::value) {
-        if (O.bj) {   // the tile's 2 x 2 groups from its final rows, still in LDS
-            __syncthreads();
-            bj_factor_tile(C, O, tile);
-        }
-    }
 }
 
 template <class T>
@@ -1275,7 +1266,7 @@ __device__ __forceinline__ void bj_invert_store(int W, int Lg, int Wp, int P, do
     }
 }
 
-// Standalone form, from A in HBM: three groups per 64-thread block.
+// From A (or A^T) in HBM: three groups per 64-thread block.
 __global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx,
                                                    int ngx, int ngroups) {
     __shared__ double colk[BJF_GROUPS][20];
@@ -1288,23 +1279,6 @@ __global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow,
     });
 }
 
-// Fused form, at the end of the second assembly kernel: the tile's 16 groups (groups never
-// straddle 8 x 8 tiles) from its final rows in LDS, three at a time.
-__device__ void bj_factor_tile(const ClothDev& C, const AsmOut& O, int tile) {
-    extern __shared__ double lds_rows[];   // slot s of node lane l at s * TILE_N + l
-    __shared__ double colk[BJF_GROUPS][20];
-    const int tu0 = (tile % C.ntx) * TILE, tv0 = (tile / C.ntx) * TILE;
-    const int gl = threadIdx.x / 20, j = threadIdx.x - 20 * gl;
-    constexpr int NG = (TILE / 2) * (TILE / 2);
-    for (int round = 0; round < (NG + BJF_GROUPS - 1) / BJF_GROUPS; ++round) {
-        const int g = round * BJF_GROUPS + gl;
-        const int gx = 2 * (g % (TILE / 2)), gy = 2 * (g / (TILE / 2));
-        const bool live = gl < BJF_GROUPS && g < NG && tu0 + gx < C.W && tv0 + gy < C.Lg;
-        bj_invert_store(C.W, C.Lg, O.Wp, O.P, O.bj, live, tu0 + gx, tv0 + gy, gl, j, colk, [&](int qi, int, int s) {
-            return lds_rows[(size_t)s * TILE_N + (gx + (qi & 1)) + (gy + (qi >> 1)) * TILE];
-        });
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Update (src/Cloth.cpp:111-148) and geometry refresh (src/Cloth.cpp:307-348)
@@ -1605,7 +1579,6 @@ AsmOut make_out(yc_ctx* c) {
     O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
     O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
     O.dext = nullptr;
-    O.bj = nullptr;
     O.detach = 0;
     O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
     O.bend_rec = c->bend_rec;
@@ -1749,7 +1722,7 @@ const void* cgp_kernel(int kp) {
 
 // The block-Jacobi rows of the matrix Aptr (k_bj_factor), unless they are already those
 // of Aptr: c->bj_of names the matrix the rows were computed from (the assembly computes
-// them for A; everything that rewrites A or At without them clears it).
+// them for the new A; everything else that rewrites A or At clears it).
 int bj_factor(yc_ctx* c, const double* Aptr) {
     if (c->bj_of == Aptr) return YC_OK;
     const int ngx = (c->W + 1) / 2, ngy = (c->Lg + 1) / 2, ng = ngx * ngy;
@@ -1853,14 +1826,13 @@ int do_assemble(yc_ctx* c, double h, bool async = false) {
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
                            c->shear_rec, 4 * c->Nc);
-    {
-        // with the block-Jacobi preconditioner the second kernel also computes its rows
-        AsmOut O2 = make_out(c);
-        O2.bj = c->precond == YC_PRECOND_BLOCK2 ? c->bj : nullptr;
-        hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, O2, h);
-        c->bj_of = O2.bj ? c->A : nullptr;
-    }
+    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
+    c->bj_of = nullptr;   // A is new
+    if (c->precond == YC_PRECOND_BLOCK2) {
+        int rc = bj_factor(c, c->A);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
                        c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
     if (async) {
@@ -2097,7 +2069,9 @@ int tangent_assemble_all(yc_ctx* c, int mode, double h) {
 // M and K_v held constant, the non-jamming shear angle held constant, and the linear solve
 // transposed (A^T dx = db), so that the transpose of this tangent map is the reference's
 // backward solve with A (src/Solvers.cpp:51-53).
-int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int* iters) {
+// queued: with the persistent CG the solve is only launched; its outcome accumulates in
+// the context's scalars and the caller reads it once for all directions.
+int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int* iters, bool queued = false) {
     const size_t PV = (size_t)5 * c->P;
     StateDev sd = make_state(c);
     TanState ts = tan_state(c, d);
@@ -2117,7 +2091,14 @@ int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int
     } else {
         HIPCHK(hipMemcpyAsync(c->b, O.db, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
-    int rc = solve_on(c, mode == YC_GRAD_EXACT ? c->A : c->At, tol, maxit, iters);
+    const double* Au = mode == YC_GRAD_EXACT ? c->A : c->At;
+    int rc = YC_OK;
+    if (queued) {
+        hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 0);
+        rc = launch_cgp(c, tol, maxit, Au);
+    } else {
+        rc = solve_on(c, Au, tol, maxit, iters);
+    }
     if (rc) return rc;
     hipLaunchKernelGGL(k_update_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd, ts, (const double*)c->xsol,
                        (const double*)c->cx, (const double*)c->is_slide, (const double*)O.dis_slide, c->Wp, c->P, h);
@@ -2710,7 +2691,25 @@ int yc_step_tangent(yc_ctx* c, double h, int mode, double tol, int maxit, yc_ste
     }
     HIPCHK(hipGetLastError());
     rc = tangent_assemble_all(c, mode, h);
-    for (int d = 0; d < c->ndir && !rc; ++d) rc = tangent_dir(c, d, mode, h, tol, maxit, nullptr);
+    // with the persistent CG the directions' solves are queued back to back and their
+    // outcomes read once (the solves' totals are reset first)
+    const bool queued = c->coop_ok != 0;
+    if (queued)
+        HIPCHK(hipMemsetAsync((char*)c->sc + offsetof(CGScal, total_iters), 0, sizeof(long long) + 2 * sizeof(int), c->stream));
+    for (int d = 0; d < c->ndir && !rc; ++d) rc = tangent_dir(c, d, mode, h, tol, maxit, nullptr, queued);
+    if (!rc && queued) {
+        HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_flag + 2, c->cg_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_flag[2]) {
+            (void)hipMemsetAsync(c->cg_err, 0, sizeof(int), c->stream);
+            rc = fail(YC_ERR_HIP, "persistent CG: a block waited past its spin limit");
+        } else if (c->h_sc->nonfinite) {
+            rc = fail(YC_ERR_NONFINITE, "tangent: NaN/Inf in the right-hand side or the solution");
+        } else if (c->h_sc->nonconv) {
+            rc = fail(YC_ERR_NOT_CONVERGED, "tangent CG stopped at maxit above tolerance");
+        }
+    }
     // restore the step's own right-hand side and solution, then update
     HIPCHK(hipMemcpyAsync(c->b, c->bsave, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->cx, c->xsol, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));

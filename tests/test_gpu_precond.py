@@ -190,8 +190,8 @@ def test_block_jacobi_needs_the_persistent_cg(ys, cs):
 @pytest.mark.parametrize("idx", [0, 2])
 def test_block_jacobi_rows_are_the_group_inverses(ys, cs, idx):
     """The preconditioner rows against numpy inverses of the assembled A's 2 x 2-group
-    blocks, for the rows the assembly computes and for the standalone kernel's (the same
-    bits)."""
+    blocks; the rows computed with the assembly and those recomputed once the context has
+    forgotten them are the same bits."""
     name, spec = _specs(cs)[idx]
     W, Lg = spec["width"], spec["length"]
     N = W * Lg
@@ -203,7 +203,7 @@ def test_block_jacobi_rows_are_the_group_inverses(ys, cs, idx):
     g.set_ClothState(r0.get_state())
     g.assemble(spec["h"])
     fused = g.precond_rows()
-    g.set_preconditioner("block2")       # forgets the rows: the next ones come from k_bj_factor
+    g.set_preconditioner("block2")       # forgets the rows: the export recomputes them
     alone = g.precond_rows()
     assert np.array_equal(fused, alone)
     A = g.csr().toarray()
