@@ -233,3 +233,28 @@ def test_block_jacobi_rows_are_the_group_inverses(ys, cs, idx):
     say("  %s: max |rows - inv| / max|inv| over groups %.2e" % (name, worst))
     assert worst < 1e-12
     g.close()
+
+
+def test_headline_configuration_trajectory(ys, cs):
+    """The benchmark's configuration -- twill128, persistent CG, block Jacobi, learned warm
+    start, tolerance 1e-8 -- for 6 steps against the oracle's steps (Jacobi CG at 1e-12):
+    the states agree to 1e-6 relative, and every warm-started solve meets the residual."""
+    import yarnstep.warmstart as ws
+    spec, state, ext, _ = cs.workload("twill128")
+    net = ws.WarmStartMLP.load(ws.WEIGHTS)
+    g = _block(ys, spec)
+    g.set_ClothState(state)
+    r = OracleCloth(spec)
+    r.set_state(state)
+    worst = 0.0
+    for s in range(6):
+        it, rr, rc = g.step_warm(net, spec["h"], tol=1e-8)
+        st_, it_r, _ = r.step(spec["h"], ext, tol=1e-12, maxit=20000)
+        assert rc == 0 and st_ == 0 and rr < 1e-8
+        sg, so = g.get_ClothState(), r.get_state()
+        e = max(rel(sg[k], so[k]) for k in ("LPos", "LVel", "EPos", "EVel"))
+        worst = max(worst, e)
+        say("  step %d: iterations %d (oracle Jacobi %d), state rel %.2e" % (s + 1, it, it_r, e))
+    net.close()
+    g.close()
+    assert worst < 1e-6
