@@ -293,6 +293,8 @@ def main():
             "plain_cg_steps_per_s": plain_steps_per_s, "plain_cg_ms_per_step": 1e3 * dt_plain / args.steps,
             "plain_cg_iters_per_step": it_plain / args.steps,
             "warm_over_plain": steps_per_s / plain_steps_per_s,
+            # against the reference's own method: zero-guess Jacobi-PCG (persistent form)
+            "warm_over_plain_jacobi": steps_per_s / jacobi["steps_per_s"] if jacobi else None,
             "plain_cg_reference_form": ref_form,
             "plain_cg_jacobi_persistent": jacobi,
             "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
