@@ -158,10 +158,13 @@ __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& 
     for (int j = 0; j < 3; ++j) S.LL(KIND_KX, k[r], k[j], B[j]);
 }
 
-// Segment s = 4 e + q of interior crossing e, q in the reference's shear_segs order
-// (left,bot), (left,top), (right,bot), (right,top) (src/Cloth.cpp:170-181): its nodes.
+// Segment s = q Nc + e: segment q of interior crossing e, q in the reference's shear_segs
+// order (left,bot), (left,top), (right,bot), (right,top) (src/Cloth.cpp:170-181), so that
+// neighbouring crossings' records of one kind are adjacent (the node kernels' wavefronts
+// read them for neighbouring nodes): its nodes.
+__device__ __forceinline__ int shear_seg(const ClothDev& C, int e, int q) { return q * C.Nc + e; }
 __device__ __forceinline__ void shear_nodes(const ClothDev& C, int s, int (&k)[3]) {
-    int e = s >> 2, q = s & 3;
+    int q = s / C.Nc, e = s - q * C.Nc;
     int cu = 1 + e % (C.W - 2), cv = 1 + e / (C.W - 2);
     int kc = cu + cv * C.W;
     k[0] = kc;

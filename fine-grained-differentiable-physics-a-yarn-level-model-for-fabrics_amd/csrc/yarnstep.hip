@@ -256,6 +256,7 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble1_dirs(const ClothDev* __re
 // rows.
 template <class T, class Sink>
 __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& F, const mat3<T> (&B)[3]);
+__device__ __forceinline__ int shear_seg(const ClothDev& C, int e, int q);
 __device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
 
 template <class T>
@@ -287,7 +288,7 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
             // replay the segment's record for this node's role: its values (value run) or
             // their tangents (tangent run, k_shear_elem_tan; the tangent outputs take only
             // the tangent parts of the shear's forces and K_x blocks, so the values are 0)
-            int sidx = 4 * crossing(C, cu, cv) + 2 * a + bq;
+            int sidx = shear_seg(C, crossing(C, cu, cv), 2 * a + bq);
             int r = kc == k ? 0 : (k1 == k ? 1 : 2);
             const int kk[3] = {kc, k1, k3};
             d3 F;
