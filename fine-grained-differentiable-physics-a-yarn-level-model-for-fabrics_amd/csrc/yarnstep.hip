@@ -267,9 +267,17 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
     int u = S.mu_, v = S.mv_;
     extern __shared__ double lds_rows[];
     double* gA = S.A;
-    if (!tan) {   // the node's slots into LDS, back to HBM at the end
+    // the shear writes only the L-L blocks of the 3 x 3 node neighbourhood (offsets 0-4 and
+    // 9-12 of the stencil: 81 slots); only those go through LDS
+    constexpr int SHEAR_O[9] = {0, 1, 2, 3, 4, 9, 10, 11, 12};
+    if (!tan) {   // those slots into LDS, back to HBM at the end
         S.A = lds_rows + threadIdx.x;
-        for (int s = 0; s < NSLOT; ++s) S.A[(size_t)s * TILE_N] = gA[(size_t)s * TILE_N];
+        for (int rc = 0; rc < 3; ++rc)
+            for (int oi = 0; oi < 9; ++oi)
+                for (int cc = 0; cc < 3; ++cc) {
+                    const size_t sl = (size_t)(rc * 45 + SHEAR_O[oi] * 3 + cc) * TILE_N;
+                    S.A[sl] = gA[sl];
+                }
     }
     auto K = [&](int uu, int vv) { return uu + vv * C.W; };
     // shear_segs order per crossing: (left,bot), (left,top), (right,bot), (right,top)
@@ -324,17 +332,20 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
     for (int c = 0; c < 5; ++c) {
         double d = 0.0;
         if (c < 3) d = S.A[(size_t)(c * 45 + c) * TILE_N];
-        else if (S.inter) d = S.A[(size_t)((c == 3 ? SLOT_EU : SLOT_EV) + 10) * TILE_N];
+        else if (S.inter) d = gA[(size_t)((c == 3 ? SLOT_EU : SLOT_EV) + 10) * TILE_N];   // untouched by the shear
         double inv = 0.0;
         if (c < 3 || S.inter) inv = d != 0.0 ? 1.0 / d : 1.0;
         O.invdiag[c * O.P + pk] = inv;
         if (!isfinite(O.b[c * O.P + pk])) bad = true;
     }
-    for (int s = 0; s < NSLOT; ++s) {
-        double a = S.A[(size_t)s * TILE_N];
-        if (!isfinite(a)) bad = true;
-        gA[(size_t)s * TILE_N] = a;
-    }
+    for (int rc = 0; rc < 3; ++rc)
+        for (int oi = 0; oi < 9; ++oi)
+            for (int cc = 0; cc < 3; ++cc) {
+                const size_t sl = (size_t)(rc * 45 + SHEAR_O[oi] * 3 + cc) * TILE_N;
+                const double a = S.A[sl];
+                if (!isfinite(a)) bad = true;
+                gA[sl] = a;
+            }
     if (bad) atomicOr(&O.flag[1], 1);
     if (S.bad) atomicOr(&O.flag[0], 1);
 }
