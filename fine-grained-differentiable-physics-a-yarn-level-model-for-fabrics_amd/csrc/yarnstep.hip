@@ -385,8 +385,51 @@ struct SpmvArgs {
 // *y0, *y1.  X(c, du, dv) returns entry c of the input vector at the node offset by
 // (du, dv).  S(i) returns slot i of the rows: slot g*45 + i for an L row, slot
 // SLOT_EU + i (i < 24: the E_u row, then the E_v row) for the E rows.
-template <class S, class X>
+// FUSED: fused multiply-adds in three (L) or two (E) independent chains, for the latency of
+// the persistent solve's product; otherwise one chain of separate products and sums in slot
+// order (the reference form, whose iteration counts match the oracle's exactly).
+template <bool FUSED = false, class S, class X>
 __device__ __forceinline__ void stencil_rows(int g, S slot, X xin, double* y0, double* y1) {
+    if constexpr (FUSED) {
+        if (g < 3) {
+            const int odu[13] = {0, -1, 1, 0, 0, -2, 2, 0, 0, -1, 1, -1, 1};
+            const int odv[13] = {0, 0, 0, -1, 1, 0, 0, -2, 2, -1, -1, 1, 1};
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+            for (int o = 0; o < 13; ++o) {
+                a0 = __builtin_fma(slot(o * 3 + 0), xin(0, odu[o], odv[o]), a0);
+                a1 = __builtin_fma(slot(o * 3 + 1), xin(1, odu[o], odv[o]), a1);
+                a2 = __builtin_fma(slot(o * 3 + 2), xin(2, odu[o], odv[o]), a2);
+            }
+            a0 = __builtin_fma(slot(SLOT_LE + 0), xin(3, -1, 0), a0);
+            a1 = __builtin_fma(slot(SLOT_LE + 1), xin(3, 0, 0), a1);
+            a2 = __builtin_fma(slot(SLOT_LE + 2), xin(3, 1, 0), a2);
+            a0 = __builtin_fma(slot(SLOT_LE + 3), xin(4, 0, -1), a0);
+            a1 = __builtin_fma(slot(SLOT_LE + 4), xin(4, 0, 0), a1);
+            a2 = __builtin_fma(slot(SLOT_LE + 5), xin(4, 0, 1), a2);
+            *y0 = (a0 + a1) + a2;
+        } else {
+            double au0 = 0.0, au1 = 0.0, av0 = 0.0, av1 = 0.0;
+#pragma unroll
+            for (int d = -1; d <= 1; ++d) {
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    if (cc & 1) {
+                        au1 = __builtin_fma(slot((d + 1) * 3 + cc), xin(cc, d, 0), au1);
+                        av1 = __builtin_fma(slot(12 + (d + 1) * 3 + cc), xin(cc, 0, d), av1);
+                    } else {
+                        au0 = __builtin_fma(slot((d + 1) * 3 + cc), xin(cc, d, 0), au0);
+                        av0 = __builtin_fma(slot(12 + (d + 1) * 3 + cc), xin(cc, 0, d), av0);
+                    }
+                }
+                au1 = __builtin_fma(slot(9 + d + 1), xin(3, d, 0), au1);
+                av1 = __builtin_fma(slot(12 + 9 + d + 1), xin(4, 0, d), av1);
+            }
+            *y0 = au0 + au1;
+            *y1 = av0 + av1;
+        }
+        return;
+    }
     if (g < 3) {
         const int odu[13] = {0, -1, 1, 0, 0, -2, 2, 0, 0, -1, 1, -1, 1};
         const int odv[13] = {0, 0, 0, -1, 1, 0, 0, -2, 2, -1, -1, 1, 1};
@@ -1116,13 +1159,17 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
 #pragma unroll
                 for (int cc = 0; cc < 5; ++cc)
                     in[qn * 5 + cc] = sr[cc][bj_ln0 + (qn & 1) + (qn >> 1) * HALO_W];
+            // three independent fused multiply-add chains (the build contracts nothing by
+            // itself; a preconditioner needs no particular rounding, only the same one for
+            // every block that computes an entry, which one code path gives)
+            double uu[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int o = 0; o < 3; ++o) {
-                double uu = 0.0;
+            for (int m = 0; m < 20; ++m)
 #pragma unroll
-                for (int m = 0; m < 20; ++m) uu += brow[o][m] * in[m];
-                if (bj_ok[o]) su[bj_c[o]][bj_ln[o]] = uu;
-            }
+                for (int o = 0; o < 3; ++o) uu[o] = __builtin_fma(brow[o][m], in[m], uu[o]);
+#pragma unroll
+            for (int o = 0; o < 3; ++o)
+                if (bj_ok[o]) su[bj_c[o]][bj_ln[o]] = uu[o];
         }
 #pragma unroll
         for (int t = 0; t < CG1_NE; ++t) {
@@ -1170,7 +1217,7 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
         if (node_in) {
             auto xin = [&](int c, int du, int dv) { return su[c][l0 + du + dv * HALO_W]; };
             auto slotv = [&](int i) { return Ar[i]; };
-            stencil_rows(g, slotv, xin, &y0, &y1);
+            stencil_rows<true>(g, slotv, xin, &y0, &y1);
             if (g < 3) {
                 acc[1] += y0 * su[g][l0];
                 acc[3] += y0 * sp[g][lane];
