@@ -194,17 +194,22 @@ __global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* co
     shear_elem_seg(C, comp, rec, nseg, blockIdx.x * 256 + threadIdx.x);
 }
 
-// Tangent run: the tangents of the same 30 doubles per role, one thread per segment,
-// along the direction held in C (its state and parameter tangents and the compression
-// tangents dcomp), in the value records' layout.
-__global__ void __launch_bounds__(256) k_shear_elem_tan(ClothDev C, const double* comp, const double* dcomp, double* rec,
-                                                        int nseg) {
+// Tangent run: the tangents of the same 30 doubles per role, one thread per segment, along
+// the direction of each block's descriptors (blockIdx.y: the direction; its state and
+// parameter tangents in C, its compression tangents O.dcomp, its records O.shear_rec), in the
+// value records' layout.
+__global__ void __launch_bounds__(256) k_shear_elem_tan(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
+                                                        const double* comp) {
+    const ClothDev C = Cs[blockIdx.y];
+    const AsmOut O = Os[blockIdx.y];
+    const int nseg = O.nseg;
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= nseg) return;
     int k[3];
     shear_nodes(C, s, k);
     ShearOut<dd> o;
-    shear_compute<dd>(C, k[0], k[1], k[2], comp, dcomp, o);
+    shear_compute<dd>(C, k[0], k[1], k[2], comp, O.dcomp, o);
+    double* rec = const_cast<double*>(O.shear_rec);   // the direction's own buffer (the readers take it const)
     for (int r = 0; r < 3; ++r) {
         double* base = rec + (size_t)r * 30 * nseg + s;
         base[0] = o.F[r].x.d; base[(size_t)nseg] = o.F[r].y.d; base[(size_t)2 * nseg] = o.F[r].z.d;

@@ -361,6 +361,14 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, dou
     assemble2_tile<T>(C, O, h, blockIdx.x);
 }
 
+// The second kernel's tangent run for every direction in one launch (blockIdx.y).
+__global__ void __launch_bounds__(NBLK, 1) k_assemble2_dirs(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
+                                                           double h) {
+    const ClothDev C = Cs[blockIdx.y];
+    const AsmOut O = Os[blockIdx.y];
+    assemble2_tile<dd>(C, O, h, blockIdx.x);
+}
+
 }  // namespace
 
 #include "shear.hpp"
@@ -1598,6 +1606,7 @@ struct yc_ctx {
     double *dA, *dF, *db, *dis_slide, *dcomp, *At, *xsol, *bsave, *dsol, *trhs;
     double* tdext;                          // ndir x n tangents of the external force, or null
     ClothDev* tdev;                         // ndir tangent descriptors (device), for k_assemble1_dirs
+    double* tshear;                         // ndir x 90 x 4Nc shear-tangent records
     AsmOut* tout;                           // ndir tangent outputs (device)
     std::vector<ClothDev> h_tdev;           // their host copies (kept alive for the upload)
     std::vector<AsmOut> h_tout;
@@ -2098,6 +2107,7 @@ AsmOut make_tan_out(yc_ctx* c, int d, int mode) {
     O.detach = mode == YC_GRAD_REF;
     O.b = c->bsave;   // the tangent kernels write db only; b is not touched
     O.dext = c->tdext ? c->tdext + (size_t)d * c->n : nullptr;
+    O.shear_rec = c->tshear + (size_t)d * 90 * O.nseg;   // the direction's shear-tangent records
     return O;
 }
 
@@ -2118,6 +2128,11 @@ int tangent_assemble_all(yc_ctx* c, int mode, double h) {
     HIPCHK(hipMemcpyAsync(c->tout, c->h_tout.data(), sizeof(AsmOut) * c->ndir, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_assemble1_dirs, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream, (const ClothDev*)c->tdev,
                        (const AsmOut*)c->tout, h);
+    // the shear segments' tangents, once per segment and direction, then the second kernel
+    hipLaunchKernelGGL(k_shear_elem_tan, dim3((4 * c->Nc + 255) / 256, c->ndir), dim3(256), 0, c->stream,
+                       (const ClothDev*)c->tdev, (const AsmOut*)c->tout, (const double*)c->comp);
+    hipLaunchKernelGGL(k_assemble2_dirs, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream, (const ClothDev*)c->tdev,
+                       (const AsmOut*)c->tout, h);
     HIPCHK(hipGetLastError());
     return YC_OK;
 }
@@ -2134,14 +2149,7 @@ int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int
     const size_t PV = (size_t)5 * c->P;
     StateDev sd = make_state(c);
     TanState ts = tan_state(c, d);
-    const ClothDev& C = c->h_tdev[d];
     const AsmOut& O = c->h_tout[d];
-    // the shear segments' tangents, once per segment, into the shear records (the step's
-    // own records are no longer needed: its assembly is done)
-    hipLaunchKernelGGL(k_shear_elem_tan, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, C, (const double*)c->comp,
-                       (const double*)O.dcomp, c->shear_rec, 4 * c->Nc);
-    hipLaunchKernelGGL(k_assemble2<dd>, dim3(c->nT), dim3(NBLK), 0, c->stream, C, O, h);
-    HIPCHK(hipGetLastError());
     if (mode == YC_GRAD_EXACT) {
         hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->xsol, c->trhs, O.dA), (double*)nullptr,
                            (const CGScal*)c->sc, 0);
@@ -2285,7 +2293,7 @@ void yc_destroy(yc_ctx* c) {
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
                     c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
-                    c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs, c->tdext,
+                    c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs, c->tdext, c->tshear,
                     c->shear_rec, c->bend_rec, c->bepoch, c->bj};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
@@ -2631,7 +2639,7 @@ int yc_tangent_end(yc_ctx* c) {
     if (!c) return fail(YC_ERR_ARG, "null context");
     (void)hipStreamSynchronize(c->stream);
     double** bufs[] = {&c->tstate, &c->dA, &c->dF, &c->db, &c->dis_slide, &c->dcomp, &c->At, &c->xsol, &c->bsave, &c->dsol, &c->trhs,
-                       &c->tdext};
+                       &c->tdext, &c->tshear};
     for (double** p : bufs) { if (*p) (void)hipFree(*p); *p = nullptr; }
     if (c->tdev) (void)hipFree(c->tdev);
     if (c->tout) (void)hipFree(c->tout);
@@ -2655,6 +2663,7 @@ int yc_tangent_begin(yc_ctx* c, int ndir, const double* dparam, const double* ds
     chk(dalloc(&c->dF, (size_t)ndir * 5 * N)); chk(dalloc(&c->db, (size_t)ndir * PV));
     chk(dalloc(&c->dis_slide, (size_t)ndir * 2 * N)); chk(dalloc(&c->dcomp, (size_t)ndir * N));
     if (!rc && hipMalloc((void**)&c->tdev, sizeof(ClothDev) * ndir) != hipSuccess) rc = fail(YC_ERR_HIP, "hipMalloc");
+    chk(dalloc(&c->tshear, (size_t)ndir * 90 * 4 * c->Nc + 1));
     if (!rc && hipMalloc((void**)&c->tout, sizeof(AsmOut) * ndir) != hipSuccess) rc = fail(YC_ERR_HIP, "hipMalloc");
     chk(dalloc(&c->xsol, PV)); chk(dalloc(&c->bsave, PV)); chk(dalloc(&c->dsol, PV)); chk(dalloc(&c->trhs, PV));
     if (rc) { yc_tangent_end(c); return rc; }
