@@ -1463,7 +1463,10 @@ __global__ void __launch_bounds__(NBLK) k_normals(StateDev s) { normals_node(s, 
 // dn = sum_{j != min} e_j (e_j^T dC e_min) / (lam_min - lam_j), signed as n (the sign
 // is locally constant).  This is the derivative the reference's autograd takes through
 // torch::svd (src/Cloth.cpp:446) when the singular values are distinct.
-__global__ void __launch_bounds__(NBLK) k_normals_tan(StateDev s, const double* dx, double* dnrm) {
+__global__ void __launch_bounds__(NBLK) k_normals_tan(StateDev s, const double* dx, double* dnrm, size_t dstride) {
+    // direction blockIdx.y: its tangents dstride doubles after the previous direction's
+    dx += (size_t)blockIdx.y * dstride;
+    dnrm += (size_t)blockIdx.y * dstride;
     int k = blockIdx.x * NBLK + threadIdx.x;
     if (k >= s.N) return;
     int u = k % s.W, v = k / s.W;
@@ -2115,9 +2118,8 @@ AsmOut make_tan_out(yc_ctx* c, int d, int mode) {
 // kernel of all directions in one launch.
 int tangent_assemble_all(yc_ctx* c, int mode, double h) {
     StateDev sd = make_state(c);
-    for (int d = 0; d < c->ndir; ++d)
-        hipLaunchKernelGGL(k_normals_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd,
-                           (const double*)tan_state(c, d).dx, tan_nrm(c, d));
+    hipLaunchKernelGGL(k_normals_tan, dim3((c->N + NBLK - 1) / NBLK, c->ndir), dim3(NBLK), 0, c->stream, sd,
+                       (const double*)tan_state(c, 0).dx, tan_nrm(c, 0), (size_t)TAN_PLANES * c->N);
     c->h_tdev.resize(c->ndir);
     c->h_tout.resize(c->ndir);
     for (int d = 0; d < c->ndir; ++d) {
