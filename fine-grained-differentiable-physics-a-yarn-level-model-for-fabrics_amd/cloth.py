@@ -23,6 +23,21 @@ def _arr(a, shape):
     return a
 
 
+def configure_solver(cloth, cg="persistent", precond="block2"):
+    """Selects the CG form and preconditioner for a rollout, falling back to the fused form
+    with the Jacobi preconditioner where the persistent one does not fit the device."""
+    try:
+        cloth.set_cg_variant(cg)
+    except _abi.YcError:
+        cloth.set_cg_variant("fused")
+        return
+    if cg == "persistent" and precond != "jacobi":
+        try:
+            cloth.set_preconditioner(precond)
+        except _abi.YcError:
+            pass
+
+
 class Cloth:
     """One cloth on the current HIP device (reference Cloth + PhysicsCloth)."""
 

@@ -21,7 +21,7 @@ computed ("exact"), as for the material parameters.
 import numpy as np
 
 from . import _abi
-from .cloth import Cloth
+from .cloth import Cloth, configure_solver
 
 PHASES = 6
 ZERO_FORCE = PHASES                    # force index of the zero external force
@@ -130,7 +130,7 @@ def mass_centre_loss(lpos, goal_lpos):
 
 
 def rollout_loss_and_grad(net, init_state, goal_lpos, batch=1, sample=8, mode="exact", tol=1e-12, maxit=None,
-                          force_stop_frame=FORCE_STOP_FRAME):
+                          force_stop_frame=FORCE_STOP_FRAME, cg="persistent", precond="block2"):
     """The driver's loss for the current strengths and its gradient with respect to the
     twelve of them (src/TrainControl.h:120-201).  A failed step ends its sample, as the
     reference's break does, and the next sample goes on from the last good state.
@@ -138,6 +138,7 @@ def rollout_loss_and_grad(net, init_state, goal_lpos, batch=1, sample=8, mode="e
     c = Cloth(net.spec)
     ok = 0
     try:
+        configure_solver(c, cg, precond)
         c.set_ClothState(init_state)
         c.tangent_begin(np.zeros((2 * PHASES, len(Cloth.PARAMS))))
         for _ in range(batch):

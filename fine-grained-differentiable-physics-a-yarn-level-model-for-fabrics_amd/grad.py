@@ -17,7 +17,7 @@ import copy
 
 import numpy as np
 
-from .cloth import Cloth
+from .cloth import Cloth, configure_solver
 
 PARAMS = Cloth.PARAMS
 
@@ -39,7 +39,8 @@ def with_params(spec, names, values):
     return s
 
 
-def rollout_jacobian(spec, state, steps, names, mode="ref", tol=1e-10, maxit=None, ext_force=None, cg="persistent"):
+def rollout_jacobian(spec, state, steps, names, mode="ref", tol=1e-10, maxit=None, ext_force=None, cg="persistent",
+                     precond="block2"):
     """Runs `steps` SimSteps from `state` and returns (positions, jac): positions
     (steps, N, 3) after each step, jac (len(names), steps, N, 3) their derivatives with
     respect to the named parameters."""
@@ -47,10 +48,7 @@ def rollout_jacobian(spec, state, steps, names, mode="ref", tol=1e-10, maxit=Non
         raise ValueError("a one-yarn cloth has no second yarn parameters")
     c = Cloth(spec)
     try:
-        try:
-            c.set_cg_variant(cg)
-        except Exception:
-            c.set_cg_variant("fused")
+        configure_solver(c, cg, precond)
         c.set_ClothState(state)
         c.set_ext_force(ext_force)
         dp = np.zeros((len(names), len(PARAMS)))

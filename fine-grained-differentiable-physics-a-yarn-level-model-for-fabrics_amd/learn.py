@@ -20,7 +20,7 @@ import math
 
 import numpy as np
 
-from .cloth import Cloth
+from .cloth import Cloth, configure_solver
 from .grad import get_param, with_params
 
 # reference parameter names -> this build's (yarn 1/2 of the reference are yarns 0/1)
@@ -73,7 +73,7 @@ def trajectory(spec, state, steps, tol=1e-10, cg="persistent"):
     return {k: np.stack([f[k] for f in frames]) for k in frames[0]}
 
 
-def sample_loss_terms(spec, frames, names, mode="ref", tol=1e-10, cg="persistent"):
+def sample_loss_terms(spec, frames, names, mode="ref", tol=1e-10, cg="persistent", precond="block2"):
     """Rollout from frames[0] along the stored trajectory.  Returns (loss_sum, grad_sum,
     steps_ok): the summed per-step position losses, their derivatives with respect to
     the named (constrained) parameters, and the number of steps that succeeded; the
@@ -84,10 +84,7 @@ def sample_loss_terms(spec, frames, names, mode="ref", tol=1e-10, cg="persistent
     grad = np.zeros(len(names))
     ok = 0
     try:
-        try:
-            c.set_cg_variant(cg)
-        except Exception:
-            c.set_cg_variant("fused")
+        configure_solver(c, cg, precond)
         c.set_ClothState({k: frames[k][0] for k in frames})
         dp = np.zeros((len(names), len(Cloth.PARAMS)))
         for i, n in enumerate(names):
