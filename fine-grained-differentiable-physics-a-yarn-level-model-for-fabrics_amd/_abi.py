@@ -82,7 +82,7 @@ def load_library():
     # failed to find the device in torch after many contexts had come and gone.
     try:
         import torch  # noqa: F401
-    except ImportError:
+    except Exception:   # absent or unusable: the library loads its own runtime
         pass
     L = ctypes.CDLL(p)
     dp = ctypes.POINTER(ctypes.c_double)
