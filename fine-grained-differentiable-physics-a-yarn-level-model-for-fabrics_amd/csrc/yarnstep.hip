@@ -2053,10 +2053,11 @@ ClothDev make_tan_dev(yc_ctx* c, int d, int ref) {
 
 // Solve A_use x = c->b into c->cx without hipGraphs (the graphs hold c->A), through the
 // persistent CG when it fits, else the reference-order kernels.
-int solve_on(yc_ctx* c, const double* Aptr, double tol, int maxit, int* iters) {
+// x0: start from c->cx, whose product A x0 is in c->cy (else from zero).
+int solve_on(yc_ctx* c, const double* Aptr, double tol, int maxit, int* iters, bool x0 = false) {
     if (c->precond != YC_PRECOND_JACOBI && !c->coop_ok)
         return fail(YC_ERR_ARG, "the block-Jacobi preconditioner needs the persistent CG");
-    hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 0);
+    hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, x0 ? 1 : 0);
     if (c->coop_ok) {
         int rc = launch_cgp(c, tol, maxit, Aptr);
         if (rc) return rc;
@@ -2161,12 +2162,18 @@ int tangent_dir(yc_ctx* c, int d, int mode, double h, double tol, int maxit, int
         HIPCHK(hipMemcpyAsync(c->b, O.db, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
     const double* Au = mode == YC_GRAD_EXACT ? c->A : c->At;
+    // the direction's velocity tangent of the previous step as the initial guess (the
+    // primal step's warm start from its previous velocity, for the tangent)
+    hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)ts.dv, (const double*)ts.duvd,
+                       c->W, c->Lg, c->N, c->Wp, c->P, c->cx);
+    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->cx, c->cy, Au), (double*)nullptr,
+                       (const CGScal*)c->sc, 0);
     int rc = YC_OK;
     if (queued) {
-        hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 0);
+        hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 1);
         rc = launch_cgp(c, tol, maxit, Au);
     } else {
-        rc = solve_on(c, Au, tol, maxit, iters);
+        rc = solve_on(c, Au, tol, maxit, iters, true);
     }
     if (rc) return rc;
     hipLaunchKernelGGL(k_update_tan, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, sd, ts, (const double*)c->xsol,
