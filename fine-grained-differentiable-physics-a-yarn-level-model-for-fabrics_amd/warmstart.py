@@ -3,7 +3,7 @@ training with the library's BF16-MFMA kernels, and the network wrapper.
 
 The network (32-128-128-128-5, SiLU) maps per-node features to the velocity change
 of the implicit step; CG starts from v + dv.  Plain zero-guess CG (Cloth.step)
-remains the fallback and the judge: the warm-started solve stops at the same
+remains the fallback and the yardstick: the warm-started solve stops at the same
 relative residual.
 """
 import ctypes

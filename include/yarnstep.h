@@ -196,7 +196,7 @@ int yc_time_assemble(yc_ctx* ctx, double h, int reps, double* avg_ms);
  * src/ForceNet.h, predicts control forces): a 32-128-128-128-5 MLP with BF16 weights
  * and FP32 accumulation on MFMA predicts the velocity change of the implicit step
  * per node; CG then starts from v + dv and refines to the same tolerance as from a
- * zero guess (yc_step stays the zero-guess fallback and judge).
+ * zero guess (yc_step stays the zero-guess fallback and yardstick).
  * Parameters: one flat FP32 array of yc_mlp_param_count() values in the order
  * W1[128x32] b1[128] W2[128x128] b2[128] W3[128x128] b3[128] W4[5x128] b4[5]
  * (row-major [out][in]).  Inputs are standardised with (x - in_mean) / in_std; the
