@@ -258,3 +258,31 @@ def test_headline_configuration_trajectory(ys, cs):
     net.close()
     g.close()
     assert worst < 1e-6
+
+
+def test_headline_configuration_100_steps(ys, cs):
+    """100 steps of the benchmark's code path (twill128, persistent CG, block Jacobi,
+    learned warm start) against the oracle's steps (about a minute of oracle time): the
+    trajectory stays within 1e-6 relative.  Both solve to 1e-12: this cloth amplifies
+    solve-tolerance differences, and the oracle against itself at 1e-8 and at 1e-12 is
+    already 1.9e-6 apart in the velocities after 100 steps."""
+    import yarnstep.warmstart as ws
+    spec, state, ext, _ = cs.workload("twill128")
+    net = ws.WarmStartMLP.load(ws.WEIGHTS)
+    g = _block(ys, spec)
+    g.set_ClothState(state)
+    r = OracleCloth(spec)
+    r.set_state(state)
+    worst = 0.0
+    for s in range(100):
+        it, rr, rc = g.step_warm(net, spec["h"], tol=1e-12, maxit=20000)
+        st_, _, _ = r.step(spec["h"], ext, tol=1e-12, maxit=20000)
+        assert rc == 0 and st_ == 0
+        if s % 10 == 9:
+            sg, so = g.get_ClothState(), r.get_state()
+            e = max(rel(sg[k], so[k]) for k in ("LPos", "LVel", "EPos", "EVel"))
+            worst = max(worst, e)
+            say("  step %d: state rel %.2e (warm-started iterations %d)" % (s + 1, e, it))
+    net.close()
+    g.close()
+    assert worst < 1e-6
