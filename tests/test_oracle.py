@@ -14,6 +14,8 @@ reference's own data fix, independently of the HIP product.
 - normals against an SVD of the centred five-point matrix (LAPACK via numpy)
 - CG against a dense solve and a textbook Jacobi-PCG restatement
 - the reference's shipped 5x5 trajectory (TrainingData/Table_2), loosely
+- the non-jamming shear force against the gradient of its energy, and its
+  self blocks against central differences
 """
 import copy
 import os
@@ -139,6 +141,52 @@ def _fd(spec, state, which, eps, cols=None):
         s2[key][k, a] -= eps
         Kfd[:, j] = (_forces(spec, s1).vec("F") - _forces(spec, s2).vec("F")) / (2 * eps)
     return K, Kfd, N
+
+
+def test_shear_non_jamming_force_and_outer_blocks(cs):
+    """Shear alone (no stretch or bending, so the compression, and with it the shear
+    stiffness kx = S pi R^2, is constant) on a 3 x 3 plain cloth: its one crossing has
+    four non-jamming shear elements (the crossing and its four neighbours have equal
+    weave flips, src/ClothUnit.h:226-230).  Their force is minus the gradient of
+    E = sum 1/2 kx L (phi - pi/2)^2, phi the angle between the two yarn segments at the
+    crossing, taken here by central differences of E written from that definition.  The
+    Jacobian blocks of each outer node with itself are exact derivatives of the force;
+    the blocks that couple two nodes are the reference's hand-derived values
+    (src/PhysicsCloth.cpp:3056-3088) and differ from the derivative."""
+    W = 3
+    s = _strip(cs, W, Y=0.0, B=0.0, G=[0.0, 0.0, 0.0], handle_k=0.0)
+    st = excited_state(cs, s, 7, pos_jit=2e-4, vel=0.0, evel=0.0)
+    c = _forces(s, st)
+    assert np.all(c.crossing()[1] == 0.0)
+    kx = s["S"] * np.pi * s["R"] ** 2
+    centre, left, right, top, bottom = 4, 3, 5, 1, 7
+
+    def energy(X):
+        e = 0.0
+        for a, b in ((left, bottom), (left, top), (right, bottom), (right, top)):
+            d1, d3 = X[a] - X[centre], X[b] - X[centre]
+            phi = np.arccos(d1 @ d3 / (np.linalg.norm(d1) * np.linalg.norm(d3)))
+            e += 0.5 * kx * s["L"] * (phi - np.pi / 2) ** 2
+        return e
+
+    X = st["LPos"]
+    grad = np.zeros_like(X)
+    for i in range(W * W):
+        for a in range(3):
+            Xp, Xm = X.copy(), X.copy()
+            Xp[i, a] += 1e-9
+            Xm[i, a] -= 1e-9
+            grad[i, a] = (energy(Xp) - energy(Xm)) / 2e-9
+    F = c.vec("F")
+    assert np.linalg.norm(grad) > 1e-4
+    assert np.linalg.norm(F[: 3 * W * W].reshape(-1, 3) + grad) < 1e-6 * np.linalg.norm(grad)
+    assert np.all(F[3 * W * W:] == 0.0)
+    K, Kfd, N = _fd(s, st, "x", 1e-9)
+    for k in (left, right, top, bottom):
+        blk = slice(3 * k, 3 * k + 3)
+        assert np.linalg.norm(K[blk, blk] - Kfd[blk, blk]) < 1e-6 * np.linalg.norm(Kfd[blk, blk])
+    blk1, blk3 = slice(3 * left, 3 * left + 3), slice(3 * bottom, 3 * bottom + 3)
+    assert np.linalg.norm(K[blk1, blk3] - Kfd[blk1, blk3]) > 0.5 * np.linalg.norm(Kfd[blk1, blk3])
 
 
 def _masked_rel(K, Kfd, rows, cols):
