@@ -148,7 +148,7 @@ def rollout_loss_and_grad(net, init_state, goal_lpos, batch=1, sample=8, mode="e
                 c.tangent_set_ext(net.force_tangents(idx))
                 try:
                     c.step_tangent(net.h, mode, tol=tol, maxit=maxit)
-                except Exception:
+                except (_abi.NonFinite, _abi.NotConverged):
                     break
                 ok += 1
         lpos = c.get_ClothState()["LPos"]

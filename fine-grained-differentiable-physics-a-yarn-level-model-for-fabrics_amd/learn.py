@@ -20,6 +20,7 @@ import math
 
 import numpy as np
 
+from . import _abi
 from .cloth import Cloth, configure_solver
 from .grad import get_param, with_params
 
@@ -59,7 +60,7 @@ def trajectory(spec, state, steps, tol=1e-10, cg="persistent"):
     try:
         try:
             c.set_cg_variant(cg)
-        except Exception:
+        except _abi.YcError:   # the persistent form does not fit this cloth
             c.set_cg_variant("fused")
         c.set_ClothState(state)
         frames = [c.get_ClothState()]
@@ -94,7 +95,7 @@ def sample_loss_terms(spec, frames, names, mode="ref", tol=1e-10, cg="persistent
         for t in range(nsteps):
             try:
                 c.step_tangent(spec["h"], mode, tol=tol)
-            except Exception:
+            except (_abi.NonFinite, _abi.NotConverged):
                 break
             s = c.get_ClothState()
             dL = s["LPos"] - frames["LPos"][t + 1]

@@ -60,7 +60,8 @@ class WarmStartMLP:
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32)
-        assert p.size == self.nparam
+        if p.size != self.nparam:
+            raise ValueError("expected %d parameters, got %d" % (self.nparam, p.size))
         _abi.check(_abi.load_library().yc_mlp_set_params(self._h, _fp(p)))
 
     def params(self):
