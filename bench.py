@@ -142,6 +142,28 @@ def tangent_measure(ys, spec, state, steps, precond):
     return out
 
 
+def hbm_stream_measure(launches=20):
+    """Sustained HBM bandwidth on this GPU: a STREAM-style FP64 negation of 1 GiB into
+    another 1 GiB (2 GiB moved per launch), timed with HIP events on torch's stream."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(launches):
+        torch.neg(x, out=y)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / launches
+    nbytes = 2 * x.numel() * 8
+    del x, y
+    torch.cuda.empty_cache()
+    return {"kernel": "torch.neg, FP64, 1 GiB -> 1 GiB", "bytes_per_launch": nbytes, "avg_launch_ms": ms,
+            "achieved": nbytes / (ms * 1e-3) / 1e9, "unit": "GB/s", "peak": HBM_PEAK_GBS}
+
+
 def measured_traffic(kernel, workload, iterations=None):
     """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE +
     WRITE_SIZE runs, scaled by the copy calibration), or None when there is none.  A
@@ -309,6 +331,9 @@ def main():
         out["roofline"] = spmv
     else:
         out["config"]["spmv"] = spmv
+    stream = hbm_stream_measure()
+    stream["roofline_over_stream"] = out["roofline"]["achieved"] / stream["achieved"]
+    out["config"]["hbm_stream"] = stream
     if args.batch_steps > 0 and not args.plain_only:
         out["config"]["batch"] = batch_measure(ys, args.batch_steps)
     if args.large_steps > 0 and not args.plain_only:
