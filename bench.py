@@ -198,6 +198,8 @@ def main():
     ap.add_argument("--precond", default="block2", choices=["block2", "jacobi"],
                     help="preconditioner of the persistent-CG runs (the Jacobi one, the reference solver's, is "
                          "also timed once, zero guess)")
+    ap.add_argument("--tangent-only", action="store_true",
+                    help="time only the differentiation line and print it (profiling runs)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run 5 launches of a 256 MiB -> 256 MiB FP64 negation, the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counter runs")
@@ -220,6 +222,9 @@ def main():
     spec, state, ext, _ = cs.workload(args.workload)
     h = spec["h"]
     tol = 1e-8
+    if args.tangent_only:
+        print(json.dumps({"tangents": tangent_measure(ys, spec, state, args.tangent_steps, args.precond)}), flush=True)
+        return
     if args.plain_only:
         net = None
     elif os.path.exists(ws.WEIGHTS):
