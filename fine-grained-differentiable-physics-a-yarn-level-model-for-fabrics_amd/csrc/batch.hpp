@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(ASM1_THREADS, 1) k_b_assemble1(const BatchItem
     const ClothDev C = bi.C;
     const AsmOut O = bi.O;
     const int tile = blockIdx.x - bi.t0;
+    constexpr bool ASM1_REF = false;
 #include "assemble1_body.inc"
 }
 

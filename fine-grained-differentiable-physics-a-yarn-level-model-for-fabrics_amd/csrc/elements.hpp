@@ -163,8 +163,10 @@ enum Kind { KIND_M = 0, KIND_KX = 1, KIND_KV = 2 };
 // tangents of the A entries go to dA when dA is set (exact mode).  With `detach`
 // (the reference's gradient), M v and K_v^T v take tangents through v only, as the
 // reference builds M and K_v outside the autograd graph (src/MatrixMp.cpp:46-98).
+// REF = true fixes the reference's gradient at compile time (detached, no dA): the
+// derivative parts of the Jacobian blocks are then unused and the compiler drops them.
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, bool REF = false>
 struct RowSinkT {
     const ClothDev* C;
     int me, mu_, mv_;
@@ -205,7 +207,7 @@ struct RowSinkT {
             return;
         }
         if (A) A[(size_t)s * TILE_N] += val(x);
-        if (dA) dA[(size_t)s * TILE_N] += tan_of(x);
+        if (!REF && dA) dA[(size_t)s * TILE_N] += tan_of(x);
     }
     // accumulate into element i of a small array through a switch, so that an index known
     // only at run time (the yarn direction c) does not force the array into scratch memory
@@ -217,7 +219,7 @@ struct RowSinkT {
     YS_DEV T velL(int k, int c) { return par<T>(C->v[c * C->N + k], tget(C->dv, c * C->N + k)); }
     YS_DEV T velE(int k, int c) { return par<T>(C->uvd[c * C->N + k], tget(C->duvd, c * C->N + k)); }
     // a block entry in a product with the velocity (value only when detached)
-    YS_DEV T blk(T b) const { return detach ? T(val(b)) : b; }
+    YS_DEV T blk(T b) const { return (REF || detach) ? T(val(b)) : b; }
 
     // LL block B of (force node F, dof node D): A[3F+v, 3D+u] = B[u][v] (src/MatrixMp.cpp:65-76)
     YS_DEV void LL(int kind, int kF, int kD, const mat3<T>& B) {
@@ -744,8 +746,8 @@ YS_DEV void face_element(const ClothDev& C, Sink& S, int ka, int kb2, int kc) {
 // ConSlideFriction at the thread's own crossing (src/PhysicsCloth.cpp:2579-2641),
 // run after every edge / bend of the node has been accumulated.  Returns comp.
 // ---------------------------------------------------------------------------
-template <class T>
-YS_DEV T friction(const ClothDev& C, RowSinkT<T>& S, T* is_slide) {
+template <class T, class Sink>
+YS_DEV T friction(const ClothDev& C, Sink& S, T* is_slide) {
     typedef v3<T> V;
     int k = S.me;
     V n = plane3<T>(C.nrm, C.dnrm, C.N, k);

@@ -16,8 +16,10 @@ struct ShearOut {
     mat3<T> B[3][3];
 };
 
+// Inlined into each kernel, so that a kernel that discards the blocks does not compute
+// them and the outputs need no stack copy.
 template <class T>
-__device__ void shear_compute(const ClothDev& C, int k0, int k1, int k3, const double* comp_arr, const double* dcomp_arr,
+__device__ __forceinline__ void shear_compute(const ClothDev& C, int k0, int k1, int k3, const double* comp_arr, const double* dcomp_arr,
                               ShearOut<T>& out) {
     typedef v3<T> V;
     typedef mat3<T> M;
@@ -198,6 +200,7 @@ __global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* co
 // the direction of each block's descriptors (blockIdx.y: the direction; its state and
 // parameter tangents in C, its compression tangents O.dcomp, its records O.shear_rec), in the
 // value records' layout.
+template <bool REF>
 __global__ void __launch_bounds__(256) k_shear_elem_tan(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
                                                         const double* comp) {
     const ClothDev C = Cs[blockIdx.y];
@@ -213,6 +216,7 @@ __global__ void __launch_bounds__(256) k_shear_elem_tan(const ClothDev* __restri
     for (int r = 0; r < 3; ++r) {
         double* base = rec + (size_t)r * 30 * nseg + s;
         base[0] = o.F[r].x.d; base[(size_t)nseg] = o.F[r].y.d; base[(size_t)2 * nseg] = o.F[r].z.d;
+        if constexpr (REF) continue;   // the reference gradient needs no block tangents
         for (int j = 0; j < 3; ++j)
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) base[(size_t)(3 + j * 9 + a * 3 + b) * nseg] = o.B[r][j].m[a][b].d;

@@ -235,15 +235,19 @@ __device__ __forceinline__ int tile_node(const ClothDev& C, int tile) { return t
 template <class T>
 __global__ void __launch_bounds__(ASM1_THREADS, 1) k_assemble1(ClothDev C, AsmOut O, double h) {
     const int tile = blockIdx.x;
+    constexpr bool ASM1_REF = false;
 #include "assemble1_body.inc"
 }
 
 // The tangent run of every direction in one launch (blockIdx.y: the direction, with its
 // own descriptors): one direction's tiles use a quarter of the SIMDs at one wavefront
-// each, so the directions go together.
+// each, so the directions go together.  REF: the reference's gradient (every direction
+// detached, no dA), so that the Jacobian blocks' derivative parts are not computed.
+template <bool REF>
 __global__ void __launch_bounds__(NBLK, 1) k_assemble1_dirs(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
                                                            double h) {
     typedef dd T;
+    constexpr bool ASM1_REF = REF;
     const ClothDev C = Cs[blockIdx.y];
     const AsmOut O = Os[blockIdx.y];
     const int tile = blockIdx.x;
@@ -259,10 +263,10 @@ __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& 
 __device__ __forceinline__ int shear_seg(const ClothDev& C, int e, int q);
 __device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
 
-template <class T>
+template <class T, bool REF = false>
 __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& O, double h, int k) {
     constexpr bool tan = !std::is_same<T, double>::value;
-    RowSinkT<T> S;
+    RowSinkT<T, REF> S;
     S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
     int u = S.mu_, v = S.mv_;
     extern __shared__ double lds_rows[];
@@ -350,10 +354,10 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
     if (S.bad) atomicOr(&O.flag[0], 1);
 }
 
-template <class T>
+template <class T, bool REF = false>
 __device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& O, double h, int tile) {
     int k = tile_node(C, tile);
-    if (k >= 0) assemble2_node<T>(C, O, h, k);
+    if (k >= 0) assemble2_node<T, REF>(C, O, h, k);
 }
 
 template <class T>
@@ -361,12 +365,14 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, dou
     assemble2_tile<T>(C, O, h, blockIdx.x);
 }
 
-// The second kernel's tangent run for every direction in one launch (blockIdx.y).
+// The second kernel's tangent run for every direction in one launch (blockIdx.y); REF as
+// for k_assemble1_dirs (the shear records then hold no block tangents, and none are read).
+template <bool REF>
 __global__ void __launch_bounds__(NBLK, 1) k_assemble2_dirs(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
                                                            double h) {
     const ClothDev C = Cs[blockIdx.y];
     const AsmOut O = Os[blockIdx.y];
-    assemble2_tile<dd>(C, O, h, blockIdx.x);
+    assemble2_tile<dd, REF>(C, O, h, blockIdx.x);
 }
 
 }  // namespace
@@ -2129,13 +2135,25 @@ int tangent_assemble_all(yc_ctx* c, int mode, double h) {
     }
     HIPCHK(hipMemcpyAsync(c->tdev, c->h_tdev.data(), sizeof(ClothDev) * c->ndir, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->tout, c->h_tout.data(), sizeof(AsmOut) * c->ndir, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_assemble1_dirs, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream, (const ClothDev*)c->tdev,
-                       (const AsmOut*)c->tout, h);
+    // make_tan_out gives every direction of the reference mode detach = 1 and no dA
+    if (mode == YC_GRAD_REF)
+        hipLaunchKernelGGL(k_assemble1_dirs<true>, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream, (const ClothDev*)c->tdev,
+                           (const AsmOut*)c->tout, h);
+    else
+        hipLaunchKernelGGL(k_assemble1_dirs<false>, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream, (const ClothDev*)c->tdev,
+                           (const AsmOut*)c->tout, h);
     // the shear segments' tangents, once per segment and direction, then the second kernel
-    hipLaunchKernelGGL(k_shear_elem_tan, dim3((4 * c->Nc + 255) / 256, c->ndir), dim3(256), 0, c->stream,
-                       (const ClothDev*)c->tdev, (const AsmOut*)c->tout, (const double*)c->comp);
-    hipLaunchKernelGGL(k_assemble2_dirs, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream, (const ClothDev*)c->tdev,
-                       (const AsmOut*)c->tout, h);
+    if (mode == YC_GRAD_REF) {
+        hipLaunchKernelGGL(k_shear_elem_tan<true>, dim3((4 * c->Nc + 255) / 256, c->ndir), dim3(256), 0, c->stream,
+                           (const ClothDev*)c->tdev, (const AsmOut*)c->tout, (const double*)c->comp);
+        hipLaunchKernelGGL(k_assemble2_dirs<true>, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream,
+                           (const ClothDev*)c->tdev, (const AsmOut*)c->tout, h);
+    } else {
+        hipLaunchKernelGGL(k_shear_elem_tan<false>, dim3((4 * c->Nc + 255) / 256, c->ndir), dim3(256), 0, c->stream,
+                           (const ClothDev*)c->tdev, (const AsmOut*)c->tout, (const double*)c->comp);
+        hipLaunchKernelGGL(k_assemble2_dirs<false>, dim3(c->nT, c->ndir), dim3(NBLK), 0, c->stream,
+                           (const ClothDev*)c->tdev, (const AsmOut*)c->tout, h);
+    }
     HIPCHK(hipGetLastError());
     return YC_OK;
 }
