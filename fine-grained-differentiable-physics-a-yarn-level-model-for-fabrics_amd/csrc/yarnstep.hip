@@ -1400,7 +1400,14 @@ struct NodeFrame {
     int im;
 };
 
-__device__ void node_frame(const StateDev& s, int k, NodeFrame& f) {
+// M[r][j] for a column j known only at run time, by selects
+__device__ __forceinline__ double frame_col(const double (&M)[3][3], int r, int j) {
+    return j == 0 ? M[r][0] : (j == 1 ? M[r][1] : M[r][2]);
+}
+
+// Inlined, with the smallest eigenpair picked by selects rather than a run-time index, so
+// that the frame stays in registers.
+__device__ __forceinline__ void node_frame(const StateDev& s, int k, NodeFrame& f) {
     int ids[5] = {k, k - 1, k + 1, k - s.W, k + s.W};
     double p[5][3];
     for (int i = 0; i < 5; ++i) for (int c = 0; c < 3; ++c) p[i][c] = s.x[c * s.N + ids[i]];
@@ -1431,9 +1438,12 @@ __device__ void node_frame(const StateDev& s, int k, NodeFrame& f) {
     }
     for (int i = 0; i < 3; ++i) f.lam[i] = A[i][i];
     int im = 0;
-    for (int i = 1; i < 3; ++i) if (A[i][i] < A[im][im]) im = i;
+    double lmin = A[0][0];
+    if (A[1][1] < lmin) { im = 1; lmin = A[1][1]; }
+    if (A[2][2] < lmin) im = 2;
     f.im = im;
-    double n[3] = {V[0][im], V[1][im], V[2][im]};
+    double n[3];
+    for (int c = 0; c < 3; ++c) n[c] = frame_col(V, c, im);
     double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     for (int c = 0; c < 3; ++c) n[c] = n[c] / nn;
     double a[3], b[3];
@@ -1457,7 +1467,7 @@ __device__ __forceinline__ void normals_node(const StateDev& s, int k) {
     NodeFrame f;
     node_frame(s, k, f);
     // the eigenvector normalised, then signed
-    double n[3] = {f.V[0][f.im], f.V[1][f.im], f.V[2][f.im]};
+    double n[3] = {frame_col(f.V, 0, f.im), frame_col(f.V, 1, f.im), frame_col(f.V, 2, f.im)};
     double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     for (int c = 0; c < 3; ++c) s.nrm[c * s.N + k] = f.sg * (n[c] / nn);
 }
@@ -1498,12 +1508,14 @@ __global__ void __launch_bounds__(NBLK) k_normals_tan(StateDev s, const double* 
         for (int r = 0; r < 3; ++r) E[r][j] = f.V[r][j] / nn;
     }
     int im = f.im;
+    const double e0 = frame_col(E, 0, im), e1 = frame_col(E, 1, im), e2 = frame_col(E, 2, im);
+    const double lam_min = im == 0 ? f.lam[0] : (im == 1 ? f.lam[1] : f.lam[2]);
     double Ce[3];
-    for (int a = 0; a < 3; ++a) Ce[a] = dC[a][0] * E[0][im] + dC[a][1] * E[1][im] + dC[a][2] * E[2][im];
+    for (int a = 0; a < 3; ++a) Ce[a] = dC[a][0] * e0 + dC[a][1] * e1 + dC[a][2] * e2;
     double dn[3] = {0, 0, 0};
     for (int j = 0; j < 3; ++j) {
         if (j == im) continue;
-        double cj = (E[0][j] * Ce[0] + E[1][j] * Ce[1] + E[2][j] * Ce[2]) / (f.lam[im] - f.lam[j]);
+        double cj = (E[0][j] * Ce[0] + E[1][j] * Ce[1] + E[2][j] * Ce[2]) / (lam_min - f.lam[j]);
         for (int r = 0; r < 3; ++r) dn[r] += cj * E[r][j];
     }
     for (int c = 0; c < 3; ++c) dnrm[c * s.N + k] = f.sg * dn[c];
