@@ -67,8 +67,8 @@ __global__ void __launch_bounds__(64) k_b_status(const BatchItem* __restrict__ i
 __global__ void __launch_bounds__(ASM1_THREADS, 1) k_b_assemble1(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
     typedef double T;
     const BatchItem& bi = it[map[blockIdx.x]];
-    const ClothDev C = bi.C;
-    const AsmOut O = bi.O;
+    const ClothDev& C = bi.C;
+    const AsmOut& O = bi.O;
     const int tile = blockIdx.x - bi.t0;
     constexpr bool ASM1_REF = false;
 #include "assemble1_body.inc"

@@ -203,8 +203,8 @@ __global__ void __launch_bounds__(256) k_shear_elem(ClothDev C, const double* co
 template <bool REF>
 __global__ void __launch_bounds__(256) k_shear_elem_tan(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
                                                         const double* comp) {
-    const ClothDev C = Cs[blockIdx.y];
-    const AsmOut O = Os[blockIdx.y];
+    const ClothDev& C = Cs[blockIdx.y];
+    const AsmOut& O = Os[blockIdx.y];
     const int nseg = O.nseg;
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= nseg) return;

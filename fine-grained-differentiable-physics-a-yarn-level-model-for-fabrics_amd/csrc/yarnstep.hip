@@ -248,8 +248,8 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble1_dirs(const ClothDev* __re
                                                            double h) {
     typedef dd T;
     constexpr bool ASM1_REF = REF;
-    const ClothDev C = Cs[blockIdx.y];
-    const AsmOut O = Os[blockIdx.y];
+    const ClothDev& C = Cs[blockIdx.y];
+    const AsmOut& O = Os[blockIdx.y];
     const int tile = blockIdx.x;
 #include "assemble1_body.inc"
 }
@@ -370,8 +370,8 @@ __global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, dou
 template <bool REF>
 __global__ void __launch_bounds__(NBLK, 1) k_assemble2_dirs(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
                                                            double h) {
-    const ClothDev C = Cs[blockIdx.y];
-    const AsmOut O = Os[blockIdx.y];
+    const ClothDev& C = Cs[blockIdx.y];
+    const AsmOut& O = Os[blockIdx.y];
     assemble2_tile<dd, REF>(C, O, h, blockIdx.x);
 }
 
