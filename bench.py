@@ -41,10 +41,10 @@ def cpu_baseline(cs, name, nsteps):
             "sample": "steps 2..%d of the %s workload, oracle single-thread FP64 CG to 1e-8" % (nsteps + 1, name)}
 
 
-def batch_measure(ys, steps, warmup=2):
+def batch_measure(ys, steps, warmup=2, precond="block2"):
     """64 cloths of uneven size (sides 10..64, two-yarn blends) stepped by shared
     launches with the autotuned options, against the same cloths stepped one after
-    another on their own (persistent CG, queued steps)."""
+    another on their own (persistent CG, queued steps, the same preconditioner)."""
     cs = ys.cloth_spec
     items = cs.batch_workload()
     h, tol = items[0][0]["h"], 1e-8
@@ -52,6 +52,7 @@ def batch_measure(ys, steps, warmup=2):
     for spec, state in items:
         c = ys.Cloth(spec)
         c.set_cg_variant("persistent")
+        c.set_preconditioner(precond)
         c.set_ClothState(state)
         cloths.append(c)
     b = ys.ClothBatch(cloths)
@@ -73,7 +74,8 @@ def batch_measure(ys, steps, warmup=2):
         c.close()
     nodes = sum(s["width"] * s["length"] for s, _ in items)
     return {"workload": "64 cloths, sides 10..64 (seed 3), two-yarn blends in plain/twill/satin, h=0.001, "
-                        "Jacobi-PCG to 1e-08, zero guess", "steps": steps,
+                        "%s-PCG to 1e-08, zero guess" % ("2x2-block-Jacobi" if precond == "block2" else "Jacobi"),
+            "steps": steps,
             "cloth_steps_per_s": len(items) * steps / dt, "node_steps_per_s": nodes * steps / dt,
             "ms_per_step": 1e3 * dt / steps, "sequential_ms_per_step": 1e3 * dt_seq / steps,
             "speedup_over_sequential": dt_seq / dt, "cg_iters_per_cloth_step": float(iters.sum()) / (len(items) * steps),
@@ -340,7 +342,9 @@ def main():
     stream["roofline_over_stream"] = out["roofline"]["achieved"] / stream["achieved"]
     out["config"]["hbm_stream"] = stream
     if args.batch_steps > 0 and not args.plain_only:
-        out["config"]["batch"] = batch_measure(ys, args.batch_steps)
+        # Jacobi: two solve blocks per CU, twice the resident tiles of the block-Jacobi form,
+        # which makes the batch faster despite the iterations (6.6 ms against 7.1 ms per step)
+        out["config"]["batch"] = batch_measure(ys, args.batch_steps, precond="jacobi")
     if args.large_steps > 0 and not args.plain_only:
         out["config"]["homo256"] = large_measure(ys, args.large_steps)
     if args.tangent_steps > 0 and not args.plain_only and args.cg == "persistent":

@@ -97,14 +97,16 @@ __global__ void __launch_bounds__(BLK) k_b_cg_init(const BatchItem* __restrict__
 // for peers still finishing an earlier group's cloth, which needs no block of a later
 // group, so every wait ends.  Blocks freed by a cloth that converged early go on to the
 // next group at once.
-template <int KP>
-__global__ void __launch_bounds__(BLK, 2) k_b_cgp(const BatchItem* __restrict__ it, const int* __restrict__ map,
+// With the block-Jacobi rows in registers a tile needs a whole SIMD's registers (one block
+// per CU); the Jacobi form fits two blocks per CU.
+template <int KP, bool BJ>
+__global__ void __launch_bounds__(BLK, BJ ? 1 : 2) k_b_cgp(const BatchItem* __restrict__ it, const int* __restrict__ map,
                                                const int* __restrict__ grp, int ngroups) {
     for (int g = 0; g < ngroups; ++g) {
         if ((int)blockIdx.x < grp[2 * g + 1]) {
             const int blk = grp[2 * g] + blockIdx.x;
             const BatchItem& b = it[map[blk]];
-            cgp_tile<KP>(b.cg, blk - b.t0, *b.epoch);
+            cgp_tile<KP, BJ>(b.cg, blk - b.t0, *b.epoch);
         }
         __syncthreads();   // the tile's LDS is reused by the next group's tile
     }
@@ -122,12 +124,21 @@ __global__ void __launch_bounds__(NB) k_b_normals(const BatchItem* __restrict__ 
     normals_node(b.st, (blockIdx.x - b.n0) * NB + threadIdx.x);
 }
 
-const void* b_cgp_fn(int kp) {
+// bj: the 2x2-block-Jacobi preconditioner (every cloth of the batch uses it), else Jacobi
+const void* b_cgp_fn(int kp, bool bj) {
+    if (bj) {
+        switch (kp) {
+            case 1: return (const void*)k_b_cgp<1, true>;
+            case 2: return (const void*)k_b_cgp<2, true>;
+            case 3: return (const void*)k_b_cgp<3, true>;
+            default: return (const void*)k_b_cgp<4, true>;
+        }
+    }
     switch (kp) {
-        case 1: return (const void*)k_b_cgp<1>;
-        case 2: return (const void*)k_b_cgp<2>;
-        case 3: return (const void*)k_b_cgp<3>;
-        default: return (const void*)k_b_cgp<4>;
+        case 1: return (const void*)k_b_cgp<1, false>;
+        case 2: return (const void*)k_b_cgp<2, false>;
+        case 3: return (const void*)k_b_cgp<3, false>;
+        default: return (const void*)k_b_cgp<4, false>;
     }
 }
 
@@ -140,6 +151,7 @@ struct yc_batch {
     hipStream_t stream = nullptr;
     yc_batch_opts opts;
     int kp = 1, capacity = 0;
+    bool bj = false;                           // every member uses the 2x2-block-Jacobi preconditioner
     int n_tile = 0, n_seg = 0, n_vec = 0, n_node = 0;
     std::vector<std::pair<int, int>> groups;   // (first tile, tiles) of each group of the solve launch
     int max_group = 0;
@@ -180,7 +192,7 @@ int batch_layout(yc_batch* b) {
     int per_cu = 0, ncu = 0, coop = 0;
     HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b->device));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_cgp_fn(b->kp), BLK, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_cgp_fn(b->kp, b->bj), BLK, 0));
     if (!coop) return fail(YC_ERR_HIP, "batch: the device has no cooperative launch");
     b->capacity = per_cu * ncu;
     std::vector<int> mt, ms, mv, mn;
@@ -244,7 +256,7 @@ int batch_upload_items(yc_batch* b, double tol, int maxit) {
         it.part = c->part;
         CgpArgs& a = it.cg;
         a.A = c->A; a.invd = c->invd; a.bbpart = c->part.bb; a.rrpart = c->part.rr;
-        a.brow = nullptr;   // the batch solves with the Jacobi preconditioner
+        a.brow = b->bj ? c->bj : nullptr;   // the members' common preconditioner
         a.x = c->cx;
         a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1; a.s[0] = c->cs0; a.s[1] = c->cs1;
         a.slot[0] = c->cgslot; a.slot[1] = c->cgslot + c->nT;
@@ -296,13 +308,19 @@ int batch_step(yc_batch* b, double h) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_b_assemble2, dim3(b->n_tile), dim3(NBLK), ASM_LDS, b->stream, it, (const int*)b->map_tile, h);
     hipLaunchKernelGGL(k_b_cg_init, dim3(b->n_vec), dim3(BLK), 0, b->stream, it, (const int*)b->map_vec);
+    if (b->bj)   // each cloth's block-Jacobi rows from its new A, on the batch stream
+        for (yc_ctx* c : b->m) {
+            const int ngx = (c->W + 1) / 2, ngy = (c->Lg + 1) / 2, ng = ngx * ngy;
+            hipLaunchKernelGGL(k_bj_factor, dim3((ng + BJF_GROUPS - 1) / BJF_GROUPS), dim3(64), 0, b->stream,
+                               (const double*)c->A, c->bj, c->W, c->Lg, c->Wp, c->P, c->ntx, ngx, ng);
+        }
     HIPCHK(hipGetLastError());
     {
         const int* mp = b->map_tile;
         const int* gp = b->d_groups;
         int ng = (int)b->groups.size();
         void* args[] = {(void*)&it, (void*)&mp, (void*)&gp, (void*)&ng};
-        HIPCHK(hipLaunchCooperativeKernel(b_cgp_fn(b->kp), dim3(b->max_group), dim3(BLK), args, 0, b->stream));
+        HIPCHK(hipLaunchCooperativeKernel(b_cgp_fn(b->kp, b->bj), dim3(b->max_group), dim3(BLK), args, 0, b->stream));
     }
     return batch_node_phase(b, h, 1);
 }
@@ -345,13 +363,15 @@ int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out) {
     for (int i = 0; i < n; ++i) {
         if (!ctxs[i]) return fail(YC_ERR_ARG, "batch: null context");
         if (ctxs[i]->device != ctxs[0]->device) return fail(YC_ERR_ARG, "batch: contexts on different devices");
-        if (ctxs[i]->precond != YC_PRECOND_JACOBI) return fail(YC_ERR_ARG, "batch: the batch solves with the Jacobi preconditioner");
+        if (ctxs[i]->precond != ctxs[0]->precond)
+            return fail(YC_ERR_ARG, "batch: the cloths of a batch use one preconditioner");
         for (int j = 0; j < i; ++j)
             if (ctxs[j] == ctxs[i]) return fail(YC_ERR_ARG, "batch: a context appears twice");
     }
     yc_batch* b = new yc_batch();
     b->m.assign(ctxs, ctxs + n);
     b->device = ctxs[0]->device;
+    b->bj = ctxs[0]->precond == YC_PRECOND_BLOCK2;
     b->opts.graph = 1;
     b->opts.group_order = 0;
     b->opts.node_block = 64;

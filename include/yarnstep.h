@@ -298,7 +298,9 @@ typedef struct {
     int node_block;   /* threads per block of the per-node and per-segment kernels: 64, 128 or 256 */
 } yc_batch_opts;
 /* ctxs: n distinct contexts on one device; they stay owned by the caller and must
- * outlive the batch.  Fails if one cloth's solve alone cannot be resident at once. */
+ * outlive the batch.  Fails if one cloth's solve alone cannot be resident at once.  The
+ * members share one preconditioner (yc_set_preconditioner before creating the batch: all
+ * YC_PRECOND_JACOBI or all YC_PRECOND_BLOCK2); the batch keeps the one they had. */
 int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out);
 void yc_batch_destroy(yc_batch* b);
 int yc_batch_set_opts(yc_batch* b, const yc_batch_opts* opts);

@@ -28,22 +28,24 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-def _cloths(ys, items):
+def _cloths(ys, items, precond="jacobi"):
     out = []
     for spec, state in items:
         c = ys.Cloth(spec)
         c.set_cg_variant("persistent")
+        c.set_preconditioner(precond)
         c.set_ClothState(state)
         out.append(c)
     return out
 
 
-def _alone(ys, items, h, steps, tol):
+def _alone(ys, items, h, steps, tol, precond="jacobi"):
     """Each cloth stepped on its own (persistent CG, queued steps)."""
     res = []
     for spec, state in items:
         c = ys.Cloth(spec)
         c.set_cg_variant("persistent")
+        c.set_preconditioner(precond)
         c.set_ClothState(state)
         it, rc = c.run(h, steps, tol=tol)
         assert rc == 0
@@ -63,13 +65,15 @@ def _small_items(cs):
     return items
 
 
-@pytest.mark.parametrize("graph,order,nb", [(1, 0, 64), (0, 1, 128), (1, 1, 256)])
-def test_batch_equals_cloths_alone(ys, cs, graph, order, nb):
+@pytest.mark.parametrize("graph,order,nb,precond", [(1, 0, 64, "jacobi"), (0, 1, 128, "jacobi"), (1, 1, 256, "jacobi"),
+                                                    (1, 0, 64, "block2"), (0, 1, 128, "block2")])
+def test_batch_equals_cloths_alone(ys, cs, graph, order, nb, precond):
     items = _small_items(cs)
     h, steps, tol = items[0][0]["h"], 6, 1e-8
-    say("batch of %s, graph %d order %d node_block %d" % ([s["width"] for s, _ in items], graph, order, nb))
-    ref = _alone(ys, items, h, steps, tol)
-    cl = _cloths(ys, items)
+    say("batch of %s, graph %d order %d node_block %d precond %s" % ([s["width"] for s, _ in items], graph, order, nb,
+                                                                    precond))
+    ref = _alone(ys, items, h, steps, tol, precond)
+    cl = _cloths(ys, items, precond)
     b = ys.ClothBatch(cl)
     b.set_opts(graph=graph, group_order=order, node_block=nb)
     it, st, rc = b.run(h, steps, tol=tol)
@@ -85,14 +89,16 @@ def test_batch_equals_cloths_alone(ys, cs, graph, order, nb):
         c.close()
 
 
-def test_batch_uneven_64_cloths_several_groups(ys, cs):
+@pytest.mark.parametrize("precond", ["jacobi", "block2"])
+def test_batch_uneven_64_cloths_several_groups(ys, cs, precond):
     """The 64-cloth workload (sides 10..64, 1,743 tiles) needs more tiles than the
     device holds at once, so its solves run as several cooperative groups; two runs
     of one step each equal the cloths stepped alone."""
     items = cs.batch_workload()
     h, tol = items[0][0]["h"], 1e-8
-    ref = _alone(ys, items, h, 2, tol)
-    cl = _cloths(ys, items)
+    say("64-cloth batch, precond %s" % precond)
+    ref = _alone(ys, items, h, 2, tol, precond)
+    cl = _cloths(ys, items, precond)
     b = ys.ClothBatch(cl)
     info = b.info()
     say("  64-cloth batch info %s" % info)
@@ -143,6 +149,12 @@ def test_batch_rejects_bad_input(ys, cs):
     with pytest.raises(ys.YcError):
         b.set_opts(node_block=96)
     b.close()
+    c2 = ys.Cloth(spec)
+    c2.set_cg_variant("persistent")
+    c2.set_preconditioner("block2")
+    with pytest.raises(ys.YcError):   # one preconditioner per batch
+        ys.ClothBatch([c, c2])
+    c2.close()
     c.close()
 
 
