@@ -179,11 +179,16 @@ def test_block_jacobi_needs_the_persistent_cg(ys, cs):
     g.set_cg_variant("persistent")
     it, rr, ok = g.solve(tol=1e-8)
     assert ok
-    with pytest.raises(ys.YcError):
-        ys.ClothBatch([g])           # the batch solves with Jacobi only
-    g.set_preconditioner("jacobi")
-    b = ys.ClothBatch([g])
+    b = ys.ClothBatch([g])           # a batch of block-Jacobi cloths
     b.close()
+    g2 = ys.Cloth(spec)
+    g2.set_cg_variant("persistent")
+    with pytest.raises(ys.YcError):
+        ys.ClothBatch([g, g2])       # one preconditioner per batch
+    g.set_preconditioner("jacobi")
+    b = ys.ClothBatch([g, g2])
+    b.close()
+    g2.close()
     g.close()
 
 
