@@ -10,7 +10,7 @@ raises if the HIP library is missing or no device is present.
 from . import cloth_spec  # noqa: F401
 from ._abi import (YcError, NotConverged, NonFinite, YC_OK, YC_ERR_NONFINITE,  # noqa: F401
                    YC_ERR_NOT_CONVERGED, YC_ERR_HIP, YC_ERR_ARG, lib_path, load_library)
-from .cloth import Cloth, PhysicsCloth, SimStep  # noqa: F401
+from .cloth import Cloth, EigenSolver, PhysicsCloth, SimStep  # noqa: F401
 from . import grad  # noqa: F401
 from .batch import ClothBatch, autotune_batch, autotune_cloth  # noqa: F401
 from . import learn  # noqa: F401

@@ -57,7 +57,8 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_time_solve", "yc_cg_iteration_bytes",
            "yc_batch_create", "yc_batch_destroy", "yc_batch_set_opts", "yc_batch_get_opts", "yc_batch_run",
            "yc_batch_info", "yc_time_assemble", "yc_tangent_set_ext", "yc_set_preconditioner",
-           "yc_set_state_device", "yc_get_state_device", "yc_export_precond"]
+           "yc_set_state_device", "yc_get_state_device", "yc_export_precond", "yc_export_matrix",
+           "yc_load_system", "yc_update_vel"]
 
 
 def lib_path():
@@ -99,6 +100,9 @@ def load_library():
     L.yc_assemble.argtypes = [vp, ctypes.c_double]
     L.yc_export_csr.argtypes = [vp, ip, ip, dp, ip]
     L.yc_export_vec.argtypes = [vp, ctypes.c_int, dp]
+    L.yc_export_matrix.argtypes = [vp, ctypes.c_int, ip, ip, dp, ip]
+    L.yc_load_system.argtypes = [vp, ip, ip, dp, dp]
+    L.yc_update_vel.argtypes = [vp, dp, dp, ctypes.c_double]
     L.yc_export_crossing.argtypes = [vp, dp, dp, dp]
     L.yc_export_normals.argtypes = [vp, dp]
     L.yc_solve.argtypes = [vp, dp, ctypes.c_double, ctypes.c_int, ip, dp]

@@ -111,23 +111,52 @@ class Cloth:
     def assemble(self, h):
         _abi.check(_abi.load_library().yc_assemble(self._h, float(h)))
 
-    def csr(self):
-        """A (as assembled) -> scipy.sparse.csr_matrix in the reference's global layout."""
+    MATRICES = {"A": 0, "M": 1, "Kx": 2, "Kv": 3}
+
+    def csr(self, which="A"):
+        """A (as assembled), or M, K_x or K_v alone (get_SpaM / get_SpaFPos / get_SpaFVel)
+        -> scipy.sparse.csr_matrix in the reference's global layout."""
         import scipy.sparse as sp
         L = _abi.load_library()
         nnz = ctypes.c_int()
         ip = ctypes.POINTER(ctypes.c_int)
-        _abi.check(L.yc_export_csr(self._h, ip(), ip(), _NULL, ctypes.byref(nnz)))
+        kind = self.MATRICES[which]
+        _abi.check(L.yc_export_matrix(self._h, kind, ip(), ip(), _NULL, ctypes.byref(nnz)))
         rp = np.zeros(self.n + 1, dtype=np.int32)
         cl = np.zeros(nnz.value, dtype=np.int32)
         val = np.zeros(nnz.value)
-        _abi.check(L.yc_export_csr(self._h, rp.ctypes.data_as(ip), cl.ctypes.data_as(ip), _dp(val), ctypes.byref(nnz)))
+        _abi.check(L.yc_export_matrix(self._h, kind, rp.ctypes.data_as(ip), cl.ctypes.data_as(ip), _dp(val),
+                                      ctypes.byref(nnz)))
         return sp.csr_matrix((val, cl, rp), shape=(self.n, self.n))
+
+    VECTORS = {"F": 0, "b": 1, "v": 2, "x": 3, "pos": 4}
 
     def vec(self, which):
         out = np.zeros(self.n)
-        _abi.check(_abi.load_library().yc_export_vec(self._h, {"F": 0, "b": 1, "v": 2, "x": 3}[which], _dp(out)))
+        _abi.check(_abi.load_library().yc_export_vec(self._h, self.VECTORS[which], _dp(out)))
         return out
+
+    def load_system(self, A, b):
+        """Makes a caller-formed system the context's (EigenSolver::apply's input side):
+        A a scipy sparse matrix inside the yarn stencil, b an n-vector."""
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A)
+        if A.shape != (self.n, self.n):
+            raise ValueError("expected an %d x %d matrix" % (self.n, self.n))
+        ip = ctypes.POINTER(ctypes.c_int)
+        rp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        cl = np.ascontiguousarray(A.indices, dtype=np.int32)
+        val = np.ascontiguousarray(A.data, dtype=np.float64)
+        bb = _arr(b, (self.n,))
+        _abi.check(_abi.load_library().yc_load_system(self._h, rp.ctypes.data_as(ip), cl.ctypes.data_as(ip), _dp(val),
+                                                       _dp(bb)))
+
+    def update_vel(self, new_vel, is_slide, h):
+        """Cloth::update(new_vel, F_E_is_Slide, h) (src/Cloth.cpp:111-148) with a caller's
+        velocity; is_slide (Nc, 2) or None for the last assembly's."""
+        nv = _arr(new_vel, (self.n,))
+        sl = _NULL if is_slide is None else _dp(_arr(is_slide, (self.Nc, 2)))
+        _abi.check(_abi.load_library().yc_update_vel(self._h, _dp(nv), sl, float(h)))
 
     def crossing(self):
         sl, cp, fp = np.zeros(2 * self.Nc), np.zeros(self.Nc), np.zeros(2 * self.Nc)
@@ -316,26 +345,60 @@ class Cloth:
 
 
 class PhysicsCloth:
-    """Reference-named view of the assembled system (src/PhysicsCloth.h:27-77)."""
+    """Reference-named view of the assembled system (src/PhysicsCloth.h:27-77).  The
+    get_Spa* matrices are scipy CSR (dense n x n tensors in the reference), vectors are
+    n-element numpy arrays, both in the reference's global layout."""
 
     def __init__(self, cloth):
         self.cloth = cloth
+        self._h = None
 
     def FillMV(self, h):
+        """FillMV; the step h is the one A and b are formed with (SimStep's)."""
         self.cloth.assemble(h)
+        self._h = h
+
+    def ResetMV(self):
+        """Nothing to reset: the next assembly overwrites the system."""
 
     def get_SpaA(self):
-        return self.cloth.csr()
+        return self.cloth.csr("A")
+
+    def get_SpaM(self):
+        return self.cloth.csr("M")
+
+    def get_SpaFPos(self):
+        return self.cloth.csr("Kx")
+
+    def get_SpaFVel(self):
+        return self.cloth.csr("Kv")
 
     def get_SpaF(self):
         return self.cloth.vec("F")
+
+    def get_SpaPos(self):
+        return self.cloth.vec("pos")
 
     def get_SpaVel(self):
         return self.cloth.vec("v")
 
     @property
     def F_E_is_Slides(self):
-        return self.cloth.crossing()[0]
+        """[F_E_is_Slides[0]]: the is_slide of every crossing, (Nc, 2)."""
+        return [self.cloth.crossing()[0]]
+
+
+class EigenSolver:
+    """EigenSolver::apply (src/Solvers.h:35-52) forward on the GPU: the caller's system
+    becomes the cloth's and is solved with the reference's Jacobi-PCG to tol."""
+
+    def __init__(self, cloth, tol=1e-8, maxit=None):
+        self.cloth, self.tol, self.maxit = cloth, tol, maxit
+
+    def apply(self, A, b):
+        self.cloth.load_system(A, b)
+        self.cloth.solve(tol=self.tol, maxit=self.maxit, raise_on_nonconv=False)
+        return self.cloth.vec("x")
 
 
 def SimStep(cloth, sim_phy, h, ext_force=None, tol=1e-8, maxit=None):

@@ -194,6 +194,14 @@ struct RowSinkT {
         fric_owner = inter ? omp_owner(c->Nc, crossing(*c, mu_, mv_)) : -1;
         bad = 0;
     }
+    // Rows of one matrix of the system alone (kind 0: M, 1: K_x, 2: K_v, each with
+    // coefficient 1): the reference's get_SpaM / get_SpaFPos / get_SpaFVel
+    // (src/PhysicsCloth.h:72-77).  kind < 0 keeps A = M - h^2 K_x - h K_v.  The force, M v,
+    // K_v^T v and the friction inputs do not depend on the coefficients.
+    YS_DEV void select_kind(int kind) {
+        if (kind < 0) return;
+        for (int i = 0; i < 3; ++i) coef[i] = i == kind ? 1.0 : 0.0;
+    }
     YS_DEV int slot(int rc, int kcol, int cc) {
         int s = slot_of(rc, kcol % C->W - mu_, kcol / C->W - mv_, cc);
         if (s < 0) bad = 1;

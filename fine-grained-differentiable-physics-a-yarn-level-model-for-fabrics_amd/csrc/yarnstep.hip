@@ -199,6 +199,8 @@ struct AsmOut {
     const double* shear_rec;
     int nseg;
     const double* bend_rec;
+    // value run: -1 for A; 0, 1, 2 for M, K_x or K_v alone (RowSinkT::select_kind)
+    int only_kind;
 };
 
 __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
@@ -268,6 +270,7 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
     constexpr bool tan = !std::is_same<T, double>::value;
     RowSinkT<T, REF> S;
     S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
+    if (!tan) S.select_kind(O.only_kind);
     int u = S.mu_, v = S.mv_;
     extern __shared__ double lds_rows[];
     double* gA = S.A;
@@ -1633,6 +1636,9 @@ struct yc_ctx {
     std::vector<AsmOut> h_tout;
     double* shear_rec;                      // 3 roles x 30 x (4 Nc) shear records
     double* bend_rec;                       // 3 roles x BEND_REC x n_bends bend records
+    double last_h;                          // step of the last assembly
+    double* Aexp;                           // M, K_x or K_v alone, for yc_export_matrix (on first use)
+    double* dexp;                           // the diagonal scratch of that assembly (5 P)
 };
 
 namespace {
@@ -1671,6 +1677,7 @@ AsmOut make_out(yc_ctx* c) {
     O.detach = 0;
     O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
     O.bend_rec = c->bend_rec;
+    O.only_kind = -1;
     O.invdiag = c->invd; O.ext = c->has_ext ? c->ext : nullptr; O.flag = c->flag; O.Wp = c->Wp; O.P = c->P;
     return O;
 }
@@ -1906,6 +1913,7 @@ void from_padded(const yc_ctx* c, const std::vector<double>& pad, double* glob) 
 int do_assemble(yc_ctx* c, double h, bool async = false) {
     if (!async) HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
+    c->last_h = h;
     static_assert(NBLK == TILE_N, "assembly blocks are matrix tiles");
     if constexpr (USE_BEND_REC) if (c->cd.n_bends > 0)
         hipLaunchKernelGGL(k_bend_elem, dim3((c->cd.n_bends + 255) / 256), dim3(256), 0, c->stream, c->cd, c->bend_rec,
@@ -2333,7 +2341,7 @@ void yc_destroy(yc_ctx* c) {
                     c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
                     c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
                     c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs, c->tdext, c->tshear,
-                    c->shear_rec, c->bend_rec, c->bepoch, c->bj};
+                    c->shear_rec, c->bend_rec, c->bepoch, c->bj, c->Aexp, c->dexp};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (c->h_sc) (void)hipHostFree(c->h_sc);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
@@ -2461,11 +2469,15 @@ int yc_assemble(yc_ctx* c, double h) {
     return do_assemble(c, h);
 }
 
-int yc_export_csr(yc_ctx* c, int* rowptr, int* col, double* val, int* nnz) {
-    if (!c) return fail(YC_ERR_ARG, "null context");
-    if (!c->assembled) return fail(YC_ERR_ARG, "export before assemble");
+}  // extern "C"
+
+namespace {
+
+// The structural pattern of a tiled stencil matrix (A, or M, K_x, K_v alone) as CSR of the
+// global n x n matrix, rows in global order and columns sorted.
+int export_tiles(yc_ctx* c, const double* Adev, int* rowptr, int* col, double* val, int* nnz) {
     std::vector<double> Am(c->a_size);
-    HIPCHK(hipMemcpyAsync(Am.data(), c->A, Am.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(Am.data(), Adev, Am.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     // rows in global order; every structurally possible slot inside the grid
     std::vector<std::vector<std::pair<int, double>>> rows(c->n);
@@ -2517,6 +2529,132 @@ int yc_export_csr(yc_ctx* c, int* rowptr, int* col, double* val, int* nnz) {
     return YC_OK;
 }
 
+// The value assembly with the rows of one matrix alone (kind 0: M, 1: K_x, 2: K_v) into
+// c->Aexp.  Everything else it writes (F, b, the friction outputs, the shear records) gets
+// the values the last assembly wrote, from the same state and step; the Jacobi diagonal goes
+// to scratch, so the context's system stays as assembled.
+int assemble_kind(yc_ctx* c, int kind) {
+    if (!c->Aexp) {
+        int rc = dalloc(&c->Aexp, c->a_size);
+        if (!rc) rc = dalloc(&c->dexp, (size_t)5 * c->P);
+        if (rc) return rc;
+    }
+    AsmOut O = make_out(c);
+    O.A = c->Aexp;
+    O.invdiag = c->dexp;
+    O.only_kind = kind;
+    HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_assemble1<double>, dim3(c->nT), dim3(ASM1_THREADS), ASM1_LDS, c->stream, c->cd, O, c->last_h);
+    if (c->Nc > 0)
+        hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
+                           c->shear_rec, 4 * c->Nc);
+    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, O, c->last_h);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->h_flag[0]) return fail(YC_ERR_HIP, "assembly: stencil slot out of range");
+    return YC_OK;
+}
+
+// Global row or column index -> (node u, v, component 0..4); false for an index past n.
+bool global_dof(const yc_ctx* c, int g, int& u, int& v, int& comp) {
+    if (g < 0 || g >= c->n) return false;
+    if (g < 3 * c->N) {
+        int k = g / 3;
+        comp = g - 3 * k; u = k % c->W; v = k / c->W;
+        return true;
+    }
+    int r = g - 3 * c->N, e = r / 2;
+    comp = 3 + (r - 2 * e);
+    u = e % (c->W - 2) + 1; v = e / (c->W - 2) + 1;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yc_export_csr(yc_ctx* c, int* rowptr, int* col, double* val, int* nnz) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (!c->assembled) return fail(YC_ERR_ARG, "export before assemble");
+    return export_tiles(c, c->A, rowptr, col, val, nnz);
+}
+
+int yc_export_matrix(yc_ctx* c, int which, int* rowptr, int* col, double* val, int* nnz) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (!c->assembled) return fail(YC_ERR_ARG, "export before assemble");
+    if (which < YC_MAT_A || which > YC_MAT_KV) return fail(YC_ERR_ARG, "unknown matrix");
+    // the structural pattern (every slot inside the grid) is the same for all four
+    if (which == YC_MAT_A || (!rowptr && !col && !val)) return export_tiles(c, c->A, rowptr, col, val, nnz);
+    int rc = assemble_kind(c, which - 1);
+    if (rc) return rc;
+    return export_tiles(c, c->Aexp, rowptr, col, val, nnz);
+}
+
+int yc_load_system(yc_ctx* c, const int* rowptr, const int* col, const double* val, const double* b) {
+    if (!c || !rowptr || !col || !val || !b) return fail(YC_ERR_ARG, "null argument");
+    g_err.clear();
+    std::vector<double> At(c->a_size, 0.0);
+    std::vector<double> diag(c->n, 0.0);
+    for (int r = 0; r < c->n; ++r)
+        for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
+            int u, v, rc_, uc, vc, cc;
+            if (!global_dof(c, col[j], uc, vc, cc)) return fail(YC_ERR_ARG, "column index out of range");
+            global_dof(c, r, u, v, rc_);
+            if (col[j] == r) diag[r] = val[j];
+            const int s = slot_of(rc_, uc - u, vc - v, cc);
+            if (s < 0) {
+                if (val[j] != 0.0) return fail(YC_ERR_ARG, "entry outside the yarn stencil: row " + std::to_string(r) +
+                                                               ", column " + std::to_string(col[j]));
+                continue;
+            }
+            At[a_base(c->ntx, u, v) + (size_t)s * TILE_N] += val[j];
+        }
+    for (int i = 0; i < c->n; ++i)
+        if (!std::isfinite(diag[i]) || !std::isfinite(b[i])) return fail(YC_ERR_NONFINITE, "NaN/Inf in A or b");
+    for (double x : At)
+        if (!std::isfinite(x)) return fail(YC_ERR_NONFINITE, "NaN/Inf in A or b");
+    std::vector<double> bp, dp, inv(c->n);
+    // the reference solver's diagonal preconditioner: 1/a_ii, or 1 where a_ii is zero
+    for (int i = 0; i < c->n; ++i) inv[i] = diag[i] != 0.0 ? 1.0 / diag[i] : 1.0;
+    to_padded(c, b, bp);
+    to_padded(c, inv.data(), dp);
+    HIPCHK(hipMemcpyAsync(c->A, At.data(), At.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->b, bp.data(), bp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->invd, dp.data(), dp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->bj_of = nullptr;
+    c->assembled = true;
+    c->solved = false;
+    return YC_OK;
+}
+
+int yc_update_vel(yc_ctx* c, const double* new_vel, const double* is_slide, double h) {
+    if (!c || !new_vel) return fail(YC_ERR_ARG, "null argument");
+    std::vector<double> pad;
+    to_padded(c, new_vel, pad);
+    HIPCHK(hipMemcpyAsync(c->cx, pad.data(), pad.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> sl;
+    if (is_slide) {
+        sl.assign((size_t)2 * c->N, 0.0);
+        for (int v = 1; v < c->Lg - 1; ++v)
+            for (int u = 1; u < c->W - 1; ++u) {
+                const int k = u + v * c->W, e = (u - 1) + (v - 1) * (c->W - 2);
+                sl[k] = is_slide[2 * e];
+                sl[(size_t)c->N + k] = is_slide[2 * e + 1];
+            }
+        HIPCHK(hipMemcpyAsync(c->is_slide, sl.data(), sl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    if (!c->assembled)   // the warm-start history reads v_old in padded form, as after an assembly
+        hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
+                           c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
+    c->solved = true;
+    int rc = do_update(c, h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return YC_OK;
+}
+
 int yc_export_vec(yc_ctx* c, int which, double* out) {
     if (!c || !out) return fail(YC_ERR_ARG, "null argument");
     size_t N = c->N;
@@ -2535,10 +2673,10 @@ int yc_export_vec(yc_ctx* c, int which, double* out) {
             }
         return YC_OK;
     }
-    if (which == 2) {
+    if (which == 2 || which == 4) {   // velocities (get_SpaVel) or positions (get_SpaPos)
         std::vector<double> v3(3 * N), v2(2 * N);
-        HIPCHK(hipMemcpyAsync(v3.data(), c->v, v3.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(v2.data(), c->uvd, v2.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(v3.data(), which == 2 ? c->v : c->x, v3.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(v2.data(), which == 2 ? c->uvd : c->uv, v2.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int v = 0; v < c->Lg; ++v)
             for (int u = 0; u < c->W; ++u) {

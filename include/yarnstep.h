@@ -109,7 +109,23 @@ int yc_assemble(yc_ctx* ctx, double h);
  * global n x n matrix.  rowptr n+1 ints, col/val nnz entries (nnz from yc_export_csr
  * with col == val == NULL). */
 int yc_export_csr(yc_ctx* ctx, int* rowptr, int* col, double* val, int* nnz);
-/* which: 0 F (get_SpaF), 1 b, 2 v (get_SpaVel), 3 last solution x.  out: n doubles. */
+/* One matrix of the assembled system in the reference's global layout (dense in the
+ * reference, CSR here, with A's structural pattern):
+ *   YC_MAT_A   A = M - h^2 K_x - h K_v, as yc_export_csr;
+ *   YC_MAT_M   get_SpaM (src/PhysicsCloth.h:72);
+ *   YC_MAT_KX  get_SpaFPos, the position Jacobian of the forces (src/PhysicsCloth.h:74);
+ *   YC_MAT_KV  get_SpaFVel, the velocity Jacobian (src/PhysicsCloth.h:75).
+ * L-L blocks are placed transposed as To_Sparse_Mat places them (src/MatrixMp.cpp:65-76).
+ * M, K_x and K_v are re-evaluated from the state and step of the last yc_assemble (a
+ * second assembly pass; the context's system is left as assembled).  Pointers as
+ * yc_export_csr: with rowptr, col and val NULL only nnz is returned. */
+#define YC_MAT_A 0
+#define YC_MAT_M 1
+#define YC_MAT_KX 2
+#define YC_MAT_KV 3
+int yc_export_matrix(yc_ctx* ctx, int which, int* rowptr, int* col, double* val, int* nnz);
+/* which: 0 F (get_SpaF), 1 b, 2 v (get_SpaVel), 3 last solution x, 4 positions
+ * (get_SpaPos, src/PhysicsCloth.h:76: LPos, then EPos of the crossings).  out: n doubles. */
 int yc_export_vec(yc_ctx* ctx, int which, double* out);
 /* Per crossing: is_slide (Nc*2, the F_E_is_Slides[0] of src/PhysicsCloth.cpp:2633),
  * compression (Nc), friction input F_E partial (Nc*2).  Any pointer may be NULL. */
@@ -126,6 +142,21 @@ int yc_solve(yc_ctx* ctx, const double* x0, double tol, int maxit, int* iters, d
  * the last solution as new_vel, including the geometry refresh. */
 int yc_update(yc_ctx* ctx, double h);
 
+/* Replaces: Cloth::update(new_vel, F_E_is_Slide, h) (src/Cloth.cpp:111-148) with a
+ * caller's velocity: new_vel n doubles in the global layout, is_slide Nc x 2 doubles
+ * (NULL: the last assembly's F_E_is_Slides[0]).  For callers that form the step
+ * themselves, as the reference's inline step does (src/Simulate.h:86-96). */
+int yc_update_vel(yc_ctx* ctx, const double* new_vel, const double* is_slide, double h);
+
+/* Replaces: the input side of EigenSolver::apply(A, b) (src/Solvers.h:35-52) for a system
+ * the caller formed, e.g. from the yc_export_matrix pieces as src/Simulate.h:86-90 forms
+ * it: A as CSR of the global n x n matrix (rowptr n+1, col/val by row; entries may repeat
+ * and are summed) and b (n doubles) become the context's system, and yc_solve then solves
+ * it with the reference's Jacobi preconditioner (its diagonal, 1/a_ii or 1 where zero).
+ * A non-zero entry outside the yarn stencil (A's structural pattern) is YC_ERR_ARG;
+ * NaN/Inf in A or b is YC_ERR_NONFINITE. */
+int yc_load_system(yc_ctx* ctx, const int* rowptr, const int* col, const double* val, const double* b);
+
 /* Replaces: SimStep (src/TrainHelper.cpp:15-97): assemble, solve, update.  x0 as in
  * yc_solve (NULL = zero guess).  info may be NULL. */
 int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_step_info* info);
@@ -133,9 +164,11 @@ int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_s
 /* CG variant of yc_solve / yc_step / yc_run and the warm-started forms.
  * YC_CG_REFERENCE (default): the reference solver's Jacobi-PCG recurrences in its order
  *   (src/Solvers.cpp:22-24), three kernels and two reductions per iteration.
- * YC_CG_FUSED: the single-reduction (Chronopoulos-Gear) form of the same Jacobi-PCG --
- *   the same Krylov iterates in exact arithmetic, the same stopping rule on |r|^2 and
- *   the same iteration count convention -- with one kernel per iteration.  Solutions
+ * YC_CG_FUSED: a single-reduction form of the same Jacobi-PCG (A u and A p carried as
+ *   vectors, so p.Ap is known from the previous iteration's five inner products; no
+ *   symmetry of A assumed) -- the same Krylov iterates in exact arithmetic, the same
+ *   stopping rule on |r|^2 and the same iteration count convention -- with one kernel
+ *   per iteration.  Solutions
  *   agree with the reference form to the solve tolerance; counts may differ by rounding. */
 #define YC_CG_REFERENCE 0
 #define YC_CG_FUSED 1

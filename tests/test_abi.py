@@ -69,3 +69,12 @@ def test_compute_raises_without_device(ys, cs):
         pytest.skip("a device is present")
     with pytest.raises(ys.YcError):
         ys.Cloth(cs.homo_spec(5))
+
+
+def test_cpp_adapter_driver_builds(tmp_path):
+    """include/yarnstep.hpp (the C++ mirror of Cloth / PhysicsCloth / EigenSolver /
+    SimStep over the C ABI) compiles and links into tests/abi_driver.cpp, a driver
+    written as the reference's inline step (src/Simulate.h:86-96); the GPU tests run it."""
+    import __graft_entry__
+    exe = __graft_entry__.build_abi_driver(out=str(tmp_path / "abi_driver"))
+    assert os.path.exists(exe)
