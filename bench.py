@@ -235,7 +235,7 @@ def main():
         X, T, M = ws.generate(ws.training_cloths(6), steps=40, skip=5)
         net, _ = ws.fit(X, T, M, epochs=16)
 
-    def timed(warm, variant=None, precond=None):
+    def timed(warm, variant=None, precond=None, guess=None):
         cloth = ys.Cloth(spec)
         try:
             cloth.set_cg_variant(variant or args.cg)
@@ -247,7 +247,12 @@ def main():
             cloth.set_preconditioner(precond or args.precond)
         cloth.set_ClothState(state)
         cloth.set_ext_force(ext)
-        run = (lambda n: cloth.run_warm(net, h, n, tol=tol)) if warm else (lambda n: cloth.run(h, n, tol=tol))
+        if warm:
+            run = lambda n: cloth.run_warm(net, h, n, tol=tol)   # noqa: E731
+        elif guess:
+            run = lambda n: cloth.run_guess(guess, h, n, tol=tol)   # noqa: E731
+        else:
+            run = lambda n: cloth.run(h, n, tol=tol)   # noqa: E731
         run(args.warmup)
         t0 = time.perf_counter()
         iters, rc = run(args.steps)
@@ -280,20 +285,43 @@ def main():
     if args.cg == "persistent":
         cloth.assemble(h)
         solve_ms, solve_iters = cloth.time_solve(tol=tol, reps=10)
-        solve_bytes = solve_iters * cloth.cg_iteration_bytes()
+        # algorithmic bytes of one Jacobi-PCG iteration streamed from memory (A once, 15
+        # vector passes of n doubles): 8 nnz + 120 n.  The block-Jacobi preconditioner's
+        # rows add 152 n; that unit is reported beside it, labelled.
+        n = cloth.n
+        it_jacobi = spmv_bytes - 16.0 * n + 120.0 * n
+        it_block = cloth.cg_iteration_bytes() if args.precond == "block2" else it_jacobi
+        solve_bytes = solve_iters * it_jacobi
         bj = args.precond == "block2"
-        dominant = {"bound": "hbm", "achieved": solve_bytes / (solve_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s",
-                    "traffic": measured_traffic("k_cgp_bj" if bj else "k_cgp", args.workload, solve_iters),
+        traffic = measured_traffic("k_cgp_bj" if bj else "k_cgp", args.workload, solve_iters)
+        secs = solve_ms * 1e-3
+        dominant = {"bound": "hbm", "achieved": solve_bytes / secs / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "traffic": traffic,
                     "kernel": "k_cgp (one persistent %s-PCG solve)" % ("2x2-block-Jacobi" if bj else "Jacobi"),
-                    "bytes_per_launch": solve_bytes,
-                    "avg_launch_ms": solve_ms, "iterations_per_launch": solve_iters}
+                    "bytes_per_launch": solve_bytes, "bytes_unit": "Jacobi-PCG iteration: 8 nnz + 120 n",
+                    "avg_launch_ms": solve_ms, "iterations_per_launch": solve_iters,
+                    "us_per_iteration": 1e3 * solve_ms / max(solve_iters, 1)}
         dominant["frac"] = dominant["achieved"] / HBM_PEAK_GBS
+        dominant["frac_algorithmic_jacobi"] = dominant["frac"]
+        if bj:
+            dominant["frac_algorithmic_block_jacobi"] = solve_iters * it_block / secs / 1e9 / HBM_PEAK_GBS
+        # HBM bytes the counters saw (FETCH_SIZE + WRITE_SIZE, copy-calibrated) over the
+        # same launch time: the fraction of the HBM peak the kernel really moves
+        dominant["frac_counter"] = traffic / secs / 1e9 / HBM_PEAK_GBS if traffic else None
     cloth.close()
+    guesses = None
     if net is not None:
         cloth, dt_warm, it_warm = timed(True)
         cloth.close()
         mlp_ms = net.time_forward(spec["width"] * spec["length"], 100)
+        if args.cg == "persistent":
+            # the free guesses the network has to beat, same preconditioner and tolerance
+            guesses = {"zero": {"steps_per_s": args.steps / dt_plain, "cg_iters_per_step": it_plain / args.steps},
+                       "mlp": {"steps_per_s": args.steps / dt_warm, "cg_iters_per_step": it_warm / args.steps}}
+            for g in ("prev", "extrap"):
+                cg_, dt_g, it_g = timed(False, guess=g)
+                cg_.close()
+                guesses[g] = {"steps_per_s": args.steps / dt_g, "cg_iters_per_step": it_g / args.steps}
     else:
         dt_warm, it_warm, mlp_ms = dt_plain, it_plain, None
     mlp_tflops = MLP_FLOP_PER_NODE * spec["width"] * spec["length"] / (mlp_ms * 1e-3) / 1e12 if net else None
@@ -324,6 +352,8 @@ def main():
             "warm_over_plain": steps_per_s / plain_steps_per_s,
             # against the reference's own method: zero-guess Jacobi-PCG (persistent form)
             "warm_over_plain_jacobi": steps_per_s / jacobi["steps_per_s"] if jacobi else None,
+            "initial_guesses": guesses,
+            "mlp_over_extrapolation": (guesses["mlp"]["steps_per_s"] / guesses["extrap"]["steps_per_s"]) if guesses else None,
             "plain_cg_reference_form": ref_form,
             "plain_cg_jacobi_persistent": jacobi,
             "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -338,9 +368,7 @@ def main():
         out["roofline"] = spmv
     else:
         out["config"]["spmv"] = spmv
-    stream = hbm_stream_measure()
-    stream["roofline_over_stream"] = out["roofline"]["achieved"] / stream["achieved"]
-    out["config"]["hbm_stream"] = stream
+    out["config"]["hbm_stream"] = hbm_stream_measure()
     if args.batch_steps > 0 and not args.plain_only:
         # Jacobi: two solve blocks per CU, twice the resident tiles of the block-Jacobi form,
         # which makes the batch faster despite the iterations (6.6 ms against 7.1 ms per step)
@@ -351,6 +379,13 @@ def main():
         out["config"]["tangents"] = tangent_measure(ys, spec, state, args.tangent_steps, args.precond)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cs, args.workload, args.cpu_steps)
+    # release every HIP object before the interpreter shuts down (no destructor runs
+    # during teardown, after a profiler's own exit handlers)
+    if net is not None:
+        net.close()
+    import torch
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
 

@@ -58,7 +58,7 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_batch_create", "yc_batch_destroy", "yc_batch_set_opts", "yc_batch_get_opts", "yc_batch_run",
            "yc_batch_info", "yc_time_assemble", "yc_tangent_set_ext", "yc_set_preconditioner",
            "yc_set_state_device", "yc_get_state_device", "yc_export_precond", "yc_export_matrix",
-           "yc_load_system", "yc_update_vel"]
+           "yc_load_system", "yc_update_vel", "yc_run_guess"]
 
 
 def lib_path():
@@ -110,6 +110,8 @@ def load_library():
     L.yc_step.argtypes = [vp, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]
     L.yc_run.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
     L.yc_time_spmv.argtypes = [vp, ctypes.c_int, dp]
+    L.yc_run_guess.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_longlong)]
     L.yc_set_cg_variant.argtypes = [vp, ctypes.c_int]
     L.yc_set_preconditioner.argtypes = [vp, ctypes.c_int]
     L.yc_set_state_device.argtypes = [vp] + [ctypes.c_void_p] * 7

@@ -204,6 +204,20 @@ class Cloth:
             _abi.check(rc)
         return tot.value, rc
 
+    GUESSES = {"zero": 0, "prev": 1, "extrap": 2}
+
+    def run_guess(self, guess, h, steps, tol=1e-8, maxit=None):
+        """run() from a cheap initial guess: "prev" (v_n) or "extrap" (2 v_n - v_{n-1});
+        persistent CG only.  Returns (total CG iterations, status)."""
+        if maxit is None:
+            maxit = 2 * self.n
+        tot = ctypes.c_longlong()
+        rc = _abi.load_library().yc_run_guess(self._h, self.GUESSES[guess], float(h), int(steps), float(tol), int(maxit),
+                                              ctypes.byref(tot))
+        if rc not in (_abi.YC_OK, _abi.YC_ERR_NOT_CONVERGED):
+            _abi.check(rc)
+        return tot.value, rc
+
     # --- learned warm start ---
     def features(self):
         out = np.zeros((self.N, 32), dtype=np.float32)

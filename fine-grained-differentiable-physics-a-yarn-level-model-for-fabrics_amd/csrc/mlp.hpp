@@ -349,6 +349,13 @@ __global__ void __launch_bounds__(256) k_warm_x0(const double* vpad, const float
 }
 
 // dv of the last solve relative to the velocity it started from
+// x0 = v_old (kind 1) or the linear extrapolation v_old + (v_old - v_older) (kind 2) of
+// the velocities, from the padded previous velocity and the last step's change
+__global__ void __launch_bounds__(256) k_guess_x0(const double* vpad, const double* dprev, int kind, int N5, double* x0) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N5) x0[i] = kind == 2 ? vpad[i] + dprev[i] : vpad[i];
+}
+
 __global__ void __launch_bounds__(256) k_dprev(const double* x, const double* vpad, int N5, double* dprev) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N5) dprev[i] = x[i] - vpad[i];

@@ -2381,6 +2381,8 @@ int yc_set_state(yc_ctx* c, const double* lpos, const double* lposfix, const dou
     HIPCHK(hipMemcpyAsync(c->uv, d.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->uvbar, e.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->uvd, f.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // a new state has no previous step: the extrapolated guesses start from v alone
+    HIPCHK(hipMemsetAsync(c->dprev, 0, (size_t)5 * c->P * sizeof(double), c->stream));
     int rc = refresh_geometry(c);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2408,6 +2410,7 @@ int yc_set_state_device(yc_ctx* c, const double* lpos, const double* lposfix, co
     StateIO io = {{lpos, lposfix, lvel, epos, eposbar, evel}, {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
     hipLaunchKernelGGL(k_state_in, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), io);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(c->dprev, 0, (size_t)5 * c->P * sizeof(double), c->stream));
     rc = refresh_geometry(c);
     if (rc) return rc;
     c->assembled = c->solved = false;
@@ -2766,14 +2769,22 @@ int warm_guess(yc_ctx* c, yc_mlp* m);
 
 // Steps back to back with the persistent CG and no host synchronisation until the end:
 // assembly flags and the solves' outcomes accumulate on the device.
-int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters) {
+// guess (without a network): YC_GUESS_ZERO, YC_GUESS_PREV or YC_GUESS_EXTRAP.
+int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters,
+              int guess = YC_GUESS_ZERO) {
     g_err.clear();
     HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync((char*)c->sc + offsetof(CGScal, total_iters), 0, sizeof(long long) + 2 * sizeof(int), c->stream));
+    const bool x0 = m != nullptr || guess != YC_GUESS_ZERO;
     for (int s = 0; s < steps; ++s) {
         int rc = do_assemble(c, h, true);
         if (!rc && m) rc = warm_guess(c, m);
-        if (!rc) rc = do_solve(c, nullptr, tol, maxit, nullptr, nullptr, m != nullptr, true);
+        if (!rc && !m && guess != YC_GUESS_ZERO) {
+            hipLaunchKernelGGL(k_guess_x0, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->vpad,
+                               (const double*)c->dprev, guess, 5 * c->P, c->cx);
+            HIPCHK(hipGetLastError());
+        }
+        if (!rc) rc = do_solve(c, nullptr, tol, maxit, nullptr, nullptr, x0, true);
         if (!rc) rc = do_update(c, h);
         if (rc) return rc;
     }
@@ -2810,6 +2821,13 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
     }
     if (total_iters) *total_iters = tot;
     return worst;
+}
+
+int yc_run_guess(yc_ctx* c, int guess, double h, int steps, double tol, int maxit, long long* total_iters) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (guess != YC_GUESS_ZERO && guess != YC_GUESS_PREV && guess != YC_GUESS_EXTRAP) return fail(YC_ERR_ARG, "unknown guess");
+    if (c->cg_variant != YC_CG_PERSISTENT || !c->coop_ok) return fail(YC_ERR_ARG, "yc_run_guess runs the persistent CG");
+    return run_async(c, nullptr, h, steps, tol, maxit, total_iters, guess);
 }
 
 int yc_tangent_end(yc_ctx* c) {

@@ -203,6 +203,17 @@ int yc_export_precond(yc_ctx* ctx, double* rows);
  * the other variants synchronise once per CG chunk and stop at the first NaN/Inf. */
 int yc_run(yc_ctx* ctx, double h, int steps, double tol, int maxit, long long* total_iters);
 
+/* yc_run (persistent CG only) with a cheap initial guess in place of the zero one:
+ * YC_GUESS_PREV the previous velocity v_n, YC_GUESS_EXTRAP its linear extrapolation
+ * 2 v_n - v_{n-1} (v_{n-1} from the previous step of this context; zero before the
+ * first).  CG refines either to the same tolerance.  No counterpart in the reference
+ * (its solver starts from zero, src/Solvers.cpp:22-24): the baselines a learned warm
+ * start (yc_run_warm) is measured against. */
+#define YC_GUESS_ZERO 0
+#define YC_GUESS_PREV 1
+#define YC_GUESS_EXTRAP 2
+int yc_run_guess(yc_ctx* ctx, int guess, double h, int steps, double tol, int maxit, long long* total_iters);
+
 /* Times `reps` launches of the CG loop's stencil product y = A p on the context's
  * stream with HIP events; avg_ms receives the mean duration of one launch.  Call after
  * yc_assemble; it overwrites the CG workspace. */
