@@ -58,7 +58,7 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_batch_create", "yc_batch_destroy", "yc_batch_set_opts", "yc_batch_get_opts", "yc_batch_run",
            "yc_batch_info", "yc_time_assemble", "yc_tangent_set_ext", "yc_set_preconditioner",
            "yc_set_state_device", "yc_get_state_device", "yc_export_precond", "yc_export_matrix",
-           "yc_load_system", "yc_update_vel", "yc_run_guess"]
+           "yc_load_system", "yc_update_vel", "yc_run_guess", "yc_mlp_forward_noalloc"]
 
 
 def lib_path():
@@ -138,6 +138,7 @@ def load_library():
     L.yc_mlp_set_norm.argtypes = [vp, fp, fp, fp]
     L.yc_mlp_set_data.argtypes = [vp, fp, fp, fp, ctypes.c_int]
     L.yc_mlp_forward.argtypes = [vp, fp, ctypes.c_int, fp]
+    L.yc_mlp_forward_noalloc.argtypes = [vp, fp, ctypes.c_int, fp]
     L.yc_mlp_grad.argtypes = [vp, ip, ctypes.c_int, fp, fp]
     L.yc_mlp_train.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp]
     L.yc_mlp_time_forward.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]

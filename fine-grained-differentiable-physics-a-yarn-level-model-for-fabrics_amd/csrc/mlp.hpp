@@ -376,4 +376,6 @@ struct yc_mlp {
     hipStream_t stream;
     // inference scratch for one context at a time
     float* F; float* Yi; int capN;
+    // this device's copy of the staging array of yc_mlp_forward_noalloc (not owned)
+    float* scratch;
 };

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include <type_traits>
@@ -3118,6 +3119,8 @@ int mlp_refresh_bf16(yc_mlp* m) {
     return YC_OK;
 }
 
+int noalloc_scratch_dev(float** out);
+
 MlpW mlp_w(const yc_mlp* m) {
     MlpW w;
     w.W1 = m->W1; w.W2 = m->W2; w.W3 = m->W3; w.W4 = m->W4; w.b = m->param;
@@ -3206,6 +3209,7 @@ int yc_mlp_create(yc_mlp** out) {
     chk(dalloc(&m->W1, MLP_H * MLP_IN)); chk(dalloc(&m->W2, MLP_H * MLP_H)); chk(dalloc(&m->W3, MLP_H * MLP_H));
     chk(dalloc(&m->W4, MLP_OUTP * MLP_H));
     chk(dalloc(&m->in_mean, MLP_IN)); chk(dalloc(&m->in_istd, MLP_IN)); chk(dalloc(&m->out_std, MLP_OUT));
+    chk(noalloc_scratch_dev(&m->scratch));   // the device's staging of yc_mlp_forward_noalloc
     if (rc) { yc_mlp_destroy(m); return rc; }
     std::vector<float> one(MLP_IN, 1.0f);
     if (hipMemcpy(m->in_istd, one.data(), MLP_IN * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -3288,6 +3292,65 @@ int yc_mlp_forward(yc_mlp* m, const float* X, int rows, float* Y) {
     HIPCHK(hipStreamSynchronize(m->stream));
     (void)hipFree(dX);
     (void)hipFree(dY);
+    return YC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Staging of yc_mlp_forward_noalloc: the features of up to MLP_NOALLOC_ROWS rows, then
+// their outputs, in one file-scope device array.  Each device holds its own copy for the
+// life of the process; host code reaches a device's copy only through the address
+// hipGetSymbolAddress returns on that device (noalloc_scratch_dev), never through the
+// array's name.
+constexpr int MLP_NOALLOC_ROWS = 1 << 16;
+constexpr size_t MLP_NOALLOC_FLOATS = (size_t)MLP_NOALLOC_ROWS * (MLP_IN + MLP_OUT);   // 9.7 MB
+
+}  // namespace
+
+static __device__ __align__(256) float scratch[MLP_NOALLOC_FLOATS];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[MAX_DEVICES];
+
+// The current device's copy of `scratch`, resolved once per device.
+int noalloc_scratch_dev(float** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(YC_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (float*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return YC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yc_mlp_forward_noalloc(yc_mlp* m, const float* X, int rows, float* Y) {
+    if (!m || !X || !Y) return fail(YC_ERR_ARG, "null argument");
+    if (rows <= 0 || rows > MLP_NOALLOC_ROWS)
+        return fail(YC_ERR_ARG, "yc_mlp_forward_noalloc: rows must be 1.." + std::to_string(MLP_NOALLOC_ROWS));
+    if (!m->scratch) return fail(YC_ERR_ARG, "yc_mlp_forward_noalloc: the network has no staging (created on another device?)");
+    float* dX = m->scratch;
+    float* dY = m->scratch + (size_t)MLP_NOALLOC_ROWS * MLP_IN;
+    HIPCHK(hipMemcpyAsync(dX, X, (size_t)rows * MLP_IN * 4, hipMemcpyHostToDevice, m->stream));
+    MlpFwdArgs f;
+    f.X = dX; f.idx = nullptr; f.rows = rows; f.in_mean = m->in_mean; f.in_istd = m->in_istd; f.w = mlp_w(m);
+    f.Y = dY; f.sX0 = nullptr; f.sZ = nullptr; f.sH = nullptr;
+    hipLaunchKernelGGL(k_mlp_fwd, dim3((rows + MLP_TILE - 1) / MLP_TILE), dim3(256), 0, m->stream, f);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Y, dY, (size_t)rows * MLP_OUT * 4, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
     return YC_OK;
 }
 

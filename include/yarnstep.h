@@ -259,6 +259,14 @@ int yc_mlp_set_norm(yc_mlp* m, const float* in_mean /*32*/, const float* in_std 
 int yc_mlp_set_data(yc_mlp* m, const float* X, const float* T, const float* M, int rows);
 /* Inference on host features: Y rows x 5 (units of out_std). */
 int yc_mlp_forward(yc_mlp* m, const float* X, int rows, float* Y);
+/* yc_mlp_forward without any device allocation: the features and outputs are staged in
+ * a file-scope device array of the library (one per device, held until the process
+ * ends) instead of buffers allocated per call, for callers that cannot allocate (tight
+ * serving loops, code that must not call hipMalloc).  No counterpart in the reference.
+ * rows at most 65,536 (YC_ERR_ARG beyond); results equal yc_mlp_forward's bit for bit.
+ * Calls on one device share the staging array: calls that overlap on one device (from
+ * several threads or networks at once) race and must be serialised by the caller. */
+int yc_mlp_forward_noalloc(yc_mlp* m, const float* X, int rows, float* Y);
 /* Loss (mean of the masked squared error) and its gradient on dataset rows idx[0..rows). */
 int yc_mlp_grad(yc_mlp* m, const int* idx, int rows, float* grad, float* loss);
 /* Adam over idx[0..rows) in batches of `batch` rows (remainder dropped); mean batch loss out. */
