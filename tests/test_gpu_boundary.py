@@ -68,6 +68,7 @@ def test_system_matrices_match_oracle(ys, cs, idx):
     phy = ys.PhysicsCloth(g)
     phy.FillMV(h)
     r.assemble(h)
+    A0, b0 = phy.get_SpaA(), g.vec("b")
     M, Kx, Kv = phy.get_SpaM(), phy.get_SpaFPos(), phy.get_SpaFVel()
     eM, eKx, eKv = frel(M, r.csr(1)), frel(Kx, r.csr(2)), frel(Kv, r.csr(3))
     say("  |dM|=%.2e |dKx|=%.2e |dKv|=%.2e" % (eM, eKx, eKv))
@@ -79,9 +80,20 @@ def test_system_matrices_match_oracle(ys, cs, idx):
     assert rel(b, g.vec("b")) < 1e-12
     np.testing.assert_array_equal(phy.get_SpaPos(), _global(state, spec, "pos"))
     np.testing.assert_array_equal(v, _global(state, spec, "vel"))
-    # the exports leave the assembled system in place: a solve still matches the oracle's
+    # the exports leave the assembled system in place: A and b bit for bit
+    assert (A0 != phy.get_SpaA()).nnz == 0
+    assert np.array_equal(b0, g.vec("b"))
+    g.close()
+    # and a solve after them matches the oracle's (from the rest pose: the excited states
+    # above are not positive definite)
+    g, r = _pair(ys, spec, cs.initial_state(spec))
+    phy = ys.PhysicsCloth(g)
+    phy.FillMV(h)
+    r.assemble(h)
+    phy.get_SpaM(), phy.get_SpaFPos(), phy.get_SpaFVel()
     it, rr, ok = g.solve(tol=1e-12)
-    x_r, _, _, _ = r.solve(tol=1e-12, maxit=20000)
+    x_r, it_r, _, _ = r.solve(tol=1e-12, maxit=20000)
+    say("  cg iters gpu %d oracle %d relres %.2e" % (it, it_r, rr))
     assert ok and rel(g.vec("x"), x_r) < 1e-9
     g.close()
 
