@@ -67,6 +67,10 @@ def lib():
         L.yo_freeze_shear_angle.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.yo_set_frozen_shear_angle.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
         L.yo_get_frozen_shear_angle.argtypes = [ctypes.c_void_p, dp]
+        L.yo_set_terms.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        L.yo_set_element.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.yo_friction_eval.argtypes = [dp, dp]
+        L.yo_get_compression_parts.argtypes = [ctypes.c_void_p, dp, dp]
         _lib = L
     return _lib
 
@@ -184,6 +188,27 @@ class OracleCloth:
         st = lib().yo_solve(self._h, float(tol), int(maxit), _dp(x0a), _dp(x), ctypes.byref(it), ctypes.byref(rr))
         return x, it.value, rr.value, st
 
+    TERMS = {"mass": 1, "gravity": 2, "stretch": 4, "inertia": 8, "mdot": 16, "paracolli": 32, "bend": 64,
+             "crimp": 128, "friction": 256, "shear": 512, "wind": 1024, "handle": 2048}
+
+    def set_terms(self, names=None):
+        """Evaluate only the named element terms in the following assemblies (None: all)."""
+        mask = 4095 if names is None else sum(self.TERMS[n] for n in names)
+        lib().yo_set_terms(self._h, mask)
+
+    ELEMENTS = {"all": 0, "edge": 1, "bend": 2, "shear": 3, "face": 4, "crossing": 5}
+
+    def set_element(self, kind="all", index=-1):
+        """Assemble only element `index` of the named list (edge, bend, shear, face) or the
+        friction of crossing `index`; "all" restores the full assembly."""
+        lib().yo_set_element(self._h, self.ELEMENTS[kind], int(index))
+
+    def compression_parts(self):
+        """StrhBend_C_L per node (N, 3) and the self StrhBend_C_LE entries (N, 2, 3)."""
+        a, b = np.zeros((self.N, 3)), np.zeros((self.N, 2, 3))
+        lib().yo_get_compression_parts(self._h, _dp(a), _dp(b))
+        return a, b
+
     def freeze_shear_angle(self, on=True):
         """Hold the non-jamming shear's (phi - pi/2) at the current state's values."""
         lib().yo_freeze_shear_angle(self._h, 1 if on else 0)
@@ -209,6 +234,15 @@ class OracleCloth:
         rr = ctypes.c_double(0)
         st = lib().yo_step(self._h, float(h), _dp(ef), float(tol), int(maxit), ctypes.byref(it), ctypes.byref(rr))
         return st, it.value, rr.value
+
+
+def friction_eval(dlt, comp, compEE, FNS, Fpt, Fpos, Fvel, uvd, mu, kf, df):
+    """The oracle's ConSlideFriction arithmetic for one component -> (is_slide, force,
+    EE position entry, EE velocity entry)."""
+    i = np.array([dlt, comp, compEE, FNS, Fpt, Fpos, Fvel, uvd, mu, kf, df], dtype=np.float64)
+    o = np.zeros(4)
+    lib().yo_friction_eval(_dp(i), _dp(o))
+    return tuple(o)
 
 
 def owner(n_iter, i):

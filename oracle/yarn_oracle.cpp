@@ -229,6 +229,12 @@ struct yo_cloth {
                      double E0E0, double E0E1, double E1E0, double E1E1);
 
     int assemble(double h, const double* ext);
+    // checker option: which element terms an assembly evaluates (YO_TERM_* bits)
+    unsigned terms = YO_TERM_ALL;
+    bool on(unsigned t) const { return (terms & t) != 0; }
+    // checker option: evaluate one element of one list only (YO_ELEM_* kind, list index)
+    int only_kind = 0, only_index = -1;
+    bool pick(int kind, int i) const { return only_kind == 0 || (only_kind == kind && only_index == i); }
 };
 
 // ---------------------------------------------------------------------------
@@ -838,6 +844,29 @@ void yo_cloth::Crimp_Bending(const Bend& bd) {
 // Friction, shear, wind, handles
 // ---------------------------------------------------------------------------
 
+// One component (u or v) of ConSlideFriction's arithmetic (src/PhysicsCloth.cpp:2619-2633):
+// inputs uv - uvbar, the compression and its EE input, the no-slide E force, the
+// thread-partial E force (Q2), the no-slide EE Jacobian diagonals, the Eulerian velocity
+// and the cloth's mu, kf, df.
+struct FricIn { double dlt, comp, compEE, FNS, Fpt, Fpos, Fvel, uvd, mu, kf, df; };
+struct FricOut { double is_slide, fric, fpos, fvel; };
+FricOut friction_eval(const FricIn& in) {
+    FricOut o;
+    const double comp = in.comp, mu = in.mu, kf = in.kf, df = in.df;
+    const double lim = comp * mu;
+    const double th = std::tanh(10 * in.dlt);
+    o.is_slide = sigm(1e4 * (std::fabs(in.FNS) - lim));
+    o.fric = (-0.5 * (kf * in.dlt - th * comp * mu) * std::tanh(10 * (lim - in.Fpt)) + 0.5 * (kf * in.dlt + th * comp * mu)) -
+             df * in.uvd;
+    const double tl = std::tanh(10 * (lim - in.FNS));
+    o.fpos = (-0.5 * (kf - (1.0 - std::pow(th, 2)) * mu * comp - th * mu * in.compEE)) * tl -
+             (0.5 * (kf * in.dlt - th * mu * comp)) * (1.0 - std::pow(tl, 2)) * (in.Fpos - mu * in.compEE) -
+             (0.5 * (kf + (1.0 - std::pow(th, 2)) * mu * comp + th * mu * in.compEE));
+    const double tq = std::tanh(10 * lim - in.FNS);   // (10*fric_lim - F)[k] as written (Q3)
+    o.fvel = (-0.5 * (kf * in.dlt - th * mu * comp)) * (1.0 - std::pow(tq, 2)) * in.Fvel - df;
+    return o;
+}
+
 // ConSlideFriction (src/PhysicsCloth.cpp:2579-2641) for the interior crossing of node k
 void yo_cloth::ConSlideFriction(int k) {
     int e = eofk(k);
@@ -851,25 +880,15 @@ void yo_cloth::ConSlideFriction(int k) {
     }
     double compression = relu(0.5 * dot(nrm[k], StrhBendC_L[k]));
     comp[e] = compression;
-    double mu = p.mu, kf = p.kf, df = p.df;
-    double lim = compression * mu;
     for (int c = 0; c < 2; ++c) {
-        double FNS = FE_noslide[2 * e + c];
-        double Fpt = FEpart[2 * e + c];
-        double th = std::tanh(10 * dlt[c]);
-        is_slide[2 * e + c] = sigm(1e4 * (std::fabs(FNS) - lim));
-        double fric = (-0.5 * (kf * dlt[c] - th * compression * mu) * std::tanh(10 * (lim - Fpt)) +
-                       0.5 * (kf * dlt[c] + th * compression * mu)) - df * uvd[k][c];
-        double tl = std::tanh(10 * (lim - FNS));
-        double fpos = (-0.5 * (kf - (1.0 - std::pow(th, 2)) * mu * compression - th * mu * compEE[c])) * tl -
-                      (0.5 * (kf * dlt[c] - th * mu * compression)) * (1.0 - std::pow(tl, 2)) * (Fpos[c] - mu * compEE[c]) -
-                      (0.5 * (kf + (1.0 - std::pow(th, 2)) * mu * compression + th * mu * compEE[c]));
-        double tq = std::tanh(10 * lim - FNS);   // (10*fric_lim - F)[k] as written (Q3)
-        double fvel = (-0.5 * (kf * dlt[c] - th * mu * compression)) * (1.0 - std::pow(tq, 2)) * Fvel[c] - df;
-        FE[2 * e + c] += fric;
+        FricIn in{dlt[c], compression, compEE[c], FE_noslide[2 * e + c], FEpart[2 * e + c], Fpos[c], Fvel[c], uvd[k][c],
+                  p.mu, p.kf, p.df};
+        FricOut o = friction_eval(in);
+        is_slide[2 * e + c] = o.is_slide;
+        FE[2 * e + c] += o.fric;
         int r = 3 * N + 2 * e + c;
-        Kx.add(r, r, fpos);
-        Kv.add(r, r, fvel);
+        Kx.add(r, r, o.fpos);
+        Kv.add(r, r, o.fvel);
     }
 }
 
@@ -1090,41 +1109,48 @@ int yo_cloth::assemble(double h, const double* ext) {
     for (auto& s : StrhBendC_LEself) s = {V3{0, 0, 0}, V3{0, 0, 0}};
     M.zero(); Kx.zero(); Kv.zero();
     // src/PhysicsCloth.cpp:56-62
-    for (int k = 0; k < N; ++k) Constraint(k);
+    if (on(YO_TERM_HANDLE) && only_kind == 0) for (int k = 0; k < N; ++k) Constraint(k);
     // src/PhysicsCloth.cpp:64-80 (edge loop; t = owner thread of edge i)
     int ne = (int)edges.size();
     for (int i = 0; i < ne; ++i) {
+        if (!pick(YO_ELEM_EDGE, i)) continue;
         const Edge& e = edges[i];
         int t = owner(ne, i);
         EdgeGeo g = edge_geo(e);
-        Mass(e, g);
-        Gravity_F(e, g, t);
-        Stretch_F(e, g, t);
-        Inertia_F(e, g, t);
-        MDotVDot_F(e, g, t);
-        ParaColli_F(e, g, t);
-        Gravity_D(e, g);
-        Stretch_D(e, g);
-        Inertia_D_Pos(e, g);
-        Inertia_D_Vel(e, g);
-        MDotVDot_D_Pos(e, g);
-        MDotVDot_D_Vel(e, g);
-        ParaColli_D(e, g);
+        if (on(YO_TERM_MASS)) Mass(e, g);
+        if (on(YO_TERM_GRAVITY)) Gravity_F(e, g, t);
+        if (on(YO_TERM_STRETCH)) Stretch_F(e, g, t);
+        if (on(YO_TERM_INERTIA)) Inertia_F(e, g, t);
+        if (on(YO_TERM_MDOT)) MDotVDot_F(e, g, t);
+        if (on(YO_TERM_PARACOLLI)) ParaColli_F(e, g, t);
+        if (on(YO_TERM_GRAVITY)) Gravity_D(e, g);
+        if (on(YO_TERM_STRETCH)) Stretch_D(e, g);
+        if (on(YO_TERM_INERTIA)) { Inertia_D_Pos(e, g); Inertia_D_Vel(e, g); }
+        if (on(YO_TERM_MDOT)) { MDotVDot_D_Pos(e, g); MDotVDot_D_Vel(e, g); }
+        if (on(YO_TERM_PARACOLLI)) ParaColli_D(e, g);
     }
     // src/PhysicsCloth.cpp:82-89
     int nb = (int)bends.size();
     for (int i = 0; i < nb; ++i) {
-        Bending(bends[i], owner(nb, i));
-        Crimp_Bending(bends[i]);
+        if (!pick(YO_ELEM_BEND, i)) continue;
+        if (on(YO_TERM_BEND)) Bending(bends[i], owner(nb, i));
+        if (on(YO_TERM_CRIMP)) Crimp_Bending(bends[i]);
     }
     // src/PhysicsCloth.cpp:91-95
-    for (const Face& f : faces) Wind_F(f);
+    if (on(YO_TERM_WIND))
+        for (size_t i = 0; i < faces.size(); ++i)
+            if (pick(YO_ELEM_FACE, (int)i)) Wind_F(faces[i]);
     // src/PhysicsCloth.cpp:97-112: no-slide snapshots (the EE diagonals are read from Kx/Kv directly)
     FE_noslide = FE;
-    // src/PhysicsCloth.cpp:114-123
-    for (int j = 1; j < Lg - 1; ++j) for (int i = 1; i < W - 1; ++i) ConSlideFriction(kidx(i, j));
+    // src/PhysicsCloth.cpp:114-123 (with the friction term off, the compression is still kept)
+    for (int j = 1; j < Lg - 1; ++j) for (int i = 1; i < W - 1; ++i) {
+        if (on(YO_TERM_FRICTION) && pick(YO_ELEM_CROSSING, eidx(i, j))) ConSlideFriction(kidx(i, j));
+        else comp[eidx(i, j)] = relu(0.5 * dot(nrm[kidx(i, j)], StrhBendC_L[kidx(i, j)]));
+    }
     // src/PhysicsCloth.cpp:125-131
-    for (const Shear& s : shears) Shear_FD(s);
+    if (on(YO_TERM_SHEAR))
+        for (size_t i = 0; i < shears.size(); ++i)
+            if (pick(YO_ELEM_SHEAR, (int)i)) Shear_FD(shears[i]);
 
     // CSR pattern: union of the M, Kx, Kv rows, sorted
     rowptr.assign(n + 1, 0);
@@ -1231,6 +1257,24 @@ void yo_set_state(yo_cloth* c, const double* lpos, const double* lposfix, const 
 }
 
 int yo_assemble(yo_cloth* c, double h, const double* ext_force) { return c->assemble(h, ext_force); }
+
+void yo_set_terms(yo_cloth* c, unsigned terms) { c->terms = terms; }
+
+void yo_set_element(yo_cloth* c, int kind, int index) { c->only_kind = kind; c->only_index = index; }
+
+void yo_friction_eval(const double* in, double* out) {
+    FricIn i{in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], in[8], in[9], in[10]};
+    FricOut o = friction_eval(i);
+    out[0] = o.is_slide; out[1] = o.fric; out[2] = o.fpos; out[3] = o.fvel;
+}
+
+void yo_get_compression_parts(const yo_cloth* c, double* sbc_l, double* sbc_le) {
+    for (int k = 0; k < c->N; ++k)
+        for (int a = 0; a < 3; ++a) {
+            if (sbc_l) sbc_l[3 * k + a] = c->StrhBendC_L[k][a];
+            if (sbc_le) for (int q = 0; q < 2; ++q) sbc_le[6 * k + 3 * q + a] = c->StrhBendC_LEself[k][q][a];
+        }
+}
 
 void yo_freeze_shear_angle(yo_cloth* c, int on) {
     c->frozen_pb.clear();

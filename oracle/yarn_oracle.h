@@ -55,6 +55,45 @@ void yo_set_state(yo_cloth* c, const double* lpos, const double* lposfix, const 
  * ext_force may be NULL (n doubles otherwise).  Returns 0, or 1 on NaN/Inf in A or b. */
 int yo_assemble(yo_cloth* c, double h, const double* ext_force);
 
+/* Checker option: the element terms yo_assemble evaluates (default all), so that a test
+ * can pin one term's Jacobian against differences of that term's force alone.  With
+ * YO_TERM_FRICTION off the compression is still computed (the shear reads it). */
+#define YO_TERM_MASS 1u
+#define YO_TERM_GRAVITY 2u
+#define YO_TERM_STRETCH 4u
+#define YO_TERM_INERTIA 8u
+#define YO_TERM_MDOT 16u
+#define YO_TERM_PARACOLLI 32u
+#define YO_TERM_BEND 64u
+#define YO_TERM_CRIMP 128u
+#define YO_TERM_FRICTION 256u
+#define YO_TERM_SHEAR 512u
+#define YO_TERM_WIND 1024u
+#define YO_TERM_HANDLE 2048u
+#define YO_TERM_ALL 4095u
+void yo_set_terms(yo_cloth* c, unsigned terms);
+/* Checker option: assemble one element only -- edge, bend (with its crimp twin), shear
+ * segment or face `index` of the reference's element lists (src/Cloth.cpp:167-305), or
+ * the friction of crossing `index` -- so that a test can take differences of that
+ * element's force alone.  YO_ELEM_ALL restores the full assembly; the handles are
+ * evaluated only then. */
+#define YO_ELEM_ALL 0
+#define YO_ELEM_EDGE 1
+#define YO_ELEM_BEND 2
+#define YO_ELEM_SHEAR 3
+#define YO_ELEM_FACE 4
+#define YO_ELEM_CROSSING 5
+void yo_set_element(yo_cloth* c, int kind, int index);
+/* ConSlideFriction's arithmetic for one crossing component, the function the assembly
+ * calls (src/PhysicsCloth.cpp:2619-2633).  in: uv - uvbar, compression, its EE input
+ * 0.5 n.StrhBend_C_LE, the no-slide E force, the thread-partial E force, the no-slide EE
+ * position and velocity Jacobian diagonals, the Eulerian velocity, mu, kf, df.
+ * out: is_slide, friction force, EE position entry, EE velocity entry. */
+void yo_friction_eval(const double* in /*11*/, double* out /*4*/);
+/* The compression accumulators of the last assembly: StrhBend_C_L per node (N x 3) and the
+ * self entries of StrhBend_C_LE per node and yarn direction (N x 2 x 3). */
+void yo_get_compression_parts(const yo_cloth* c, double* sbc_l, double* sbc_le);
+
 /* Results of the last yo_assemble. */
 int yo_nnz(const yo_cloth* c);
 /* Checker option for the reference's detached gradient: hold the non-jamming shear's
