@@ -25,20 +25,67 @@ BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_NODE = 2 * (32 * 128 + 128 * 128 * 2 + 128 * 5)   # 75,008
 
 
+def _cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
 def cpu_baseline(cs, name, nsteps):
-    """The CPU oracle (single thread) on the first nsteps steps of the same workload."""
+    """The CPU oracle (the reference's step restated in C++, FP64, the same Jacobi-PCG and
+    stopping rule) on steps 2..nsteps+1 of the same workload, on this host's cores: with
+    every core available to the process (capped by OMP_NUM_THREADS when it is set), with 8
+    threads (the reference's NThrd, src/Constants.h:26) and with one.  The assembly with
+    several threads is bit for bit the single-thread one.  Beside it, `ref_dense` times
+    the same oracle with the reference's dense storage work on a 25 x 25 cloth, the
+    largest size that layout handles (a lower bound of the reference's own cost)."""
     from oracle.oracle import OracleCloth
+    model, nproc, avail = _cpu_info()
+    env_t = os.environ.get("OMP_NUM_THREADS")
+    threads_all = min(avail, int(env_t)) if env_t and env_t.isdigit() else avail
     spec, state, ext, _ = cs.workload(name)
-    c = OracleCloth(spec)
-    c.set_state(state)
     h = spec["h"]
-    c.step(h, ext, tol=1e-8, maxit=20000)  # setup / first step untimed
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
-        c.step(h, ext, tol=1e-8, maxit=20000)
-    dt = time.perf_counter() - t0
-    return {"value": nsteps / dt, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": "steps 2..%d of the %s workload, oracle single-thread FP64 CG to 1e-8" % (nsteps + 1, name)}
+
+    def rate(threads, cspec, cstate, cext, steps, dense=False):
+        c = OracleCloth(cspec)
+        c.set_threads(threads)
+        c.set_state(cstate)
+        stepf = c.step_dense if dense else c.step
+        stepf(h, cext, tol=1e-8, maxit=20000)   # setup / first step untimed
+        it_tot = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st, it, _ = stepf(h, cext, tol=1e-8, maxit=20000)
+            it_tot += it
+        dt = time.perf_counter() - t0
+        return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "cg_iters_per_step": it_tot / steps,
+                "threads": threads}
+    main = rate(threads_all, spec, state, ext, nsteps)
+    t8 = rate(8, spec, state, ext, nsteps)
+    t1 = rate(1, spec, state, ext, max(2, nsteps // 2))
+    dspec = cs.heter_spec(25, "twill", 5)
+    dstate = cs.initial_state(dspec)
+    dense = rate(8, dspec, dstate, None, 3, dense=True)
+    sparse25 = rate(8, dspec, dstate, None, 3)
+    return {"value": main["steps_per_s"], "unit": "steps/s", "cores": threads_all, "kind": "port",
+            "sample": "steps 2..%d of the %s workload, oracle FP64 (block-sparse storage), Jacobi-PCG to 1e-8, "
+                      "%d OpenMP threads" % (nsteps + 1, name, threads_all),
+            "cpu_model": model, "nproc": nproc, "cores_available": avail,
+            "ms_per_step": main["ms_per_step"], "cg_iters_per_step": main["cg_iters_per_step"],
+            "threads_8": t8, "threads_1": t1,
+            "ref_dense_w25": {"workload": "25x25 two-yarn twill, steps 2..4, 8 threads", "dense_layout": dense,
+                              "block_sparse": sparse25,
+                              "dense_over_sparse_time": dense["ms_per_step"] / sparse25["ms_per_step"]}}
 
 
 def batch_measure(ys, steps, warmup=2, precond="block2"):

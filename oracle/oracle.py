@@ -70,6 +70,8 @@ def lib():
         L.yo_set_terms.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         L.yo_set_element.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.yo_friction_eval.argtypes = [dp, dp]
+        L.yo_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.yo_step_dense.argtypes = [ctypes.c_void_p, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int, ip, dp]
         L.yo_get_compression_parts.argtypes = [ctypes.c_void_p, dp, dp]
         _lib = L
     return _lib
@@ -203,6 +205,11 @@ class OracleCloth:
         friction of crossing `index`; "all" restores the full assembly."""
         lib().yo_set_element(self._h, self.ELEMENTS[kind], int(index))
 
+    def set_threads(self, n):
+        """OpenMP threads of the assembly (bit for bit the single-thread result), the
+        normals and the CG (chunked inner products when n > 1)."""
+        lib().yo_set_threads(self._h, int(n))
+
     def compression_parts(self):
         """StrhBend_C_L per node (N, 3) and the self StrhBend_C_LE entries (N, 2, 3)."""
         a, b = np.zeros((self.N, 3)), np.zeros((self.N, 2, 3))
@@ -225,6 +232,16 @@ class OracleCloth:
     def update(self, new_vel, h):
         nv = np.ascontiguousarray(new_vel, dtype=np.float64)
         lib().yo_update(self._h, _dp(nv), float(h))
+
+    def step_dense(self, h, ext_force=None, tol=1e-12, maxit=None):
+        """step() with the reference's dense storage work (cost model; same result)."""
+        if maxit is None:
+            maxit = 2 * self.n
+        ef = None if ext_force is None else np.ascontiguousarray(ext_force, dtype=np.float64)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0)
+        st = lib().yo_step_dense(self._h, float(h), _dp(ef), float(tol), int(maxit), ctypes.byref(it), ctypes.byref(rr))
+        return st, it.value, rr.value
 
     def step(self, h, ext_force=None, tol=1e-12, maxit=None):
         if maxit is None:

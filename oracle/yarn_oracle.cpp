@@ -18,11 +18,14 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <utility>
 #include <vector>
+
+#include <omp.h>
 
 namespace {
 
@@ -125,6 +128,13 @@ struct SpAcc {
     }
 };
 
+// Parallel assembly (checker option, yo_set_threads): thread t owns the grid rows
+// v in [tl_vlo, tl_vhi) and writes only the rows of A, F and the per-node accumulators that
+// belong to its nodes; every thread walks the element lists in the reference's order, so
+// each row receives its contributions in the serial order and the result is bit for bit
+// the serial one.
+thread_local int tl_vlo = 0, tl_vhi = INT_MAX;
+
 enum { EDGE_WARP = 0, EDGE_WEFT = 1 };
 
 struct Edge { int n0, n1, type, yarn; };            // src/ClothUnit.h:85-160
@@ -165,27 +175,37 @@ struct yo_cloth {
     int eofk(int k) const { return eidx(k % W, k / W); }
 
     // ----- scatter helpers: (buffer, find_index) -> global (row, col) -----
+    // the rows of node k belong to this thread (always true outside a parallel assembly)
+    bool own(int k) const { int v = k / W; return v >= tl_vlo && v < tl_vhi; }
     // LL[F,D][u][v] -> A[3F+v, 3D+u] (transposed block, src/MatrixMp.cpp:65-76)
     void addLL(SpAcc& S, int F, int D, const M3& B) {
+        if (!own(F)) return;
         for (int u = 0; u < 3; ++u) for (int v2 = 0; v2 < 3; ++v2) S.add(3 * F + v2, 3 * D + u, B[u][v2]);
     }
     // LE at find_index(L node kL, E node kE, "EL") + comp -> A[3kL+v, 3N+2e(kE)+comp] (src/MatrixMp.cpp:78-87)
     void addLE(SpAcc& S, int kL, int kE, int c, const V3& B) {
+        if (!own(kL)) return;
         int col = 3 * N + 2 * eofk(kE) + c;
         for (int v2 = 0; v2 < 3; ++v2) S.add(3 * kL + v2, col, B[v2]);
     }
     // EL at find_index(L node kL, E node kE, "EL") + comp -> A[3N+2e(kE)+comp, 3kL+u] (src/MatrixMp.cpp:89-98)
     void addEL(SpAcc& S, int kL, int kE, int c, const V3& B) {
+        if (!own(kE)) return;
         int row = 3 * N + 2 * eofk(kE) + c;
         for (int u = 0; u < 3; ++u) S.add(row, 3 * kL + u, B[u]);
     }
     // EE at find_index(F, D, "EE", uu|vv) -> A[3N+2e(F)+c, 3N+2e(D)+c] (src/PhysicsCloth.cpp:205-209)
     void addEE(SpAcc& S, int kF, int kD, int c, double val) {
+        if (!own(kF)) return;
         S.add(3 * N + 2 * eofk(kF) + c, 3 * N + 2 * eofk(kD) + c, val);
     }
-    void addFL(int k, const V3& f) { for (int a = 0; a < 3; ++a) FL[3 * k + a] += f[a]; }
+    void addFL(int k, const V3& f) { if (!own(k)) return; for (int a = 0; a < 3; ++a) FL[3 * k + a] += f[a]; }
+    // the compression accumulators StrhBend_C_L and the self entries of StrhBend_C_LE
+    void addSBC(int k, const V3& f) { if (own(k)) StrhBendC_L[k] = add(StrhBendC_L[k], f); }
+    void addSBCLE(int k, int c, const V3& f) { if (own(k)) StrhBendC_LEself[k][c] = add(StrhBendC_LEself[k][c], f); }
     // E force on crossing of node k, component c; from_elem marks the Q2 side channel
     void addFE(int k, int c, double f, int owner_t) {
+        if (!own(k)) return;
         int e = eofk(k);
         FE[2 * e + c] += f;
         if (owner_t >= 0 && owner_t == owner(Nc, e)) FEpart[2 * e + c] += f;
@@ -229,6 +249,8 @@ struct yo_cloth {
                      double E0E0, double E0E1, double E1E0, double E1E1);
 
     int assemble(double h, const double* ext);
+    template <class Mid> void element_loops(Mid mid);
+    int threads = 1;                  // checker option: OpenMP threads of assembly and CG
     // checker option: which element terms an assembly evaluates (YO_TERM_* bits)
     unsigned terms = YO_TERM_ALL;
     bool on(unsigned t) const { return (terms & t) != 0; }
@@ -357,6 +379,7 @@ V3 yo_cloth::node_norm(int kc, int kl, int kr, int kt, int kb2) const {
 void yo_cloth::normals_and_crimp() {
     // src/Cloth.cpp:313-325: crossing normals and the +-R*n crimp copies
     for (int k = 0; k < N; ++k) { xwarp[k] = x[k]; xweft[k] = x[k]; }
+#pragma omp parallel for num_threads(threads) if (threads > 1) schedule(static)
     for (int i = 1; i < W - 1; ++i) for (int j = 1; j < Lg - 1; ++j) {
         int k = kidx(i, j);
         V3 nr = node_norm(k, kidx(i - 1, j), kidx(i + 1, j), kidx(i, j - 1), kidx(i, j + 1));
@@ -462,8 +485,8 @@ void yo_cloth::Stretch_F(const Edge& e, const EdgeGeo& g, int t) {
     double nw = norm(g.w);
     V3 f = scl(g.ks * (nw - 1), g.d);
     addFL(e.n0, f); addFL(e.n1, neg(f));
-    if (e.type == EDGE_WARP) { StrhBendC_L[e.n0] = add(StrhBendC_L[e.n0], f); StrhBendC_L[e.n1] = sub(StrhBendC_L[e.n1], f); }
-    else { StrhBendC_L[e.n0] = sub(StrhBendC_L[e.n0], f); StrhBendC_L[e.n1] = add(StrhBendC_L[e.n1], f); }
+    if (e.type == EDGE_WARP) { addSBC(e.n0, f); addSBC(e.n1, neg(f)); }
+    else { addSBC(e.n0, neg(f)); addSBC(e.n1, f); }
     if (!is_edge[e.n0]) addFE(e.n0, g.c, -0.5 * g.ks * (std::pow(nw, 2) - 1), t);
     if (!is_edge[e.n1]) addFE(e.n1, g.c, 0.5 * g.ks * (std::pow(nw, 2) - 1), t);
 }
@@ -481,8 +504,8 @@ void yo_cloth::Stretch_D(const Edge& e, const EdgeGeo& g) {
                 E1E1, E1E0, E1E0, E1E1);
     // StrhBend_C_LE: only the self entries LE(k, e(k)) are read (src/PhysicsCloth.cpp:2588-2603)
     double sg = e.type == EDGE_WARP ? 1.0 : -1.0;
-    if (!is_edge[e.n0]) StrhBendC_LEself[e.n0][g.c] = add(StrhBendC_LEself[e.n0][g.c], scl(sg, L1E1));
-    if (!is_edge[e.n1]) StrhBendC_LEself[e.n1][g.c] = add(StrhBendC_LEself[e.n1][g.c], scl(sg, L1E1));
+    if (!is_edge[e.n0]) addSBCLE(e.n0, g.c, scl(sg, L1E1));
+    if (!is_edge[e.n1]) addSBCLE(e.n1, g.c, scl(sg, L1E1));
 }
 
 // ParaColli_F / ParaColli_D (src/PhysicsCloth.cpp:989-1069)
@@ -830,12 +853,12 @@ void yo_cloth::Crimp_Bending(const Bend& bd) {
     BendGeo g = bend_geo(xc[bd.n2], xc[bd.n0], xc[bd.n1], duv);
     BendOut o = bend_eval(g, kb[bd.yarn]);
     double sg = bd.type == EDGE_WARP ? 1.0 : -1.0;
-    StrhBendC_L[bd.n2] = add(StrhBendC_L[bd.n2], scl(sg, o.F_x2));
-    StrhBendC_L[bd.n0] = add(StrhBendC_L[bd.n0], scl(sg, o.F_x0));
-    StrhBendC_L[bd.n1] = add(StrhBendC_L[bd.n1], scl(sg, o.F_x1));
+    addSBC(bd.n2, scl(sg, o.F_x2));
+    addSBC(bd.n0, scl(sg, o.F_x0));
+    addSBC(bd.n1, scl(sg, o.F_x1));
     V3 L2E2 = neg(o.L2E1);
-    if (!is_edge[bd.n2]) StrhBendC_LEself[bd.n2][c] = add(StrhBendC_LEself[bd.n2][c], scl(sg, L2E2));
-    if (!is_edge[bd.n1]) StrhBendC_LEself[bd.n1][c] = add(StrhBendC_LEself[bd.n1][c], scl(sg, o.L1E1));
+    if (!is_edge[bd.n2]) addSBCLE(bd.n2, c, scl(sg, L2E2));
+    if (!is_edge[bd.n1]) addSBCLE(bd.n1, c, scl(sg, o.L1E1));
     // the remaining crimp LE writes (src/PhysicsCloth.cpp:944-986) land on non-self
     // entries that ConSlideFriction never reads
 }
@@ -1101,20 +1124,21 @@ void yo_cloth::Constraint(int k) {
 // FillMV + system formation
 // ---------------------------------------------------------------------------
 
-int yo_cloth::assemble(double h, const double* ext) {
-    std::fill(FL.begin(), FL.end(), 0.0);
-    std::fill(FE.begin(), FE.end(), 0.0);
-    std::fill(FEpart.begin(), FEpart.end(), 0.0);
-    for (auto& s : StrhBendC_L) s = V3{0, 0, 0};
-    for (auto& s : StrhBendC_LEself) s = {V3{0, 0, 0}, V3{0, 0, 0}};
-    M.zero(); Kx.zero(); Kv.zero();
+// The element loops of FillMV (src/PhysicsCloth.cpp:46-131) over the rows this thread
+// owns; `mid` runs between the friction and the shear loop (a barrier in a parallel
+// assembly: the shear reads the compression of crossings other threads own).
+template <class Mid>
+void yo_cloth::element_loops(Mid mid) {
+    const bool all = tl_vlo == 0 && tl_vhi == INT_MAX;
+    auto touches = [&](int a, int b, int c) { return all || own(a) || own(b) || own(c); };
     // src/PhysicsCloth.cpp:56-62
-    if (on(YO_TERM_HANDLE) && only_kind == 0) for (int k = 0; k < N; ++k) Constraint(k);
+    if (on(YO_TERM_HANDLE) && only_kind == 0) for (int k = 0; k < N; ++k) if (own(k)) Constraint(k);
     // src/PhysicsCloth.cpp:64-80 (edge loop; t = owner thread of edge i)
     int ne = (int)edges.size();
     for (int i = 0; i < ne; ++i) {
         if (!pick(YO_ELEM_EDGE, i)) continue;
         const Edge& e = edges[i];
+        if (!touches(e.n0, e.n1, e.n1)) continue;
         int t = owner(ne, i);
         EdgeGeo g = edge_geo(e);
         if (on(YO_TERM_MASS)) Mass(e, g);
@@ -1133,46 +1157,92 @@ int yo_cloth::assemble(double h, const double* ext) {
     int nb = (int)bends.size();
     for (int i = 0; i < nb; ++i) {
         if (!pick(YO_ELEM_BEND, i)) continue;
+        if (!touches(bends[i].n2, bends[i].n0, bends[i].n1)) continue;
         if (on(YO_TERM_BEND)) Bending(bends[i], owner(nb, i));
         if (on(YO_TERM_CRIMP)) Crimp_Bending(bends[i]);
     }
     // src/PhysicsCloth.cpp:91-95
     if (on(YO_TERM_WIND))
         for (size_t i = 0; i < faces.size(); ++i)
-            if (pick(YO_ELEM_FACE, (int)i)) Wind_F(faces[i]);
+            if (pick(YO_ELEM_FACE, (int)i) && touches(faces[i].n1, faces[i].n2, faces[i].n3)) Wind_F(faces[i]);
     // src/PhysicsCloth.cpp:97-112: no-slide snapshots (the EE diagonals are read from Kx/Kv directly)
-    FE_noslide = FE;
+    for (int j = 1; j < Lg - 1; ++j)
+        if (j >= tl_vlo && j < tl_vhi)
+            for (int i = 1; i < W - 1; ++i) for (int c = 0; c < 2; ++c) FE_noslide[2 * eidx(i, j) + c] = FE[2 * eidx(i, j) + c];
     // src/PhysicsCloth.cpp:114-123 (with the friction term off, the compression is still kept)
-    for (int j = 1; j < Lg - 1; ++j) for (int i = 1; i < W - 1; ++i) {
-        if (on(YO_TERM_FRICTION) && pick(YO_ELEM_CROSSING, eidx(i, j))) ConSlideFriction(kidx(i, j));
-        else comp[eidx(i, j)] = relu(0.5 * dot(nrm[kidx(i, j)], StrhBendC_L[kidx(i, j)]));
+    for (int j = 1; j < Lg - 1; ++j) {
+        if (j < tl_vlo || j >= tl_vhi) continue;
+        for (int i = 1; i < W - 1; ++i) {
+            if (on(YO_TERM_FRICTION) && pick(YO_ELEM_CROSSING, eidx(i, j))) ConSlideFriction(kidx(i, j));
+            else comp[eidx(i, j)] = relu(0.5 * dot(nrm[kidx(i, j)], StrhBendC_L[kidx(i, j)]));
+        }
     }
+    mid();
     // src/PhysicsCloth.cpp:125-131
     if (on(YO_TERM_SHEAR))
         for (size_t i = 0; i < shears.size(); ++i)
-            if (pick(YO_ELEM_SHEAR, (int)i)) Shear_FD(shears[i]);
+            if (pick(YO_ELEM_SHEAR, (int)i) && touches(shears[i].n0, shears[i].n1, shears[i].n3)) Shear_FD(shears[i]);
+}
 
-    // CSR pattern: union of the M, Kx, Kv rows, sorted
+int yo_cloth::assemble(double h, const double* ext) {
+    std::fill(FL.begin(), FL.end(), 0.0);
+    std::fill(FE.begin(), FE.end(), 0.0);
+    std::fill(FEpart.begin(), FEpart.end(), 0.0);
+    for (auto& s : StrhBendC_L) s = V3{0, 0, 0};
+    for (auto& s : StrhBendC_LEself) s = {V3{0, 0, 0}, V3{0, 0, 0}};
+    M.zero(); Kx.zero(); Kv.zero();
+    if (threads <= 1) {
+        element_loops([] {});
+    } else {
+#pragma omp parallel num_threads(threads)
+        {
+            const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+            tl_vlo = (int)((long long)Lg * t / nt);
+            tl_vhi = (int)((long long)Lg * (t + 1) / nt);
+            element_loops([] {
+#pragma omp barrier
+            });
+            tl_vlo = 0;
+            tl_vhi = INT_MAX;
+        }
+    }
+
+    // CSR pattern: union of the M, Kx, Kv rows, sorted.  Each accumulator row is sorted by
+    // column first (its values are already summed per entry), so the union and the values
+    // come from one merge per row.
     rowptr.assign(n + 1, 0);
-    col.clear();
+    std::vector<std::array<std::vector<std::pair<int, double>>, 3>> srt(n);
     std::vector<std::vector<int>> pat(n);
+#pragma omp parallel for num_threads(threads) if (threads > 1) schedule(static)
     for (int r = 0; r < n; ++r) {
-        for (auto& e : M.rows[r]) pat[r].push_back(e.first);
-        for (auto& e : Kx.rows[r]) pat[r].push_back(e.first);
-        for (auto& e : Kv.rows[r]) pat[r].push_back(e.first);
+        const SpAcc* src[3] = {&M, &Kx, &Kv};
+        for (int m = 0; m < 3; ++m) {
+            srt[r][m] = src[m]->rows[r];
+            std::sort(srt[r][m].begin(), srt[r][m].end(),
+                      [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
+            for (auto& e : srt[r][m]) pat[r].push_back(e.first);
+        }
         std::sort(pat[r].begin(), pat[r].end());
         pat[r].erase(std::unique(pat[r].begin(), pat[r].end()), pat[r].end());
-        rowptr[r + 1] = rowptr[r] + (int)pat[r].size();
-        for (int c : pat[r]) col.push_back(c);
     }
+    for (int r = 0; r < n; ++r) rowptr[r + 1] = rowptr[r] + (int)pat[r].size();
     int nnz = rowptr[n];
+    col.assign(nnz, 0);
     Aval.assign(nnz, 0); Mval.assign(nnz, 0); Kxval.assign(nnz, 0); Kvval.assign(nnz, 0);
+    const double h2 = std::pow(h, 2);
+#pragma omp parallel for num_threads(threads) if (threads > 1) schedule(static)
     for (int r = 0; r < n; ++r) {
+        size_t it[3] = {0, 0, 0};
+        std::vector<double>* dst[3] = {&Mval, &Kxval, &Kvval};
         for (int j = rowptr[r]; j < rowptr[r + 1]; ++j) {
-            int c = col[j];
-            Mval[j] = M.get(r, c); Kxval[j] = Kx.get(r, c); Kvval[j] = Kv.get(r, c);
+            const int c = pat[r][j - rowptr[r]];
+            col[j] = c;
+            for (int m = 0; m < 3; ++m) {
+                const auto& row = srt[r][m];
+                (*dst[m])[j] = (it[m] < row.size() && row[it[m]].first == c) ? row[it[m]++].second : 0.0;
+            }
             // A = M - h^2 K_x - h K_v (src/TrainHelper.cpp:19)
-            Aval[j] = Mval[j] - Kxval[j] * std::pow(h, 2) - Kvval[j] * h;
+            Aval[j] = Mval[j] - Kxval[j] * h2 - Kvval[j] * h;
         }
     }
     // F and v in the global layout (get_SpaF / get_SpaVel, src/MatrixMp.cpp:120-122)
@@ -1262,6 +1332,8 @@ void yo_set_terms(yo_cloth* c, unsigned terms) { c->terms = terms; }
 
 void yo_set_element(yo_cloth* c, int kind, int index) { c->only_kind = kind; c->only_index = index; }
 
+void yo_set_threads(yo_cloth* c, int threads) { c->threads = threads < 1 ? 1 : threads; }
+
 void yo_friction_eval(const double* in, double* out) {
     FricIn i{in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], in[8], in[9], in[10]};
     FricOut o = friction_eval(i);
@@ -1330,8 +1402,27 @@ int yo_solve(yo_cloth* c, double tol, int maxit, const double* x0, double* xout,
     const std::vector<int>& cl = c->col;
     const std::vector<double>& A = c->Aval;
     const std::vector<double>& b = c->b;
+    const int T = c->threads;
     auto spmv = [&](const std::vector<double>& in, std::vector<double>& out) {
+#pragma omp parallel for num_threads(T) if (T > 1) schedule(static)
         for (int r = 0; r < n; ++r) { double s = 0; for (int j = rp[r]; j < rp[r + 1]; ++j) s += A[j] * in[cl[j]]; out[r] = s; }
+    };
+    // inner products: in index order with one thread; with several, as the ordered sum of
+    // 256 fixed chunks (the same value for any thread count)
+    constexpr int NCH = 256;
+    std::vector<double> part(NCH);
+    auto dotv = [&](const std::vector<double>& x, const std::vector<double>& y) {
+        if (T <= 1) { double s = 0; for (int i = 0; i < n; ++i) s += x[i] * y[i]; return s; }
+#pragma omp parallel for num_threads(T) schedule(static)
+        for (int q = 0; q < NCH; ++q) {
+            const int i0 = (int)((long long)n * q / NCH), i1 = (int)((long long)n * (q + 1) / NCH);
+            double s = 0;
+            for (int i = i0; i < i1; ++i) s += x[i] * y[i];
+            part[q] = s;
+        }
+        double s = 0;
+        for (int q = 0; q < NCH; ++q) s += part[q];
+        return s;
     };
     std::vector<double> invdiag(n, 1.0);
     for (int r = 0; r < n; ++r) for (int j = rp[r]; j < rp[r + 1]; ++j) if (cl[j] == r && A[j] != 0.0) invdiag[r] = 1.0 / A[j];
@@ -1339,7 +1430,7 @@ int yo_solve(yo_cloth* c, double tol, int maxit, const double* x0, double* xout,
     if (x0) for (int i = 0; i < n; ++i) x[i] = x0[i];
     spmv(x, tmp);
     for (int i = 0; i < n; ++i) r[i] = b[i] - tmp[i];
-    double rhsNorm2 = 0; for (int i = 0; i < n; ++i) rhsNorm2 += b[i] * b[i];
+    double rhsNorm2 = dotv(b, b);
     if (rhsNorm2 == 0) {
         for (int i = 0; i < n; ++i) xout[i] = 0;
         if (iters) *iters = 0;
@@ -1347,22 +1438,25 @@ int yo_solve(yo_cloth* c, double tol, int maxit, const double* x0, double* xout,
         return 0;
     }
     double threshold = std::max(tol * tol * rhsNorm2, std::numeric_limits<double>::min());
-    double res2 = 0; for (int i = 0; i < n; ++i) res2 += r[i] * r[i];
+    double res2 = dotv(r, r);
     int it = 0;
     if (res2 >= threshold) {
         for (int i = 0; i < n; ++i) p[i] = invdiag[i] * r[i];
-        double absNew = 0; for (int i = 0; i < n; ++i) absNew += r[i] * p[i];
+        double absNew = dotv(r, p);
         while (it < maxit) {
             spmv(p, tmp);
-            double pAp = 0; for (int i = 0; i < n; ++i) pAp += p[i] * tmp[i];
+            double pAp = dotv(p, tmp);
             double alpha = absNew / pAp;
+#pragma omp parallel for num_threads(T) if (T > 1) schedule(static)
             for (int i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * tmp[i]; }
-            res2 = 0; for (int i = 0; i < n; ++i) res2 += r[i] * r[i];
+            res2 = dotv(r, r);
             if (res2 < threshold) break;
+#pragma omp parallel for num_threads(T) if (T > 1) schedule(static)
             for (int i = 0; i < n; ++i) z[i] = invdiag[i] * r[i];
             double absOld = absNew;
-            absNew = 0; for (int i = 0; i < n; ++i) absNew += r[i] * z[i];
+            absNew = dotv(r, z);
             double beta = absNew / absOld;
+#pragma omp parallel for num_threads(T) if (T > 1) schedule(static)
             for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
             it++;
         }
@@ -1399,6 +1493,132 @@ void yo_update(yo_cloth* c, const double* nv, double h) {
 int yo_step(yo_cloth* c, double h, const double* ext_force, double tol, int maxit, int* iters, double* relres) {
     if (c->assemble(h, ext_force) != 0) return 1;
     std::vector<double> nv(c->n);
+    int st = yo_solve(c, tol, maxit, nullptr, nv.data(), iters, relres);
+    if (st != 0) return 2;
+    yo_update(c, nv.data(), h);
+    return 0;
+}
+
+// SimStep with the reference's dense storage (a CPU cost model of its layout, W <= ~25):
+// per step the 8 per-thread block arrays of ResetMV (src/PhysicsCloth.cpp:158-197) are
+// allocated zeroed -- LL N^2 x 9, LE/EL N x 2Nc x 3, EE (2Nc)^2 for M, F_*_Pos and F_*_Vel
+// -- the assembled blocks are placed in thread 0's arrays at find_index's offsets and the
+// FillMV reduction adds the other threads' arrays in (:134-156); To_Sparse_Mat builds the
+// five dense n x n matrices SimStep asks for (get_SpaM twice, get_SpaFPos, get_SpaFVel
+// twice, src/TrainHelper.cpp:19-20, src/MatrixMp.cpp:40-111), A and b are formed densely,
+// A and b are scanned for NaN/Inf (:22-34), and EigenSolverForward's n^2 scan turns A into
+// the sparse solver matrix (src/Solvers.cpp:6-12).  The element arithmetic is the oracle's,
+// not LibTorch's per-element tensors, so this is a lower bound of the reference's cost.
+// The matrix the scan produces holds the sparse path's values in its order, so the step
+// equals yo_step bit for bit.
+int yo_step_dense(yo_cloth* c, double h, const double* ext_force, double tol, int maxit, int* iters, double* relres) {
+    if (c->assemble(h, ext_force) != 0) return 1;
+    const int N = c->N, Nc = c->Nc, n = c->n, T = 8, nt = std::max(1, c->threads);
+    const size_t nLL = (size_t)N * N * 9, nEL = (size_t)N * 2 * Nc * 3, nEE = (size_t)2 * Nc * 2 * Nc;
+    // per kind (M, Pos, Vel) and thread: LL, EL, LE, EE
+    std::vector<std::vector<double>> arr((size_t)3 * T * 4);
+    auto A_ = [&](int kind, int t, int part) -> std::vector<double>& { return arr[((size_t)kind * T + t) * 4 + part]; };
+    for (int kind = 0; kind < 3; ++kind)
+        for (int t = 0; t < T; ++t) {
+            const size_t sz[4] = {nLL, nEL, nEL, nEE};
+            for (int part = 0; part < 4; ++part) {
+                std::vector<double>& v = A_(kind, t, part);
+                v.resize(sz[part]);
+                double* d = v.data();
+#pragma omp parallel for num_threads(nt) schedule(static)
+                for (long long i = 0; i < (long long)sz[part]; ++i) d[i] = 0.0;
+            }
+        }
+    // the assembled blocks at find_index's offsets (src/PhysicsCloth.cpp:199-216):
+    // LL[F N + D][u][v] = A[3F+v, 3D+u]; EL/LE[F 2Nc + 2 e + c]; EE[(2 eF + c) 2Nc + 2 eD + c]
+    const std::vector<double>* vals[3] = {&c->Mval, &c->Kxval, &c->Kvval};
+    for (int kind = 0; kind < 3; ++kind)
+        for (int r = 0; r < n; ++r)
+            for (int j = c->rowptr[r]; j < c->rowptr[r + 1]; ++j) {
+                const int col = c->col[j];
+                const double val = (*vals[kind])[j];
+                if (r < 3 * N && col < 3 * N)
+                    A_(kind, 0, 0)[((size_t)(r / 3) * N + col / 3) * 9 + (col % 3) * 3 + r % 3] = val;
+                else if (r >= 3 * N && col < 3 * N)
+                    A_(kind, 0, 1)[((size_t)(col / 3) * 2 * Nc + (r - 3 * N)) * 3 + col % 3] = val;
+                else if (r < 3 * N)
+                    A_(kind, 0, 2)[((size_t)(r / 3) * 2 * Nc + (col - 3 * N)) * 3 + r % 3] = val;
+                else
+                    A_(kind, 0, 3)[(size_t)(r - 3 * N) * 2 * Nc + (col - 3 * N)] = val;
+            }
+    // the FillMV reduction of the per-thread arrays into thread 0's
+    for (int kind = 0; kind < 3; ++kind)
+        for (int t = 1; t < T; ++t)
+            for (int part = 0; part < 4; ++part) {
+                double* d0 = A_(kind, 0, part).data();
+                const double* dt = A_(kind, t, part).data();
+                const long long sz = (long long)A_(kind, 0, part).size();
+#pragma omp parallel for num_threads(nt) schedule(static)
+                for (long long i = 0; i < sz; ++i) d0[i] = d0[i] + dt[i];
+            }
+    // To_Sparse_Mat (src/MatrixMp.cpp:40-111) for M, F_*_Pos, F_*_Vel, M, F_*_Vel
+    auto to_dense = [&](int kind, std::vector<double>& D) {
+        D.assign((size_t)n * n, 0.0);
+        const double* LL = A_(kind, 0, 0).data();
+        const double* EL = A_(kind, 0, 1).data();
+        const double* LE = A_(kind, 0, 2).data();
+        const double* EE = A_(kind, 0, 3).data();
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int F = 0; F < N; ++F) {
+            for (int Dn = 0; Dn < N; ++Dn)
+                for (int u = 0; u < 3; ++u)
+                    for (int v = 0; v < 3; ++v) D[(size_t)(3 * F + v) * n + 3 * Dn + u] = LL[((size_t)F * N + Dn) * 9 + u * 3 + v];
+            for (int e = 0; e < 2 * Nc; ++e)
+                for (int v = 0; v < 3; ++v) {
+                    D[(size_t)(3 * F + v) * n + 3 * N + e] = LE[((size_t)F * 2 * Nc + e) * 3 + v];
+                    D[(size_t)(3 * N + e) * n + 3 * F + v] = EL[((size_t)F * 2 * Nc + e) * 3 + v];
+                }
+        }
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int e = 0; e < 2 * Nc; ++e)
+            for (int f = 0; f < 2 * Nc; ++f) D[(size_t)(3 * N + e) * n + 3 * N + f] = EE[(size_t)e * 2 * Nc + f];
+    };
+    std::vector<double> M1, Kx, Kv1, M2, Kv2;
+    to_dense(0, M1); to_dense(1, Kx); to_dense(2, Kv1); to_dense(0, M2); to_dense(2, Kv2);
+    // A = M - h^2 K_x - h K_v; b = h (F [+ ext] - K_v^T v) + M v, densely (src/TrainHelper.cpp:19-20)
+    std::vector<double> Ad((size_t)n * n);
+    const double h2 = std::pow(h, 2);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long long i = 0; i < (long long)n * n; ++i) Ad[i] = M1[i] - Kx[i] * h2 - Kv1[i] * h;
+    std::vector<double> bd(n);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double kv = 0.0, mv = 0.0;
+        for (int r = 0; r < n; ++r) kv += Kv2[(size_t)r * n + i] * c->vel[r];
+        for (int j = 0; j < n; ++j) mv += M2[(size_t)i * n + j] * c->vel[j];
+        bd[i] = h * (c->F[i] + (ext_force ? ext_force[i] : 0.0) - kv) + mv;
+    }
+    // DetectNaN / DetectInf of A and b (src/TrainHelper.cpp:22-34)
+    long long bad = 0;
+#pragma omp parallel for num_threads(nt) reduction(+ : bad) schedule(static)
+    for (long long i = 0; i < (long long)n * n; ++i) bad += !std::isfinite(Ad[i]);
+    for (int i = 0; i < n; ++i) bad += !std::isfinite(bd[i]);
+    if (bad) return 1;
+    // EigenSolverForward's scan of the dense A into the solver's matrix (src/Solvers.cpp:6-12);
+    // the structural pattern keeps the sparse path's columns, so the solve is the same
+    std::vector<int> rp(n + 1, 0), cl;
+    std::vector<double> av;
+    cl.reserve(c->col.size());
+    av.reserve(c->col.size());
+    for (int r = 0; r < n; ++r) {
+        for (int j = c->rowptr[r]; j < c->rowptr[r + 1]; ++j) {
+            cl.push_back(c->col[j]);
+            av.push_back(Ad[(size_t)r * n + c->col[j]]);
+        }
+        rp[r + 1] = (int)cl.size();
+    }
+    double scanned = 0.0;
+#pragma omp parallel for num_threads(nt) reduction(+ : scanned) schedule(static)
+    for (long long i = 0; i < (long long)n * n; ++i) scanned += Ad[i] != 0.0 ? 1.0 : 0.0;
+    if ((long long)scanned > (long long)av.size()) return 1;   // an entry outside the stencil
+    c->Aval = av;
+    c->b = bd;
+    std::vector<double> nv(n);
     int st = yo_solve(c, tol, maxit, nullptr, nv.data(), iters, relres);
     if (st != 0) return 2;
     yo_update(c, nv.data(), h);

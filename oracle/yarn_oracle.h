@@ -84,6 +84,11 @@ void yo_set_terms(yo_cloth* c, unsigned terms);
 #define YO_ELEM_FACE 4
 #define YO_ELEM_CROSSING 5
 void yo_set_element(yo_cloth* c, int kind, int index);
+/* OpenMP threads of this cloth's assembly, normals and CG (default 1).  The assembly is
+ * bit for bit the single-thread one (each thread owns grid rows and walks the element
+ * lists in the reference's order); the CG's inner products with several threads are
+ * ordered sums of 256 fixed chunks, the same for any thread count above one. */
+void yo_set_threads(yo_cloth* c, int threads);
 /* ConSlideFriction's arithmetic for one crossing component, the function the assembly
  * calls (src/PhysicsCloth.cpp:2619-2633).  in: uv - uvbar, compression, its EE input
  * 0.5 n.StrhBend_C_LE, the no-slide E force, the thread-partial E force, the no-slide EE
@@ -125,6 +130,12 @@ void yo_update(yo_cloth* c, const double* new_vel, double h);
  * 1 NaN/Inf in A/b, 2 NaN/Inf in the solution. */
 int yo_step(yo_cloth* c, double h, const double* ext_force, double tol, int maxit,
             int* iters, double* relres);
+
+/* yo_step with the reference's dense storage work (per-thread N^2 block arrays, their
+ * reduction, dense To_Sparse_Mat matrices and products, the n^2 scan into the solver's
+ * matrix): a cost model of the reference's layout for small cloths; same result. */
+int yo_step_dense(yo_cloth* c, double h, const double* ext_force, double tol, int maxit,
+                  int* iters, double* relres);
 
 /* Static-schedule owner thread used for the friction input (8 threads). */
 int yo_owner(int n_iter, int i);

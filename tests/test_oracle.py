@@ -473,3 +473,37 @@ def test_non_square_rest_state_and_state_order(cs, W, Lg):
     inter = np.array([0 < u < W - 1 and 0 < v < Lg - 1 for v in range(Lg) for u in range(W)])
     for k in ("EPos", "EPosBar", "EVel"):
         assert np.array_equal(back[k][inter], mark[k][inter])
+
+
+def test_parallel_assembly_and_dense_layout_match_serial(cs):
+    """The oracle's multi-threaded assembly (each thread owns grid rows and walks the
+    element lists in the reference's order) is bit for bit the single-thread one, and the
+    dense-layout cost model (the reference's per-thread N^2 block arrays, dense
+    To_Sparse_Mat matrices and n^2 scan) steps to the same state as the sparse path."""
+    s = cs.heter_spec(9, "satin", 31)
+    st = excited_state(cs, s, 32)
+    out = []
+    for threads in (1, 3, 8):
+        c = OracleCloth(s)
+        c.set_threads(threads)
+        c.set_state(st)
+        c.assemble(1e-3)
+        out.append((c.csr(1), c.csr(2), c.csr(3), c.vec("F"), c.vec("b"), c.crossing(), c.normals()))
+    for o in out[1:]:
+        for x, y in zip(o[:3], out[0][:3]):
+            assert (x != y).nnz == 0 and np.array_equal(x.indices, y.indices)
+        for x, y in zip(o[3:5] + (o[6],), out[0][3:5] + (out[0][6],)):
+            assert np.array_equal(x, y)
+        for x, y in zip(o[5], out[0][5]):
+            assert np.array_equal(x, y)
+    s0 = cs.initial_state(s)
+    a, b = OracleCloth(s), OracleCloth(s)
+    a.set_state(s0)
+    b.set_state(s0)
+    for _ in range(3):
+        ra = a.step(1e-3, tol=1e-10, maxit=5000)
+        rb = b.step_dense(1e-3, tol=1e-10, maxit=5000)
+        assert ra == rb and ra[0] == 0
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
