@@ -258,7 +258,7 @@ int batch_upload_items(yc_batch* b, double tol, int maxit) {
         a.A = c->A; a.invd = c->invd; a.bbpart = c->part.bb; a.rrpart = c->part.rr;
         a.brow = b->bj ? c->bj : nullptr;   // the members' common preconditioner
         a.x = c->cx;
-        a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1; a.s[0] = c->cs0; a.s[1] = c->cs1;
+        a.r0 = c->cr; a.w[0] = c->cy; a.w[1] = c->cw1;
         a.slot[0] = c->cgslot; a.slot[1] = c->cgslot + c->nT;
         a.sc = c->sc; a.err = c->cg_err;
         a.epoch = 0;   // taken from the device counter

@@ -894,7 +894,8 @@ struct CgpArgs {
     const double* A; const double* invd; const double* bbpart; const double* rrpart;
     const double* brow;     // block-Jacobi rows (20 per entry, k_bj_factor) or null: Jacobi
     double* x;
-    double* r[2]; double* w[2]; double* s[2];
+    const double* r0;       // r_0 = b - A x_0 (k_cg_init)
+    double* w[2];           // w_j = A M^-1 r_j of every tile, by iteration parity
     CgSlot* slot[2];
     CGScal* sc;
     int* err;
@@ -1094,6 +1095,16 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                 if (e_in[t] && e_own[t]) a.x[e_i[t]] = 0.0;
         return;
     }
+    // r_j and s_{j-1} on tile + halo stay in registers from stage to stage: the block
+    // computes the recurrences of its halo entries itself, from the same inputs and in
+    // the same operations as their owners, so only w_j (a stencil product) crosses
+    // tiles.  Entries outside the grid are held at zero.  r_0 comes from k_cg_init.
+    double e_r[CG1_NE], e_s[CG1_NE];
+#pragma unroll
+    for (int t = 0; t < CG1_NE; ++t) {
+        e_r[t] = e_in[t] ? a.r0[e_i[t]] : 0.0;
+        e_s[t] = 0.0;
+    }
     // stage -1: w_0 = A D^-1 r_0 and the partials of iteration 0
     double alpha = 0.0, beta = 0.0, gprev = 1.0;
     for (int j = -1;; ++j) {
@@ -1101,9 +1112,9 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
         const bool init = j < 0;
         const bool first = j == 0;
         if (!init && !cgp_wait<KP>(a, a.slot[q], (epoch << 32) | (unsigned long long)j)) return;   // error
-        // one round trip for the partials of every tile and the inputs of tile + halo
-        // (r_j, w_j, s_{j-1}); loads whose values an init or first stage ignores are
-        // still issued (their buffers exist), so that none of them sits in a branch
+        // one round trip for the partials of every tile and w_j on tile + halo; loads
+        // whose values an init stage ignores are still issued (their buffers exist), so
+        // that none of them sits in a branch
         const CgSlot* sl = a.slot[q];
         double pv[KP][6];
 #pragma unroll
@@ -1113,15 +1124,12 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
 #pragma unroll
             for (int m = 0; m < 6; ++m) pv[k][m] = ld_dev(&sl[ic].v[m]);
         }
-        const double* r_in = a.r[init ? 0 : q];
         const double* w_in = a.w[q];
-        const double* s_in = a.s[1 - q];
-        double e_r[CG1_NE], e_w[CG1_NE], e_s[CG1_NE];
+        double e_w[CG1_NE];
 #pragma unroll
         for (int t = 0; t < CG1_NE; ++t) {
-            e_r[t] = ld_dev(r_in + e_i[t]);
-            e_w[t] = ld_dev(w_in + e_i[t]);
-            e_s[t] = ld_dev(s_in + e_i[t]);
+            double wv = ld_dev(w_in + e_i[t]);
+            e_w[t] = e_in[t] ? wv : 0.0;
         }
         if (!init) {
             double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1158,7 +1166,6 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
             }
         }
         double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // r.u, u.w, u.s, p.w, p.s, r.r
-        double o_s[CG1_NE], o_r[CG1_NE];
         double k_rn[CG1_NE], k_sj[CG1_NE];
         if constexpr (BJ) {
             // r_{j+1} on tile + halo into LDS, then u_{j+1} = M^-1 r_{j+1} into su by group
@@ -1191,7 +1198,7 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
         }
 #pragma unroll
         for (int t = 0; t < CG1_NE; ++t) {
-            o_s[t] = o_r[t] = 0.0;
+            k_rn[t] = k_sj[t] = 0.0;
             if (threadIdx.x + t * BLK >= 5 * HALO_N) continue;
             double sj = e_w[t] + beta * e_s[t];
             double rn = e_r[t] - alpha * sj;
@@ -1204,8 +1211,6 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                     pj = (BJ ? u_reg[t] : e_id[t] * e_r[t]) + beta * p_reg[t];
                     x_reg[t] += alpha * pj;
                     p_reg[t] = pj;
-                    o_s[t] = sj;
-                    o_r[t] = rn;
                 }
                 if (!BJ) {
                     acc[0] += rn * un;
@@ -1231,6 +1236,12 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                     u_reg[t] = un;
                 }
         }
+        // r_{j+1} and s_j of tile + halo for the next stage
+#pragma unroll
+        for (int t = 0; t < CG1_NE; ++t) {
+            e_r[t] = k_rn[t];
+            e_s[t] = k_sj[t];
+        }
         double y0 = 0.0, y1 = 0.0;
         if (node_in) {
             auto xin = [&](int c, int du, int dv) { return su[c][l0 + du + dv * HALO_W]; };
@@ -1246,7 +1257,7 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
         }
         CgSlot* slot_out = a.slot[init ? 0 : 1 - q];
         cgp_partials(slot_out, acc, shB, tile);
-        // outputs: w_{j+1} (all), r_{j+1} and s_j (iterations only)
+        // output: w_{j+1} of the tile
         double* w_out = a.w[init ? 0 : 1 - q];
         if (node_in) {
             if (g < 3) {
@@ -1255,16 +1266,6 @@ __device__ __forceinline__ void cgp_tile(const CgpArgs& a, int tile, unsigned lo
                 st_dev(w_out + 3 * a.P + p0, y0);
                 st_dev(w_out + 4 * a.P + p0, y1);
             }
-        }
-        if (!init) {
-            double* r_out = a.r[1 - q];
-            double* s_out = a.s[q];
-#pragma unroll
-            for (int t = 0; t < CG1_NE; ++t)
-                if (e_in[t] && e_own[t]) {
-                    st_dev(s_out + e_i[t], o_s[t]);
-                    st_dev(r_out + e_i[t], o_r[t]);
-                }
         }
         cgp_tag(slot_out, (epoch << 32) | (unsigned long long)(j + 1), tile);
     }
@@ -1841,7 +1842,7 @@ int launch_cgp(yc_ctx* c, double tol, int maxit, const double* Aptr = nullptr) {
         if (rc) return rc;
     }
     a.x = c->cx;
-    a.r[0] = c->cr; a.r[1] = c->cr1; a.w[0] = c->cy; a.w[1] = c->cw1; a.s[0] = c->cs0; a.s[1] = c->cs1;
+    a.r0 = c->cr; a.w[0] = c->cy; a.w[1] = c->cw1;
     a.slot[0] = c->cgslot; a.slot[1] = c->cgslot + c->nT;
     a.sc = c->sc; a.err = c->cg_err;
     a.epoch = ++c->epoch;
