@@ -359,18 +359,21 @@ def main():
     guesses = None
     if net is not None:
         cloth, dt_warm, it_warm = timed(True)
+        # the guess's cost: x_q, A x_q and the one-launch features + network (the last
+        # alone is the network's line)
+        cloth.assemble(h)
+        guess_ms, mlp_ms = cloth.time_warm_guess(net, 100)
         cloth.close()
-        mlp_ms = net.time_forward(spec["width"] * spec["length"], 100)
         if args.cg == "persistent":
             # the free guesses the network has to beat, same preconditioner and tolerance
             guesses = {"zero": {"steps_per_s": args.steps / dt_plain, "cg_iters_per_step": it_plain / args.steps},
                        "mlp": {"steps_per_s": args.steps / dt_warm, "cg_iters_per_step": it_warm / args.steps}}
-            for g in ("prev", "extrap"):
+            for g in ("prev", "extrap", "quad"):
                 cg_, dt_g, it_g = timed(False, guess=g)
                 cg_.close()
                 guesses[g] = {"steps_per_s": args.steps / dt_g, "cg_iters_per_step": it_g / args.steps}
     else:
-        dt_warm, it_warm, mlp_ms = dt_plain, it_plain, None
+        dt_warm, it_warm, mlp_ms, guess_ms = dt_plain, it_plain, None, None
     mlp_tflops = MLP_FLOP_PER_NODE * spec["width"] * spec["length"] / (mlp_ms * 1e-3) / 1e12 if net else None
     steps_per_s = args.steps / dt_warm
     plain_steps_per_s = args.steps / dt_plain
@@ -405,6 +408,8 @@ def main():
             "plain_cg_jacobi_persistent": jacobi,
             "mlp": {"bound": "mfma", "achieved": mlp_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": mlp_tflops / BF16_PEAK_TFLOPS, "avg_launch_ms": mlp_ms,
+                    "kernel": "k_mlp_warm (features, network and correction, one launch)",
+                    "warm_guess_ms": guess_ms, "correction_scale": getattr(net, "scale", None),
                     "rows": spec["width"] * spec["length"]} if net else None},
         "roofline": dominant,
     }

@@ -58,7 +58,7 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_batch_create", "yc_batch_destroy", "yc_batch_set_opts", "yc_batch_get_opts", "yc_batch_run",
            "yc_batch_info", "yc_time_assemble", "yc_tangent_set_ext", "yc_set_preconditioner",
            "yc_set_state_device", "yc_get_state_device", "yc_export_precond", "yc_export_matrix",
-           "yc_load_system", "yc_update_vel", "yc_run_guess", "yc_mlp_forward_noalloc"]
+           "yc_load_system", "yc_update_vel", "yc_run_guess", "yc_mlp_forward_noalloc", "yc_warm_guess", "yc_time_warm_guess"]
 
 
 def lib_path():
@@ -143,6 +143,8 @@ def load_library():
     L.yc_mlp_train.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp]
     L.yc_mlp_time_forward.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.yc_features.argtypes = [vp, fp]
+    L.yc_warm_guess.argtypes = [vp, vp]
+    L.yc_time_warm_guess.argtypes = [vp, vp, ctypes.c_int, dp, dp]
     L.yc_targets.argtypes = [vp, fp]
     L.yc_step_warm.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(YcStepInfo)]
     L.yc_run_warm.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,

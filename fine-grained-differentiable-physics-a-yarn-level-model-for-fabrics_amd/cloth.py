@@ -204,11 +204,12 @@ class Cloth:
             _abi.check(rc)
         return tot.value, rc
 
-    GUESSES = {"zero": 0, "prev": 1, "extrap": 2}
+    GUESSES = {"zero": 0, "prev": 1, "extrap": 2, "quad": 3}
 
     def run_guess(self, guess, h, steps, tol=1e-8, maxit=None):
-        """run() from a cheap initial guess: "prev" (v_n) or "extrap" (2 v_n - v_{n-1});
-        persistent CG only.  Returns (total CG iterations, status)."""
+        """run() from a cheap initial guess: "prev" (v_n), "extrap" (2 v_n - v_{n-1}) or
+        "quad" (3 v_n - 3 v_{n-1} + v_{n-2}); persistent CG only.  Returns (total CG
+        iterations, status)."""
         if maxit is None:
             maxit = 2 * self.n
         tot = ctypes.c_longlong()
@@ -225,9 +226,23 @@ class Cloth:
         return out
 
     def targets(self):
+        """After a solve: x - x_q per node (N x 5), x_q = 3 v_n - 3 v_{n-1} + v_{n-2}
+        the quadratic extrapolation the network corrects."""
         out = np.zeros((self.N, 5), dtype=np.float32)
         _abi.check(_abi.load_library().yc_targets(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
+
+    def time_warm_guess(self, mlp, reps=100):
+        """(ms per warm guess, ms per launch of its features + network kernel), after
+        assemble(); leaves the solution vector unspecified."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _abi.check(_abi.load_library().yc_time_warm_guess(self._h, mlp._h, int(reps), ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def warm_guess(self, mlp):
+        """After assemble(): the network's initial guess, left in the solution vector
+        (vec("x")) without solving."""
+        _abi.check(_abi.load_library().yc_warm_guess(self._h, mlp._h))
 
     def node_mask(self):
         m = np.ones((self.N, 5), dtype=np.float32)

@@ -2,11 +2,12 @@
 // FP32 accumulation, trained on the GPU by the kernels below.  Included at the end of
 // yarnstep.hip (it reads the context's device buffers).
 //
-// Per node the network sees 32 features (velocity, Eulerian velocity, the Jacobi
-// estimate D^-1 (b - A v) of the velocity change, the previous step's velocity change,
-// the sliding offset uv - uvbar, the four neighbour offsets and three flags) and
-// predicts the velocity change dv of the implicit step; CG then starts from
-// x0 = v + dv and refines to the same residual as from a zero guess.
+// The network corrects the quadratic extrapolation x_q = 3 v_n - 3 v_{n-1} + v_{n-2}
+// of the implicit step's velocity: per node it sees 32 features (the Jacobi correction
+// D^-1 (b - A x_q) at the node and the L part of it at the four neighbours, the last two
+// steps' velocity changes, the sliding offset and three flags) and predicts x - x_q;
+// CG then starts from x0 = x_q + out_std * y and refines to the same residual as from a
+// zero guess.
 #pragma once
 
 namespace {
@@ -38,29 +39,91 @@ struct MlpW {            // BF16 weights as stored for the MFMA B operand (row-m
     const float* b;            // FP32 master params (biases read from here)
 };
 
-// One 16-row slab of C = A (LDS, bf16, row-major [64][K]) x W^T for a wave, all N
-// columns; epilogue callback gets (row, col, value-with-bias).
-template <int K, int N, class Epi>
+// Row strides of the bf16 activation tiles in LDS: 4 elements of padding move
+// consecutive rows by two banks, so the 16 rows an MFMA fragment load touches fall in
+// different banks.
+constexpr int MLP_LDX = MLP_IN + 4, MLP_LDH = MLP_H + 4;
+
+// C = A W^T over a 64-row tile, A in LDS (bf16, [64][LDA]), W in global memory (bf16,
+// row-major [N][K]).  Wave w computes columns [w N/4, (w+1) N/4) of all 64 rows, so each
+// wave reads a quarter of W (one 8-byte fragment per lane and k-slab, reused for the
+// four row tiles).  Every output element accumulates its k-slabs in ascending order.
+// epi(row, col, value + bias).
+template <int K, int N, int LDA, class Epi>
 __device__ __forceinline__ void mfma_layer(const unsigned short* As, const unsigned short* Wg, const float* bias, Epi epi) {
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int i = lane & 15, kb = lane >> 4;
-    int row0 = wave * 16;
+    constexpr int NW = N / 4, NC = NW / 16;
+    static_assert(NC >= 1 && NW % 16 == 0, "columns per wave");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kb = lane >> 4;
+    floatx4 acc[4][NC];
 #pragma unroll
-    for (int nt = 0; nt < N / 16; ++nt) {
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-        for (int kt = 0; kt < K / 16; ++kt) {
-            const unsigned short* ap = As + (row0 + i) * K + kt * 16 + 4 * kb;
-            bf16x4 a = {(short)ap[0], (short)ap[1], (short)ap[2], (short)ap[3]};
-            const unsigned short* bp = Wg + (nt * 16 + i) * K + kt * 16 + 4 * kb;
-            bf16x4 bv = {(short)bp[0], (short)bp[1], (short)bp[2], (short)bp[3]};
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+        for (int ct = 0; ct < NC; ++ct) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < K / 16; ++kt) {
+        bf16x4 bv[NC];
+#pragma unroll
+        for (int ct = 0; ct < NC; ++ct)
+            bv[ct] = *(const bf16x4*)(Wg + (size_t)(wave * NW + ct * 16 + i) * K + kt * 16 + 4 * kb);
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            bf16x4 a = *(const bf16x4*)(As + (rt * 16 + i) * LDA + kt * 16 + 4 * kb);
+#pragma unroll
+            for (int ct = 0; ct < NC; ++ct) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[ct], acc[rt][ct], 0, 0, 0);
         }
-        int col = nt * 16 + i;
-        float bb = bias ? bias[col] : 0.0f;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) epi(row0 + 4 * kb + r, col, acc[r] + bb);
     }
+#pragma unroll
+    for (int ct = 0; ct < NC; ++ct) {
+        const int col = wave * NW + ct * 16 + i;
+        const float bb = bias ? bias[col] : 0.0f;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) epi(rt * 16 + 4 * kb + r, col, acc[rt][ct][r] + bb);
+    }
+}
+
+// The 16-column output layer: wave w computes rows [16 w, 16 w + 16).  epi(row, col, value).
+template <int K, int LDA, class Epi>
+__device__ __forceinline__ void mfma_out_layer(const unsigned short* As, const unsigned short* Wg, Epi epi) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kb = lane >> 4;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < K / 16; ++kt) {
+        bf16x4 a = *(const bf16x4*)(As + (wave * 16 + i) * LDA + kt * 16 + 4 * kb);
+        bf16x4 bv = *(const bf16x4*)(Wg + (size_t)i * K + kt * 16 + 4 * kb);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) epi(wave * 16 + 4 * kb + r, i, acc[r] + 0.0f);
+}
+
+// The three hidden layers and the output layer of one 64-row tile whose standardised
+// bf16 inputs are in xs; hid(layer, row, col, z) sees every hidden pre-activation after
+// its bf16(silu(z)) is stored, out(row, col, y) every output (y = z + b4, col < 5).
+template <class Hid, class Out>
+__device__ __forceinline__ void mlp_tile(const unsigned short* xs, unsigned short* h0, unsigned short* h1, const MlpW& w,
+                                         Hid hid, Out out) {
+    auto store = [&](unsigned short* dst, int layer) {
+        return [=](int r, int c, float z) {
+            float hval = z * sigmoidf_(z);
+            unsigned short hb = f2bf(hval);
+            dst[r * MLP_LDH + c] = hb;
+            hid(layer, r, c, z, hb);
+        };
+    };
+    mfma_layer<MLP_IN, MLP_H, MLP_LDX>(xs, w.W1, w.b + OFF_B1, store(h0, 0));
+    __syncthreads();
+    mfma_layer<MLP_H, MLP_H, MLP_LDH>(h0, w.W2, w.b + OFF_B2, store(h1, 1));
+    __syncthreads();
+    mfma_layer<MLP_H, MLP_H, MLP_LDH>(h1, w.W3, w.b + OFF_B3, store(h0, 2));
+    __syncthreads();
+    const float* b4 = w.b + OFF_B4;
+    mfma_out_layer<MLP_H, MLP_LDH>(h0, w.W4, [=](int r, int c, float z) {
+        if (c < MLP_OUT) out(r, c, z + b4[c]);
+    });
 }
 
 struct MlpFwdArgs {
@@ -76,9 +139,9 @@ struct MlpFwdArgs {
 
 // Forward over 64-row tiles.  Z_l and H_l = bf16(silu(Z_l)) are kept for the backward.
 __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwdArgs a) {
-    __shared__ unsigned short xs[MLP_TILE * MLP_IN];
-    __shared__ unsigned short h0[MLP_TILE * MLP_H];
-    __shared__ unsigned short h1[MLP_TILE * MLP_H];
+    __shared__ unsigned short xs[MLP_TILE * MLP_LDX];
+    __shared__ unsigned short h0[MLP_TILE * MLP_LDH];
+    __shared__ unsigned short h1[MLP_TILE * MLP_LDH];
     int base = blockIdx.x * MLP_TILE;
     for (int t = threadIdx.x; t < MLP_TILE * MLP_IN; t += blockDim.x) {
         int r = t / MLP_IN, c = t % MLP_IN;
@@ -89,35 +152,24 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwdArgs a) {
             v = (a.X[(size_t)src * MLP_IN + c] - a.in_mean[c]) * a.in_istd[c];
         }
         unsigned short hb = f2bf(v);
-        xs[t] = hb;
+        xs[r * MLP_LDX + c] = hb;
         if (a.sX0 && gr < a.rows) a.sX0[(size_t)gr * MLP_IN + c] = hb;
     }
     __syncthreads();
     const int R = a.rows;
-    auto store = [&](unsigned short* dst, int layer) {
-        return [=](int r, int c, float z) {
-            float hval = z * sigmoidf_(z);
-            unsigned short hb = f2bf(hval);
-            dst[r * MLP_H + c] = hb;
-            int gr = base + r;
-            if (a.sZ && gr < R) {
-                a.sZ[((size_t)layer * R + gr) * MLP_H + c] = z;
-                a.sH[((size_t)layer * R + gr) * MLP_H + c] = hb;
-            }
-        };
-    };
-    mfma_layer<MLP_IN, MLP_H>(xs, a.w.W1, a.w.b + OFF_B1, store(h0, 0));
-    __syncthreads();
-    mfma_layer<MLP_H, MLP_H>(h0, a.w.W2, a.w.b + OFF_B2, store(h1, 1));
-    __syncthreads();
-    mfma_layer<MLP_H, MLP_H>(h1, a.w.W3, a.w.b + OFF_B3, store(h0, 2));
-    __syncthreads();
     float* Y = a.Y;
-    const float* b4 = a.w.b + OFF_B4;
-    mfma_layer<MLP_H, MLP_OUTP>(h0, a.w.W4, nullptr, [=](int r, int c, float z) {
-        int gr = base + r;
-        if (c < MLP_OUT && gr < R && Y) Y[(size_t)gr * MLP_OUT + c] = z + b4[c];
-    });
+    mlp_tile(xs, h0, h1, a.w,
+             [=](int layer, int r, int c, float z, unsigned short hb) {
+                 int gr = base + r;
+                 if (a.sZ && gr < R) {
+                     a.sZ[((size_t)layer * R + gr) * MLP_H + c] = z;
+                     a.sH[((size_t)layer * R + gr) * MLP_H + c] = hb;
+                 }
+             },
+             [=](int r, int c, float y) {
+                 int gr = base + r;
+                 if (gr < R && Y) Y[(size_t)gr * MLP_OUT + c] = y;
+             });
 }
 
 struct MlpBwdArgs {
@@ -284,42 +336,114 @@ __global__ void __launch_bounds__(256) k_mlp_to_bf16(const float* param, unsigne
 // Features and warm start on a cloth context
 // ---------------------------------------------------------------------------
 struct FeatArgs {
-    const double* x; const double* v; const double* uvd; const double* uv; const double* uvbar;
+    const double* uv; const double* uvbar;
     const unsigned char* flags;
-    const double* b; const double* invd; const double* Av;   // padded vectors; Av = A v_old
-    const double* dprev;                                      // padded previous dv
+    const double* b; const double* invd;   // padded
+    const double* Ae;                       // padded A x_q, x_q the extrapolated guess
+    const double* dprev; const double* dprev2;   // padded velocity changes of the last two steps
     int W, Lg, N, Wp, P;
     double L;
-    float* F;    // N x 32
 };
 
-__global__ void __launch_bounds__(256) k_features(FeatArgs a) {
+// The 32 features of node k (u = k mod W, v = k / W), in four groups of straight-line
+// loads (one group per wavefront of k_mlp_warm; k_features runs all four):
+//   0..4    D^-1 (b - A x_q) at the node: the Jacobi correction of the extrapolated
+//           guess x_q;  29..31 interior crossing, handle, flipped           (group 0)
+//   5..9    dv of the last step, 10..14 dv of the step before,
+//   15, 16  sliding offset (uv - uvbar) / L                                 (group 1)
+//   17..22  the Jacobi correction's x, y, z at (u-1, v) and (u+1, v)        (group 2)
+//   23..28  the same at (u, v-1) and (u, v+1)                               (group 3)
+// Neighbours outside the grid and the crossing DOFs (components 3, 4) of boundary
+// nodes read zero.
+template <int G>
+__device__ __forceinline__ void node_features(const FeatArgs& a, int k, float* f) {
+    const int u = k % a.W, v = k / a.W;
+    const bool inter = u > 0 && u < a.W - 1 && v > 0 && v < a.Lg - 1;
+    auto jac = [&](int uu, int vv, int c) {
+        size_t i = (size_t)c * a.P + padidx(a.Wp, uu, vv);
+        return (float)(a.invd[i] * (a.b[i] - a.Ae[i]));
+    };
+    if constexpr (G == 0) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) f[c] = (c < 3 || inter) ? jac(u, v, c) : 0.0f;
+        f[29] = inter ? 1.0f : 0.0f;
+        f[30] = (a.flags[k] & 2) ? 1.0f : 0.0f;
+        f[31] = (a.flags[k] & 1) ? 1.0f : 0.0f;
+    } else if constexpr (G == 1) {
+        const size_t pk = padidx(a.Wp, u, v);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            f[5 + c] = (c < 3 || inter) ? (float)a.dprev[(size_t)c * a.P + pk] : 0.0f;
+            f[10 + c] = (c < 3 || inter) ? (float)a.dprev2[(size_t)c * a.P + pk] : 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            f[15 + c] = inter ? (float)((a.uv[(size_t)c * a.N + k] - a.uvbar[(size_t)c * a.N + k]) / a.L) : 0.0f;
+    } else {
+        // G 2: (u-1, v), (u+1, v); G 3: (u, v-1), (u, v+1)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int uu = G == 2 ? u + 2 * q - 1 : u, vv = G == 3 ? v + 2 * q - 1 : v;
+            const bool in = uu >= 0 && uu < a.W && vv >= 0 && vv < a.Lg;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[17 + 6 * (G - 2) + 3 * q + c] = in ? jac(uu, vv, c) : 0.0f;
+        }
+    }
+}
+
+// the feature indices of group G: 0 -> {0..4, 29..31}, 1 -> 5..16, 2 -> 17..22, 3 -> 23..28
+__device__ __forceinline__ bool in_group(int G, int j) {
+    return G == 0 ? (j < 5 || j >= 29) : G == 1 ? (j >= 5 && j < 17) : G == 2 ? (j >= 17 && j < 23) : (j >= 23 && j < 29);
+}
+
+__global__ void __launch_bounds__(256) k_features(FeatArgs a, float* F) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.N) return;
-    int u = k % a.W, v = k / a.W;
-    int pk = padidx(a.Wp, u, v);
-    bool inter = u > 0 && u < a.W - 1 && v > 0 && v < a.Lg - 1;
     float f[MLP_IN];
-    const size_t N = a.N;
-    for (int c = 0; c < 3; ++c) f[c] = (float)a.v[c * N + k];
-    for (int c = 0; c < 2; ++c) f[3 + c] = inter ? (float)a.uvd[c * N + k] : 0.0f;
-    for (int c = 0; c < 5; ++c) {
-        size_t i = (size_t)c * a.P + pk;
-        f[5 + c] = (float)(a.invd[i] * (a.b[i] - a.Av[i]));
-        f[10 + c] = (float)a.dprev[i];
+    node_features<0>(a, k, f);
+    node_features<1>(a, k, f);
+    node_features<2>(a, k, f);
+    node_features<3>(a, k, f);
+    for (int j = 0; j < MLP_IN; ++j) F[(size_t)k * MLP_IN + j] = f[j];
+}
+
+template <int G>
+__device__ __forceinline__ void stage_features(const FeatArgs& a, int k, int r, const float* in_mean, const float* in_istd,
+                                               unsigned short* xs) {
+    float f[MLP_IN];
+    if (k < a.N) node_features<G>(a, k, f);
+#pragma unroll
+    for (int j = 0; j < MLP_IN; ++j)
+        if (in_group(G, j)) xs[r * MLP_LDX + j] = f2bf(k < a.N ? (f[j] - in_mean[j]) * in_istd[j] : 0.0f);
+}
+
+// The learned warm start in one launch per 64 nodes: features (wavefront G gathers group
+// G of every row), standardisation, the network (the same tile forward as k_mlp_fwd, so
+// the same bits), and x0 += out_std * y on the node's DOFs (x0 holds x_q on entry).
+__global__ void __launch_bounds__(256) k_mlp_warm(FeatArgs a, MlpW w, const float* in_mean, const float* in_istd,
+                                                   const float* out_std, double* x0) {
+    __shared__ unsigned short xs[MLP_TILE * MLP_LDX];
+    __shared__ unsigned short h0[MLP_TILE * MLP_LDH];
+    __shared__ unsigned short h1[MLP_TILE * MLP_LDH];
+    const int base = blockIdx.x * MLP_TILE;
+    {
+        const int r = threadIdx.x & (MLP_TILE - 1), k = base + r;
+        switch (threadIdx.x / MLP_TILE) {
+            case 0: stage_features<0>(a, k, r, in_mean, in_istd, xs); break;
+            case 1: stage_features<1>(a, k, r, in_mean, in_istd, xs); break;
+            case 2: stage_features<2>(a, k, r, in_mean, in_istd, xs); break;
+            default: stage_features<3>(a, k, r, in_mean, in_istd, xs); break;
+        }
     }
-    for (int c = 0; c < 2; ++c) f[15 + c] = inter ? (float)((a.uv[c * N + k] - a.uvbar[c * N + k]) / a.L) : 0.0f;
-    const int du[4] = {-1, 1, 0, 0}, dv[4] = {0, 0, -1, 1};
-    for (int q = 0; q < 4; ++q) {
-        int uu = u + du[q], vv = v + dv[q];
-        bool in = uu >= 0 && uu < a.W && vv >= 0 && vv < a.Lg;
-        int kk = uu + vv * a.W;
-        for (int c = 0; c < 3; ++c) f[17 + 3 * q + c] = in ? (float)((a.x[c * N + kk] - a.x[c * N + k]) / a.L) : 0.0f;
-    }
-    f[29] = inter ? 1.0f : 0.0f;
-    f[30] = (a.flags[k] & 2) ? 1.0f : 0.0f;
-    f[31] = (a.flags[k] & 1) ? 1.0f : 0.0f;
-    for (int c = 0; c < MLP_IN; ++c) a.F[(size_t)k * MLP_IN + c] = f[c];
+    __syncthreads();
+    mlp_tile(xs, h0, h1, w, [](int, int, int, float, unsigned short) {},
+             [=](int r, int c, float y) {
+                 const int k = base + r;
+                 if (k >= a.N) return;
+                 const int u = k % a.W, v = k / a.W;
+                 const bool inter = u > 0 && u < a.W - 1 && v > 0 && v < a.Lg - 1;
+                 if (c < 3 || inter) x0[(size_t)c * a.P + padidx(a.Wp, u, v)] += (double)y * (double)out_std[c];
+             });
 }
 
 // padded v_old (x, y, z, u, v; boundary u, v zero)
@@ -333,32 +457,27 @@ __global__ void __launch_bounds__(256) k_pack_vel(const double* v, const double*
     for (int c = 0; c < 2; ++c) out[(size_t)(3 + c) * P + pk] = inter ? uvd[(size_t)c * N + k] : 0.0;
 }
 
-// x0 = v_old + out_std * y (the network's dv in normalised units)
-__global__ void __launch_bounds__(256) k_warm_x0(const double* vpad, const float* Y, const float* out_std, int W, int Lg, int N,
-                                                  int Wp, int P, double* x0) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    int u = k % W, vv = k / W;
-    int pk = padidx(Wp, u, vv);
-    bool inter = u > 0 && u < W - 1 && vv > 0 && vv < Lg - 1;
-    for (int c = 0; c < 5; ++c) {
-        size_t i = (size_t)c * P + pk;
-        double d = (c < 3 || inter) ? (double)Y[(size_t)k * MLP_OUT + c] * (double)out_std[c] : 0.0;
-        x0[i] = vpad[i] + d;
+// x0 from the padded previous velocity v_n and the last two steps' changes:
+// v_n (kind 1), the linear extrapolation v_n + dprev = 2 v_n - v_{n-1} (kind 2), or the
+// quadratic one v_n + 2 dprev - dprev2 = 3 v_n - 3 v_{n-1} + v_{n-2} (kind 3)
+__global__ void __launch_bounds__(256) k_guess_x0(const double* vpad, const double* dprev, const double* dprev2, int kind,
+                                                   int N5, double* x0) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N5) return;
+    double x = vpad[i];
+    if (kind == 2) x = vpad[i] + dprev[i];
+    if (kind == 3) x = vpad[i] + (2.0 * dprev[i] - dprev2[i]);
+    x0[i] = x;
+}
+
+// dv of the last solve relative to the velocity it started from; the previous one moves
+// to dprev2
+__global__ void __launch_bounds__(256) k_dprev(const double* x, const double* vpad, int N5, double* dprev, double* dprev2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N5) {
+        dprev2[i] = dprev[i];
+        dprev[i] = x[i] - vpad[i];
     }
-}
-
-// dv of the last solve relative to the velocity it started from
-// x0 = v_old (kind 1) or the linear extrapolation v_old + (v_old - v_older) (kind 2) of
-// the velocities, from the padded previous velocity and the last step's change
-__global__ void __launch_bounds__(256) k_guess_x0(const double* vpad, const double* dprev, int kind, int N5, double* x0) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N5) x0[i] = kind == 2 ? vpad[i] + dprev[i] : vpad[i];
-}
-
-__global__ void __launch_bounds__(256) k_dprev(const double* x, const double* vpad, int N5, double* dprev) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N5) dprev[i] = x[i] - vpad[i];
 }
 
 }  // namespace
@@ -374,8 +493,6 @@ struct yc_mlp {
     unsigned short* sX0; float* sZ; unsigned short* sH; float* Y; float* part;
     int adam_t;
     hipStream_t stream;
-    // inference scratch for one context at a time
-    float* F; float* Yi; int capN;
     // this device's copy of the staging array of yc_mlp_forward_noalloc (not owned)
     float* scratch;
 };

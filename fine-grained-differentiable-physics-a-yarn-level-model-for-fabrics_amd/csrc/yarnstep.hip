@@ -1593,7 +1593,7 @@ struct yc_ctx {
     double *A, *F, *is_slide, *comp, *fepart;
     double *b, *invd, *cx, *cr, *cp, *cy;  // padded 5-plane vectors
     double* ext;
-    double *vpad, *dprev, *Av;   // padded v_old, previous dv, A v_old (learned warm start)
+    double *vpad, *dprev, *dprev2, *Ae;   // padded v_old, the last two steps' dv, A x_q (learned warm start)
     double* partials;
     CGPart part;
     CGScal* sc;
@@ -2023,7 +2023,7 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
 int do_update(yc_ctx* c, double h) {
     if (!c->solved) return fail(YC_ERR_ARG, "yc_update before yc_solve");
     hipLaunchKernelGGL(k_dprev, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->cx, (const double*)c->vpad,
-                       5 * c->P, c->dprev);
+                       5 * c->P, c->dprev, c->dprev2);
     hipLaunchKernelGGL(k_update, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), (const double*)c->cx,
                        (const double*)c->is_slide, c->Wp, c->P, h);
     HIPCHK(hipGetLastError());
@@ -2263,7 +2263,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     chk(dalloc(&c->b, PV)); chk(dalloc(&c->invd, PV)); chk(dalloc(&c->cx, PV)); chk(dalloc(&c->cr, PV));
     chk(dalloc(&c->cp, PV)); chk(dalloc(&c->cy, PV));
     chk(dalloc(&c->ext, (size_t)c->n));
-    chk(dalloc(&c->vpad, PV)); chk(dalloc(&c->dprev, PV)); chk(dalloc(&c->Av, PV));
+    chk(dalloc(&c->vpad, PV)); chk(dalloc(&c->dprev, PV)); chk(dalloc(&c->dprev2, PV)); chk(dalloc(&c->Ae, PV));
     chk(dalloc(&c->cr1, PV)); chk(dalloc(&c->cw1, PV)); chk(dalloc(&c->cp1, PV));
     chk(dalloc(&c->cs0, PV)); chk(dalloc(&c->cs1, PV));
     c->cg1_kp = std::min((c->nT + BLK - 1) / BLK, CG1_MAXP);
@@ -2340,7 +2340,7 @@ void yc_destroy(yc_ctx* c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
-                    c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->Av, c->partials, c->sc, c->flag,
+                    c->fepart, c->b, c->invd, c->cx, c->cr, c->cp, c->cy, c->ext, c->vpad, c->dprev, c->dprev2, c->Ae, c->partials, c->sc, c->flag,
                     c->cr1, c->cw1, c->cp1, c->cs0, c->cs1, c->part1, c->cgslot, c->cg_err,
                     c->tstate, c->dA, c->dF, c->db, c->dis_slide, c->dcomp, c->At, c->xsol, c->bsave, c->dsol, c->trhs, c->tdext, c->tshear,
                     c->shear_rec, c->bend_rec, c->bepoch, c->bj, c->Aexp, c->dexp};
@@ -2385,6 +2385,7 @@ int yc_set_state(yc_ctx* c, const double* lpos, const double* lposfix, const dou
     HIPCHK(hipMemcpyAsync(c->uvd, f.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     // a new state has no previous step: the extrapolated guesses start from v alone
     HIPCHK(hipMemsetAsync(c->dprev, 0, (size_t)5 * c->P * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->dprev2, 0, (size_t)5 * c->P * sizeof(double), c->stream));
     int rc = refresh_geometry(c);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2413,6 +2414,7 @@ int yc_set_state_device(yc_ctx* c, const double* lpos, const double* lposfix, co
     hipLaunchKernelGGL(k_state_in, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), io);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemsetAsync(c->dprev, 0, (size_t)5 * c->P * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->dprev2, 0, (size_t)5 * c->P * sizeof(double), c->stream));
     rc = refresh_geometry(c);
     if (rc) return rc;
     c->assembled = c->solved = false;
@@ -2783,7 +2785,7 @@ int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, 
         if (!rc && m) rc = warm_guess(c, m);
         if (!rc && !m && guess != YC_GUESS_ZERO) {
             hipLaunchKernelGGL(k_guess_x0, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->vpad,
-                               (const double*)c->dprev, guess, 5 * c->P, c->cx);
+                               (const double*)c->dprev, (const double*)c->dprev2, guess, 5 * c->P, c->cx);
             HIPCHK(hipGetLastError());
         }
         if (!rc) rc = do_solve(c, nullptr, tol, maxit, nullptr, nullptr, x0, true);
@@ -2827,7 +2829,7 @@ int yc_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* tot
 
 int yc_run_guess(yc_ctx* c, int guess, double h, int steps, double tol, int maxit, long long* total_iters) {
     if (!c) return fail(YC_ERR_ARG, "null context");
-    if (guess != YC_GUESS_ZERO && guess != YC_GUESS_PREV && guess != YC_GUESS_EXTRAP) return fail(YC_ERR_ARG, "unknown guess");
+    if (guess < YC_GUESS_ZERO || guess > YC_GUESS_QUADRATIC) return fail(YC_ERR_ARG, "unknown guess");
     if (c->cg_variant != YC_CG_PERSISTENT || !c->coop_ok) return fail(YC_ERR_ARG, "yc_run_guess runs the persistent CG");
     return run_async(c, nullptr, h, steps, tol, maxit, total_iters, guess);
 }
@@ -3167,27 +3169,31 @@ int mlp_batch(yc_mlp* m, const int* idx_dev, int rows, float lr) {
     return YC_OK;
 }
 
-int warm_guess(yc_ctx* c, yc_mlp* m) {
-    if (m->capN < c->N) {
-        if (m->F) (void)hipFree(m->F);
-        if (m->Yi) (void)hipFree(m->Yi);
-        HIPCHK(hipMalloc((void**)&m->F, (size_t)c->N * MLP_IN * 4));
-        HIPCHK(hipMalloc((void**)&m->Yi, (size_t)c->N * MLP_OUT * 4));
-        m->capN = c->N;
-    }
-    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->vpad, c->Av), (double*)nullptr,
-                       (const CGScal*)c->sc, 0);
+FeatArgs feat_args(yc_ctx* c) {
     FeatArgs fa;
-    fa.x = c->x; fa.v = c->v; fa.uvd = c->uvd; fa.uv = c->uv; fa.uvbar = c->uvbar; fa.flags = c->flags;
-    fa.b = c->b; fa.invd = c->invd; fa.Av = c->Av; fa.dprev = c->dprev;
-    fa.W = c->W; fa.Lg = c->Lg; fa.N = c->N; fa.Wp = c->Wp; fa.P = c->P; fa.L = c->d.L; fa.F = m->F;
-    hipLaunchKernelGGL(k_features, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, fa);
-    MlpFwdArgs f;
-    f.X = m->F; f.idx = nullptr; f.rows = c->N; f.in_mean = m->in_mean; f.in_istd = m->in_istd; f.w = mlp_w(m);
-    f.Y = m->Yi; f.sX0 = nullptr; f.sZ = nullptr; f.sH = nullptr;
-    hipLaunchKernelGGL(k_mlp_fwd, dim3((c->N + MLP_TILE - 1) / MLP_TILE), dim3(256), 0, c->stream, f);
-    hipLaunchKernelGGL(k_warm_x0, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->vpad, (const float*)m->Yi,
-                       (const float*)m->out_std, c->W, c->Lg, c->N, c->Wp, c->P, c->cx);
+    fa.uv = c->uv; fa.uvbar = c->uvbar; fa.flags = c->flags;
+    fa.b = c->b; fa.invd = c->invd; fa.Ae = c->Ae; fa.dprev = c->dprev; fa.dprev2 = c->dprev2;
+    fa.W = c->W; fa.Lg = c->Lg; fa.N = c->N; fa.Wp = c->Wp; fa.P = c->P; fa.L = c->d.L;
+    return fa;
+}
+
+// x_q = v + (2 dprev - dprev2) into xe, then A x_q into c->Ae (the features' residual)
+int extrapolate_and_apply(yc_ctx* c, double* xe) {
+    hipLaunchKernelGGL(k_guess_x0, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->vpad,
+                       (const double*)c->dprev, (const double*)c->dprev2, YC_GUESS_QUADRATIC, 5 * c->P, xe);
+    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, xe, c->Ae), (double*)nullptr,
+                       (const CGScal*)c->sc, 0);
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+// The network's initial guess in c->cx: x_q, its stencil product, then one launch of
+// features + network + correction.
+int warm_guess(yc_ctx* c, yc_mlp* m) {
+    int rc = extrapolate_and_apply(c, c->cx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mlp_warm, dim3((c->N + MLP_TILE - 1) / MLP_TILE), dim3(256), 0, c->stream, feat_args(c), mlp_w(m),
+                       (const float*)m->in_mean, (const float*)m->in_istd, (const float*)m->out_std, c->cx);
     HIPCHK(hipGetLastError());
     return YC_OK;
 }
@@ -3226,7 +3232,7 @@ void yc_mlp_destroy(yc_mlp* m) {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     void* bufs[] = {m->param, m->m1, m->m2, m->grad, m->W1, m->W2, m->W3, m->W4, m->in_mean, m->in_istd, m->out_std,
-                    m->X, m->T, m->M, m->idx, m->sX0, m->sZ, m->sH, m->Y, m->part, m->F, m->Yi};
+                    m->X, m->T, m->M, m->idx, m->sX0, m->sZ, m->sH, m->Y, m->part};
     for (void* p : bufs) if (p) (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -3413,21 +3419,40 @@ int yc_mlp_time_forward(yc_mlp* m, int rows, int reps, double* avg_ms) {
     return st;
 }
 
+int yc_time_warm_guess(yc_ctx* c, yc_mlp* m, int reps, double* avg_ms, double* avg_mlp_ms) {
+    if (!c || !m || reps <= 0 || !avg_ms || !avg_mlp_ms) return fail(YC_ERR_ARG, "bad argument");
+    if (!c->assembled) return fail(YC_ERR_ARG, "warm guess before assemble");
+    int st = time_launches(c->stream, reps, avg_ms, [&] { (void)warm_guess(c, m); });
+    if (st) return st;
+    return time_launches(c->stream, reps, avg_mlp_ms, [&] {
+        hipLaunchKernelGGL(k_mlp_warm, dim3((c->N + MLP_TILE - 1) / MLP_TILE), dim3(256), 0, c->stream, feat_args(c), mlp_w(m),
+                           (const float*)m->in_mean, (const float*)m->in_istd, (const float*)m->out_std, c->cx);
+    });
+}
+
 int yc_features(yc_ctx* c, float* out) {
     if (!c || !out) return fail(YC_ERR_ARG, "null argument");
     if (!c->assembled) return fail(YC_ERR_ARG, "features before assemble");
     float* F = nullptr;
     HIPCHK(hipMalloc((void**)&F, (size_t)c->N * MLP_IN * 4));
-    hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->vpad, c->Av), (double*)nullptr,
-                       (const CGScal*)c->sc, 0);
-    FeatArgs fa;
-    fa.x = c->x; fa.v = c->v; fa.uvd = c->uvd; fa.uv = c->uv; fa.uvbar = c->uvbar; fa.flags = c->flags;
-    fa.b = c->b; fa.invd = c->invd; fa.Av = c->Av; fa.dprev = c->dprev;
-    fa.W = c->W; fa.Lg = c->Lg; fa.N = c->N; fa.Wp = c->Wp; fa.P = c->P; fa.L = c->d.L; fa.F = F;
-    hipLaunchKernelGGL(k_features, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, fa);
-    HIPCHK(hipMemcpyAsync(out, F, (size_t)c->N * MLP_IN * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = extrapolate_and_apply(c, c->cy);   // x_q in the CG work vector
+    if (!rc) {
+        hipLaunchKernelGGL(k_features, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, feat_args(c), F);
+        if (hipGetLastError() != hipSuccess) rc = fail(YC_ERR_HIP, "k_features launch");
+    }
+    if (!rc && (hipMemcpyAsync(out, F, (size_t)c->N * MLP_IN * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = fail(YC_ERR_HIP, "features download");
     (void)hipFree(F);
+    return rc;
+}
+
+int yc_warm_guess(yc_ctx* c, yc_mlp* m) {
+    if (!c || !m) return fail(YC_ERR_ARG, "null argument");
+    if (!c->assembled) return fail(YC_ERR_ARG, "warm guess before assemble");
+    int rc = warm_guess(c, m);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return YC_OK;
 }
 
@@ -3435,14 +3460,20 @@ int yc_targets(yc_ctx* c, float* out) {
     if (!c || !out) return fail(YC_ERR_ARG, "null argument");
     if (!c->solved) return fail(YC_ERR_ARG, "targets before solve");
     size_t PV = (size_t)5 * c->P;
-    std::vector<double> x(PV), v(PV);
+    std::vector<double> x(PV), v(PV), d(PV), d2(PV);
     HIPCHK(hipMemcpyAsync(x.data(), c->cx, PV * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(v.data(), c->vpad, PV * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(d.data(), c->dprev, PV * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(d2.data(), c->dprev2, PV * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    // x - x_q with x_q = v + (2 dprev - dprev2) as k_guess_x0 forms it
     for (int vv = 0; vv < c->Lg; ++vv)
         for (int u = 0; u < c->W; ++u) {
             int k = u + vv * c->W, pk = (u + 2) + (vv + 2) * c->Wp;
-            for (int q = 0; q < 5; ++q) out[(size_t)k * 5 + q] = (float)(x[(size_t)q * c->P + pk] - v[(size_t)q * c->P + pk]);
+            for (int q = 0; q < 5; ++q) {
+                size_t i = (size_t)q * c->P + pk;
+                out[(size_t)k * 5 + q] = (float)(x[i] - (v[i] + (2.0 * d[i] - d2[i])));
+            }
         }
     return YC_OK;
 }

@@ -205,13 +205,15 @@ int yc_run(yc_ctx* ctx, double h, int steps, double tol, int maxit, long long* t
 
 /* yc_run (persistent CG only) with a cheap initial guess in place of the zero one:
  * YC_GUESS_PREV the previous velocity v_n, YC_GUESS_EXTRAP its linear extrapolation
- * 2 v_n - v_{n-1} (v_{n-1} from the previous step of this context; zero before the
- * first).  CG refines either to the same tolerance.  No counterpart in the reference
- * (its solver starts from zero, src/Solvers.cpp:22-24): the baselines a learned warm
- * start (yc_run_warm) is measured against. */
+ * 2 v_n - v_{n-1}, YC_GUESS_QUADRATIC the quadratic one 3 v_n - 3 v_{n-1} + v_{n-2}
+ * (earlier velocities from the previous steps of this context; a new state counts them
+ * equal to v_n).  CG refines any of them to the same tolerance.  No counterpart in the
+ * reference (its solver starts from zero, src/Solvers.cpp:22-24): the baselines a
+ * learned warm start (yc_run_warm) is measured against. */
 #define YC_GUESS_ZERO 0
 #define YC_GUESS_PREV 1
 #define YC_GUESS_EXTRAP 2
+#define YC_GUESS_QUADRATIC 3
 int yc_run_guess(yc_ctx* ctx, int guess, double h, int steps, double tol, int maxit, long long* total_iters);
 
 /* Times `reps` launches of the CG loop's stencil product y = A p on the context's
@@ -276,12 +278,24 @@ int yc_mlp_train(yc_mlp* m, const int* idx, int rows, int batch, float lr, float
  * on the network's stream; avg_ms receives the mean duration of one launch. */
 int yc_mlp_time_forward(yc_mlp* m, int rows, int reps, double* avg_ms);
 
-/* Network features of every node (N x 32 floats) after yc_assemble, and the velocity
- * change of the last solve (N x 5) after yc_solve: the (input, target) pairs the
- * simulator produces for training. */
+/* Network features of every node (N x 32 floats) after yc_assemble, and after yc_solve
+ * the solution's difference from the quadratic extrapolation x_q = 3 v_n - 3 v_{n-1} +
+ * v_{n-2} (N x 5):
+ * the (input, target) pairs the simulator produces for training.  The features: the
+ * Jacobi correction D^-1 (b - A x_q) at the node (5) and its x, y, z part at the four
+ * neighbours (12), the last two steps' velocity changes (10), the sliding offset
+ * (uv - uvbar) / L (2), and the interior, handle and flipped flags (3). */
 int yc_features(yc_ctx* ctx, float* out);
 int yc_targets(yc_ctx* ctx, float* out);
-/* SimStep with the network's warm start (x0 = v + dv on the device). */
+/* The network's initial guess x0 = x_q + out_std * y in the solution vector (read it with
+ * yc_export_vec), after yc_assemble and without solving: for callers that solve with a
+ * solver of their own. */
+int yc_warm_guess(yc_ctx* ctx, yc_mlp* m);
+/* Times `reps` warm guesses (x_q, A x_q, and the one-launch features + network +
+ * correction) and `reps` launches of the last kernel alone, with HIP events on the
+ * context's stream, after yc_assemble; the solution vector is left unspecified. */
+int yc_time_warm_guess(yc_ctx* ctx, yc_mlp* m, int reps, double* avg_ms, double* avg_mlp_ms);
+/* SimStep with the network's warm start (x0 = x_q + out_std * y on the device). */
 int yc_step_warm(yc_ctx* ctx, yc_mlp* m, double h, double tol, int maxit, yc_step_info* info);
 int yc_run_warm(yc_ctx* ctx, yc_mlp* m, double h, int steps, double tol, int maxit, long long* total_iters);
 

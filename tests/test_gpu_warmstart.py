@@ -84,35 +84,97 @@ def test_training_reduces_loss(ws):
     assert losses[-1] < 0.1 * losses[0]
 
 
+def _global(spec, L, E):
+    """A per-node field (L part N x 3, E part N x 2) in the solver's vector layout: the
+    L DOFs node by node, then the E DOFs of the interior crossings."""
+    W, Lg = spec["width"], spec["length"]
+    inter = [u + v * W for v in range(1, Lg - 1) for u in range(1, W - 1)]
+    return np.concatenate([np.asarray(L).ravel(), np.asarray(E)[inter].ravel()])
+
+
+def test_fused_guess_equals_features_and_forward(ys, cs, ws):
+    """The one-launch warm start (features, network, correction on the device) gives the
+    same bits as the features read back, the forward kernel on them, and
+    x0 = x_q + out_std * y formed on the host, x_q = 3 v_n - 3 v_{n-1} + v_{n-2} from three
+    states (as the device forms it: v_n + (2 dv_n - dv_{n-1}))."""
+    spec = cs.heter_spec(21, "twill", 41)
+    spec["wind_vel"] = [0.0, 4.0, 0.0]
+    c = ys.Cloth(spec)
+    c.set_cg_variant("persistent")
+    c.set_ClothState(cs.initial_state(spec))
+    h = spec["h"]
+    for _ in range(3):
+        c.step(h, tol=1e-10)
+    vs = []
+    for _ in range(3):
+        st = c.get_ClothState()
+        vs.append(_global(spec, st["LVel"], st["EVel"]))
+        if len(vs) < 3:
+            c.step(h, tol=1e-10)
+    v2, v1, v0 = vs
+    x_q = v0 + (2.0 * (v0 - v1) - (v1 - v2))
+    c.assemble(h)
+    X = c.features()
+    rng = np.random.default_rng(9)
+    net = ws.WarmStartMLP(params=ws.init_params(3) + rng.normal(0, 0.05, 37893).astype(np.float32))
+    net.set_norm(X.mean(0), X.std(0) + 1e-12, (np.abs(rng.normal(size=5)) * 1e-3 + 1e-4).astype(np.float32))
+    Y = net.forward(X).astype(np.float64) * net.out_std.astype(np.float64)
+    want = x_q + _global(spec, Y[:, :3], Y[:, 3:])
+    c.warm_guess(net)
+    got = c.vec("x")
+    say("fused guess: max |diff| %.3e (|x0 - x_q| %.3e)" % (np.abs(got - want).max(), np.abs(want - x_q).max()))
+    assert np.abs(want - x_q).max() > 0
+    np.testing.assert_array_equal(got, want)
+    net.close()
+    c.close()
+
+
 def test_warm_start_on_cloth(ys, cs, ws):
-    """Train on a few seeded cloths, then compare zero-guess and warm-started CG on an
-    unseen cloth: same tolerance, fewer total iterations, same trajectory."""
+    """Train on a few seeded cloths, then compare on an unseen cloth, all solves to the
+    same tolerance: the warm start needs fewer total CG iterations than the zero guess
+    and than the linear and quadratic extrapolations (the latter is what it corrects),
+    and follows the same trajectory."""
     specs = []
-    for i, W in enumerate((24, 32)):
-        s = cs.heter_spec(W, ("twill", "plain")[i], 300 + i)
+    for i, W in enumerate((24, 32, 28)):
+        s = cs.heter_spec(W, ("twill", "plain", "satin")[i], 300 + i)
         s["wind_vel"] = [0.0, 3.0 + 2 * i, 0.0]
         specs.append(s)
     say("warm start: generating data")
     X, T, M = ws.generate(specs, steps=40, skip=3)
-    net, losses = ws.fit(X, T, M, epochs=10, batch=2048, lr=2e-3)
+    net, losses = ws.fit(X, T, M, epochs=12, batch=2048, lr=2e-3)
     say("  train loss %.3e -> %.3e" % (losses[0], losses[-1]))
-    spec = cs.heter_spec(28, "satin", 399)
+    spec = cs.heter_spec(30, "satin", 399)
     spec["wind_vel"] = [0.0, 4.0, 0.0]
-    a, b = ys.Cloth(spec), ys.Cloth(spec)
     st = cs.initial_state(spec)
-    a.set_ClothState(st)
-    b.set_ClothState(st)
+    names = ("zero", "network", "extrap", "quad")
+    cl = {}
+    for k in names:
+        cl[k] = ys.Cloth(spec)
+        cl[k].set_cg_variant("persistent")
+        cl[k].set_ClothState(st)
     h = spec["h"]
-    ita = itb = 0
+    tot = dict.fromkeys(names, 0)
     for s in range(30):
-        i1, r1, rc1 = a.step(h, tol=1e-10, maxit=20000)
-        i2, r2, rc2 = b.step_warm(net, h, tol=1e-10, maxit=20000)
-        assert rc1 == 0 and rc2 == 0 and r2 < 1e-10
+        it = {}
+        i1, r1, rc = cl["zero"].step(h, tol=1e-10, maxit=20000)
+        assert rc == 0
+        it["zero"] = i1
+        i2, r2, rc = cl["network"].step_warm(net, h, tol=1e-10, maxit=20000)
+        assert rc == 0 and r2 < 1e-10
+        it["network"] = i2
+        for g in ("extrap", "quad"):
+            it[g], rc = cl[g].run_guess(g, h, 1, tol=1e-10, maxit=20000)
+            assert rc == 0
         if s >= 3:
-            ita += i1
-            itb += i2
-    sa, sb = a.get_ClothState(), b.get_ClothState()
-    e = max(np.linalg.norm(sa[k] - sb[k]) / max(np.linalg.norm(sa[k]), 1e-300) for k in ("LPos", "LVel"))
-    say("  iterations zero-guess %d warm %d, state rel diff %.2e" % (ita, itb, e))
-    assert itb < ita
-    assert e < 1e-6
+            for k in names:
+                tot[k] += it[k]
+    sa, sb = cl["zero"].get_ClothState(), cl["network"].get_ClothState()
+    err = max(np.linalg.norm(sa[k] - sb[k]) / max(np.linalg.norm(sa[k]), 1e-300) for k in ("LPos", "LVel"))
+    say("  total iterations %s; state rel diff %.2e" % (tot, err))
+    assert tot["network"] < tot["zero"]
+    assert tot["network"] < tot["extrap"]
+    assert tot["network"] < tot["quad"]
+    assert err < 1e-6
+    net.close()
+    for c in cl.values():
+        c.close()
