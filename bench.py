@@ -191,6 +191,29 @@ def tangent_measure(ys, spec, state, steps, precond):
     return out
 
 
+def gradient_measure(ys, steps, precond):
+    """Batched differentiable learning (64 cloths of uneven size, the batch workload):
+    one loss-and-gradient evaluation of the eight material parameters per cloth, each a
+    rollout of `steps` steps from the first frame of its ground truth with perturbed
+    parameters, as one epoch of parameter learning runs it.  The ground truth comes from
+    a batch run beforehand (untimed).  The gradient is the reference's (ref mode), from
+    forward-mode tangents: eight directions and the primal solve per step."""
+    import yarnstep.learn as ln
+    items = ys.cloth_spec.batch_workload()
+    frames = ln.batch_trajectories(items, steps)
+    t0 = time.perf_counter()
+    ok = 0
+    for i, ((spec, _), f) in enumerate(zip(items, frames)):
+        sp = ln.perturbed_spec(spec, ln.NAMES, 500 + i)
+        _, g, ok_ = ln.sample_loss_terms(sp, f, ln.NAMES, mode="ref", tol=1e-8, precond=precond)
+        ok += ok_
+    dt = time.perf_counter() - t0
+    return {"workload": "64 cloths, sides 10..64 (seed 3), %d-step rollouts, gradients of %s" % (steps, ", ".join(ln.NAMES)),
+            "form": "forward-mode tangents, 8 directions + the primal solve per step, one cloth at a time",
+            "cloth_steps": ok, "cloth_gradient_steps_per_s": ok / dt, "ms_per_cloth_step": 1e3 * dt / max(ok, 1),
+            "s_per_epoch": dt}
+
+
 def hbm_stream_measure(launches=20):
     """Sustained HBM bandwidth on this GPU: a STREAM-style FP64 negation of 1 GiB into
     another 1 GiB (2 GiB moved per launch), timed with HIP events on torch's stream."""
@@ -247,6 +270,8 @@ def main():
     ap.add_argument("--precond", default="block2", choices=["block2", "jacobi"],
                     help="preconditioner of the persistent-CG runs (the Jacobi one, the reference solver's, is "
                          "also timed once, zero guess)")
+    ap.add_argument("--gradient-steps", type=int, default=5,
+                    help="rollout steps of the 64-cloth parameter-gradient line (0: skip it)")
     ap.add_argument("--tangent-only", action="store_true",
                     help="time only the differentiation line and print it (profiling runs)")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -429,6 +454,8 @@ def main():
         out["config"]["homo256"] = large_measure(ys, args.large_steps)
     if args.tangent_steps > 0 and not args.plain_only and args.cg == "persistent":
         out["config"]["tangents"] = tangent_measure(ys, spec, state, args.tangent_steps, args.precond)
+    if args.gradient_steps > 0 and not args.plain_only and args.cg == "persistent":
+        out["config"]["batch_gradient"] = gradient_measure(ys, args.gradient_steps, args.precond)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cs, args.workload, args.cpu_steps)
     # release every HIP object before the interpreter shuts down (no destructor runs

@@ -84,8 +84,9 @@ def loss_and_grad(spec, state, steps, names, target, mode="ref", **kw):
 
 def make_autograd_function():
     """torch.autograd.Function over the rollout: forward(theta) -> positions
-    (steps, N, 3) as float64; backward contracts the upstream gradient with the
-    forward-mode Jacobian saved in the forward pass."""
+    (steps, N, 3) as float64 on theta's device; backward contracts the upstream gradient
+    with the forward-mode Jacobian saved in the forward pass and returns it with
+    theta's device and dtype."""
     import torch
 
     class YarnRollout(torch.autograd.Function):
@@ -93,13 +94,14 @@ def make_autograd_function():
         def forward(ctx, theta, spec, state, steps, names, mode):
             vals = theta.detach().cpu().numpy().astype(np.float64)
             pos, jac = rollout_jacobian(with_params(spec, names, vals), state, steps, names, mode=mode)
-            ctx.save_for_backward(torch.from_numpy(jac))
-            return torch.from_numpy(pos)
+            ctx.theta_device, ctx.theta_dtype = theta.device, theta.dtype
+            ctx.save_for_backward(torch.from_numpy(jac).to(theta.device))
+            return torch.from_numpy(pos).to(theta.device)
 
         @staticmethod
         def backward(ctx, g):
             (jac,) = ctx.saved_tensors
-            grad = (jac * g.unsqueeze(0)).reshape(jac.shape[0], -1).sum(dim=1)
-            return grad.to(torch.float64), None, None, None, None, None
+            grad = (jac * g.to(jac.dtype).unsqueeze(0)).reshape(jac.shape[0], -1).sum(dim=1)
+            return grad.to(device=ctx.theta_device, dtype=ctx.theta_dtype), None, None, None, None, None
 
     return YarnRollout

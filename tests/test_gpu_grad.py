@@ -323,6 +323,33 @@ def test_autograd_rollout_gradient_matches_fd(ys, cs):
     assert abs(an - fd) <= 1e-5 * abs(fd)
 
 
+def test_autograd_function_keeps_theta_device(ys, cs):
+    """theta on the GPU in float32: the positions come back on its device and the
+    gradient with its device and dtype, equal (to float32 rounding) to the gradient of a
+    float64 CPU theta."""
+    import torch
+    from importlib import import_module
+    grad = import_module(ys.__name__ + ".grad")
+    spec = _spec(cs)
+    state = natural_state(cs, spec)
+    names = ["rho0", "S"]
+    steps = 3
+    vals = np.array([grad.get_param(spec, n) for n in names])
+    target, _ = grad.rollout_jacobian(grad.with_params(spec, names, vals * 1.1), state, steps, names, mode="ref")
+    fn = grad.make_autograd_function()
+    out = {}
+    for dev, dt in (("cpu", torch.float64), ("cuda", torch.float32)):
+        theta = torch.tensor(vals, dtype=dt, device=dev, requires_grad=True)
+        pos = fn.apply(theta, spec, state, steps, names, "ref")
+        assert pos.device.type == dev
+        loss = torch.mean((pos - torch.from_numpy(target).to(dev)) ** 2)
+        loss.backward()
+        assert theta.grad.device.type == dev and theta.grad.dtype == dt
+        out[dev] = theta.grad.double().cpu().numpy()
+    say("  gradient cpu %s cuda %s" % (out["cpu"], out["cuda"]))
+    np.testing.assert_allclose(out["cuda"], out["cpu"], rtol=1e-5)
+
+
 @pytest.mark.parametrize("steps", [1, 3])
 def test_ext_force_tangent_matches_fd(ys, cs, steps):
     """The tangent of the trajectory along a direction of the external force (the path of
