@@ -176,57 +176,11 @@ __device__ __forceinline__ int ld_done(const CGScal* s) {
     return d;
 }
 
-// ---------------------------------------------------------------------------
-// Assembly
-// ---------------------------------------------------------------------------
-struct AsmOut {
-    double* A;          // tiles of NSLOT planes of TILE_N (a_base)
-    double* F;          // 5 planes of N
-    double* b;          // padded 5-plane vector
-    double* is_slide;   // 2 planes of N
-    double* comp;       // N
-    double* fepart;     // 2 planes of N
-    double* invdiag;    // padded 5-plane vector
-    const double* ext;  // global-layout n vector or null
-    const double* dext; // tangent run: the direction's tangent of ext (global layout) or null
-    int* flag;          // [0] structural error, [1] non-finite
-    int Wp, P;          // padded width, plane stride
-    // tangent run: tangents of A (exact mode, else null), F, b, is_slide and comp, and
-    // the comp tangent read by the shear; detach = the reference's gradient
-    double *dA, *dF, *db, *dis_slide, *dcomp;
-    int detach;
-    // value run: element-major shear and bend records (k_shear_elem, k_bend_elem)
-    // replayed by the node kernels
-    const double* shear_rec;
-    int nseg;
-    const double* bend_rec;
-    // value run: -1 for A; 0, 1, 2 for M, K_x or K_v alone (RowSinkT::select_kind)
-    int only_kind;
-};
+}  // namespace
 
-__device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
+#include "assembly.hpp"
 
-// Bends in the node kernel: computed in place (the default), or replayed from the
-// element-major records of k_bend_elem.  Measured at 128 x 128: replaying is slower
-// (455 us against 360 us for the node kernel), so the records are off.
-constexpr bool USE_BEND_REC = false;
-
-// LDS of the value assembly kernels: one tile's slot planes (81,408 bytes, above the
-// 64 KiB default, allowed per kernel at context creation)
-constexpr size_t ASM_LDS = (size_t)NSLOT * TILE_N * sizeof(double);
-// the first assembly kernel's value run: the slot planes, then the second half's L-L self
-// block (9 planes) and accumulators (RowSinkT::NACC planes) -- 101,888 bytes
-constexpr size_t ASM1_LDS = (size_t)(NSLOT + 9 + RowSinkT<double>::NACC) * TILE_N * sizeof(double);
-constexpr int ASM1_THREADS = 2 * TILE_N;
-
-// Node of thread t in the assembly kernels: one block per 8 x 8 matrix tile, so that a
-// wavefront's read-modify-writes of one slot are 512 contiguous bytes.
-__device__ __forceinline__ int tile_node(const ClothDev& C, int tile, int lane) {
-    int u = (tile % C.ntx) * TILE + (lane & (TILE - 1));
-    int v = (tile / C.ntx) * TILE + (lane / TILE);
-    return (u < C.W && v < C.Lg) ? u + v * C.W : -1;
-}
-__device__ __forceinline__ int tile_node(const ClothDev& C, int tile) { return tile_node(C, tile, threadIdx.x); }
+namespace {
 
 // T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
 // src/TrainHelper.cpp:19-20).  T = dd: their directional derivatives along the
@@ -1646,27 +1600,9 @@ struct yc_ctx {
 namespace {
 
 ClothDev make_dev(const yc_ctx* c) {
-    ClothDev C;
-    const yc_desc& d = c->d;
-    C.W = c->W; C.Lg = c->Lg; C.N = c->N; C.Nc = c->Nc; C.ntx = c->ntx;
-    C.n_edges = c->Lg * (c->W - 1) + c->W * (c->Lg - 1);
-    C.n_bends = c->Lg * (c->W - 2) + c->W * (c->Lg - 2);
-    C.L = d.L; C.R = d.R;
-    for (int y = 0; y < 2; ++y) {
-        int yy = d.n_yarns == 1 ? 0 : y;
-        C.rho[y] = d.rho[yy];
-        C.ks[y] = d.Y[yy] * PI_REF * std::pow(d.R, 2);   // src/Yarn.h:14-15
-        C.kb[y] = d.B[yy] * PI_REF * std::pow(d.R, 2);
-    }
-    C.kc = d.kc; C.kf = d.kf; C.df = d.df; C.mu = d.mu; C.S = d.S; C.handle_k = d.handle_k;
-    for (int i = 0; i < 3; ++i) { C.G[i] = d.G[i]; C.wind_vel[i] = d.wind_vel[i]; }
-    C.wind_density = d.wind_density; C.wind_drag = d.wind_drag;
-    C.jam_thrd = 2.0 * std::asin(d.R / d.L);   // src/Cloth.cpp:29
+    ClothDev C = cloth_params(c->d);
     C.x = c->x; C.v = c->v; C.xfix = c->xfix; C.uv = c->uv; C.uvd = c->uvd; C.uvbar = c->uvbar;
     C.dx = C.dv = C.dxfix = C.duv = C.duvd = C.duvbar = C.dnrm = nullptr;
-    C.drho[0] = C.drho[1] = C.dks[0] = C.dks[1] = C.dkb[0] = C.dkb[1] = 0.0;
-    C.dkc = C.dkf = C.ddf = C.dmu = C.dS = C.dhandle_k = 0.0;
-    C.ref_mode = 0;
     C.nrm = c->nrm; C.flags = c->flags;
     return C;
 }
@@ -1707,12 +1643,6 @@ SpmvArgs make_spmv(yc_ctx* c, const double* in, double* out, const double* Aptr 
 int grid_for(long long threads) { return (int)((threads + BLK - 1) / BLK); }
 
 // flip_norm by weave pattern (src/Cloth.cpp:66-92)
-bool pattern_flip(int pattern, int i, int j) {
-    if (pattern == 0) return (i + j) % 2 != 0;
-    if (pattern == 1) { int rv = j % 4; int ru = (i + rv) % 4; return ru < 2; }
-    int rv = (j - 1) % 4; int q = rv / 2; int r = rv % 2; int iu = q * 3 + r * 2; int ru = (i - 1 + iu) % 4;
-    return ru == 0;
-}
 
 // State between a caller's node-major device arrays ((N, 3) and (N, 2), state index
 // u + v * width) and the context's component planes, one thread per node.
