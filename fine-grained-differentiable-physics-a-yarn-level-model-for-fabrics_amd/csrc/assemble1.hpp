@@ -1,7 +1,6 @@
-// assemble1_body.inc -- body of the first assembly kernel (k_assemble1 and its batched
-// form k_b_assemble1), included inside each kernel's braces.  Expects in scope: the
-// scalar type T, the cloth C (ClothDev), its outputs O (AsmOut), the step h and the
-// tile number `tile` of the block, and ASM1_REF (true: the reference gradient's tangent run).
+// assemble1.hpp -- the first assembly kernel's work for one tile, shared by k_assemble1,
+// its tangent form k_assemble1_dirs, the batched k_b_assemble1 and the host build of
+// tests/asm_host.cpp.  Included after elements.hpp and assembly.hpp.
 //
 // T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
 // src/TrainHelper.cpp:19-20), two threads per node (blocks of 2 x 64 threads): half 0
@@ -10,7 +9,14 @@
 // rows except the L-L self block, which half 1 keeps apart; after a barrier half 0 adds
 // it and half 1's accumulators in, then does the friction and b.  T = dd: their
 // directional derivatives along the tangents held in C, into the d* outputs of O, one
-// thread per node (blocks of 64 threads).
+// thread per node (blocks of 64 threads).  ASM1_REF: the reference gradient's tangent run.
+#pragma once
+
+namespace {
+using namespace ys;
+
+template <class T, bool ASM1_REF>
+__device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut& O, double h, int tile) {
     typedef v3<T> V;
     constexpr bool tan = !std::is_same<T, double>::value;
     const int nl = threadIdx.x & (TILE_N - 1);              // node lane in the tile
@@ -53,7 +59,7 @@
         // call site in a loop that is not unrolled: the inlined element code is large, and
         // four or six copies of it would not fit the instruction cache.
         int ne_warp = C.Lg * (C.W - 1);
-#pragma unroll 1
+    #pragma unroll 1
         for (int ei = warp_side ? 0 : 2; ei < (weft_side ? 4 : 2); ++ei) {
             int eu, ev, ec, idx;
             if (ei == 0) { if (u == 0) continue; eu = u - 1; ev = v; ec = 0; idx = v * (C.W - 1) + u - 1; }
@@ -66,7 +72,7 @@
         // a bend centred d after the node holds it as n2 (role 0), centred on it as n0
         // (role 1), centred d before it as n1 (role 2)
         int nb_warp = C.Lg * (C.W - 2);
-#pragma unroll 1
+    #pragma unroll 1
         for (int bi6 = warp_side ? 0 : 3; bi6 < (weft_side ? 6 : 3); ++bi6) {
             int d = bi6 % 3 - 1, dir = bi6 / 3;
             int uc = dir == 0 ? u + d : u, vc = dir == 0 ? v : v + d;
@@ -78,7 +84,7 @@
         }
         // faces of the up to four cells around the node (src/Cloth.cpp:272-286)
         if (weft_side && (C.wind_density != 0.0 || C.wind_drag != 0.0)) {
-#pragma unroll 1
+    #pragma unroll 1
             for (int fi = 0; fi < 8; ++fi) {
                 int cu = u - 1 + ((fi >> 2) & 1), cv = v - 1 + ((fi >> 1) & 1), q = fi & 1;
                 if (cu < 0 || cv < 0 || cu > C.W - 2 || cv > C.Lg - 2) continue;
@@ -152,3 +158,6 @@
         }
     }
     if (active && S.bad) atomicOr(&O.flag[0], 1);
+}
+
+}  // namespace

@@ -62,16 +62,10 @@ __global__ void __launch_bounds__(64) k_b_status(const BatchItem* __restrict__ i
     out[BSTAT * i + 5] = *b.cg.err;
 }
 
-// the first assembly kernel's body (assemble1_body.inc) on the block's cloth, from a
-// local copy of its descriptors as the single-cloth kernel takes them
+// the first assembly kernel's work (assemble1_block) on the block's cloth
 __global__ void __launch_bounds__(ASM1_THREADS, 1) k_b_assemble1(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
-    typedef double T;
     const BatchItem& bi = it[map[blockIdx.x]];
-    const ClothDev& C = bi.C;
-    const AsmOut& O = bi.O;
-    const int tile = blockIdx.x - bi.t0;
-    constexpr bool ASM1_REF = false;
-#include "assemble1_body.inc"
+    assemble1_block<double, false>(bi.C, bi.O, h, blockIdx.x - bi.t0);
 }
 
 __global__ void __launch_bounds__(NBLK, 1) k_b_assemble2(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {

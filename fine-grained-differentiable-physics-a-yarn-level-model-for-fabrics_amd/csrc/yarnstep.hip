@@ -179,21 +179,15 @@ __device__ __forceinline__ int ld_done(const CGScal* s) {
 }  // namespace
 
 #include "assembly.hpp"
+#include "assemble1.hpp"
 
 namespace {
 
 // T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
-// src/TrainHelper.cpp:19-20).  T = dd: their directional derivatives along the
-// tangents held in C, into the d* outputs of O.
-// The kernel body is shared with the batched kernel by textual inclusion: built as an
-// inlined device function called from a kernel (its descriptors by reference or by
-// value), this compiler produced wrong rows for boundary nodes (one edge's self block
-// landed two nodes away), while the body written inside the kernel is correct.
+// src/TrainHelper.cpp:19-20), one block per tile (assemble1_block).
 template <class T>
 __global__ void __launch_bounds__(ASM1_THREADS, 1) k_assemble1(ClothDev C, AsmOut O, double h) {
-    const int tile = blockIdx.x;
-    constexpr bool ASM1_REF = false;
-#include "assemble1_body.inc"
+    assemble1_block<T, false>(C, O, h, blockIdx.x);
 }
 
 // The tangent run of every direction in one launch (blockIdx.y: the direction, with its
@@ -203,12 +197,7 @@ __global__ void __launch_bounds__(ASM1_THREADS, 1) k_assemble1(ClothDev C, AsmOu
 template <bool REF>
 __global__ void __launch_bounds__(NBLK, 1) k_assemble1_dirs(const ClothDev* __restrict__ Cs, const AsmOut* __restrict__ Os,
                                                            double h) {
-    typedef dd T;
-    constexpr bool ASM1_REF = REF;
-    const ClothDev& C = Cs[blockIdx.y];
-    const AsmOut& O = Os[blockIdx.y];
-    const int tile = blockIdx.x;
-#include "assemble1_body.inc"
+    assemble1_block<dd, REF>(Cs[blockIdx.y], Os[blockIdx.y], h, blockIdx.x);
 }
 
 // Shear segments touching the node (4 own, 2 each as warp neighbour of the left and

@@ -1,16 +1,16 @@
-// Host build of the first assembly kernel's body (fine-grained-..._amd/csrc/
-// assemble1_body.inc), for the undefined-behaviour check of that body: compiled with g++
-// under AddressSanitizer and UndefinedBehaviorSanitizer, every GPU thread of a block run as
-// a std::thread, the block's LDS an exact-size global array (the sanitizer's red zones
-// catch any index past it), blocks one after another.  The same body text the GPU kernel
-// includes, through the same declarations (elements.hpp, assembly.hpp).
+// Host build of the first assembly kernel's work for a tile (assemble1_block,
+// fine-grained-..._amd/csrc/assemble1.hpp), for the undefined-behaviour check of that
+// code: compiled with g++ under AddressSanitizer and UndefinedBehaviorSanitizer, every GPU
+// thread of a block run as a std::thread, the block's LDS an exact-size global array (the
+// sanitizer's red zones catch any index past it), blocks one after another.  The same
+// function the GPU kernels call, through the same declarations (elements.hpp,
+// assembly.hpp).
 //
 // Input file: yc_desc (raw), h, then the state in the kernels' planes: x (3N), v (3N),
-// xfix (3N), uv (2N), uvd (2N), uvbar (2N), nrm (3N).  Output file: A in the tile layout
-// The output file: the count of A's stored slots, then their rows, columns (int32, the
-// solver's global numbering: 3 L DOFs per node, then 2 E DOFs per interior crossing) and
-// values, then F (5N), b (5 planes of P), comp (N), is_slide (2N), fepart (2N) and the two
-// status flags as doubles.
+// xfix (3N), uv (2N), uvd (2N), uvbar (2N), nrm (3N).  Output file: the count of A's
+// stored slots, then their rows, columns (int32, the solver's global numbering: 3 L DOFs
+// per node, then 2 E DOFs per interior crossing) and values, then F (5N), b (5 planes of
+// P), comp (N), is_slide (2N), fepart (2N) and the two status flags as doubles.
 #include <barrier>
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +23,7 @@
 #include "yarnstep.h"
 #include "elements.hpp"
 #include "assembly.hpp"
+#include "assemble1.hpp"
 
 using namespace ys;
 
@@ -30,15 +31,12 @@ thread_local HostDim3 threadIdx, blockIdx, blockDim;
 static std::barrier<>* g_block_barrier = nullptr;
 void host_syncthreads() { g_block_barrier->arrive_and_wait(); }
 
-// the block's LDS, exactly the dynamic size the GPU kernel is launched with
+// the block's LDS, exactly the dynamic size the GPU kernel is launched with (in the
+// unnamed namespace the kernel code's extern declaration refers to)
+namespace {
 double lds_rows[ASM1_LDS / sizeof(double)];
-
-// one GPU thread of k_assemble1<double>
-static void assemble1_thread(const ClothDev& C, const AsmOut& O, double h, int tile) {
-    typedef double T;
-    constexpr bool ASM1_REF = false;
-#include "assemble1_body.inc"
 }
+
 
 static bool read_all(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
 
@@ -82,7 +80,7 @@ int main(int argc, char** argv) {
                 threadIdx = {(unsigned)t, 0u, 0u};
                 blockIdx = {(unsigned)tile, 0u, 0u};
                 blockDim = {(unsigned)ASM1_THREADS, 1u, 1u};
-                assemble1_thread(C, O, h, tile);
+                assemble1_block<double, false>(C, O, h, tile);
             });
         for (auto& t : th) t.join();
     }

@@ -1,11 +1,11 @@
-"""The first assembly kernel's body (csrc/assemble1_body.inc) built for the host by
-tests/asm_host.cpp under AddressSanitizer and UndefinedBehaviorSanitizer (g++), every
+"""The first assembly kernel's work (assemble1_block, csrc/assemble1.hpp) built for the
+host by tests/asm_host.cpp under AddressSanitizer and UndefinedBehaviorSanitizer (g++), every
 thread of a block an OS thread and the block's LDS an exact-size array: no sanitizer
 report on the parity cases (one to four tiles, partial tiles, handles, wind), and the
 rows it writes -- every term but the shear, which the second kernel adds -- equal the
 oracle's with the shear switched off (A, F, b within 1e-12 relative; the compression
 within 1e-12).  This is the check for undefined behaviour (an index out of range, a read
-of an uninitialised slot) in the body that the GPU kernel includes."""
+of an uninitialised slot) in the code the GPU kernels call."""
 import os
 import subprocess
 
