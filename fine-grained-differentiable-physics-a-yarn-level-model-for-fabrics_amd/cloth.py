@@ -324,6 +324,80 @@ class Cloth:
     def tangent_end(self):
         _abi.check(_abi.load_library().yc_tangent_end(self._h))
 
+    # --- reverse mode (adjoint) of a recorded rollout (differentiation) ---
+    def adjoint_begin(self, max_steps, checkpoint_every=1):
+        """Starts recording at the current state, for up to max_steps adjoint_step calls;
+        the state is stored every checkpoint_every steps (the backward pass recomputes
+        the others)."""
+        _abi.check(_abi.load_library().yc_adjoint_begin(self._h, int(max_steps), int(checkpoint_every)))
+        self._adj_max = int(max_steps)
+        self._adj_steps = 0
+
+    def adjoint_step(self, h, tol=1e-10, maxit=None):
+        """A recorded SimStep (zero CG guess); returns (iters, relres, status)."""
+        if maxit is None:
+            maxit = 2 * self.n
+        info = _abi.YcStepInfo()
+        rc = _abi.load_library().yc_adjoint_step(self._h, float(h), float(tol), int(maxit), ctypes.byref(info))
+        if rc not in (_abi.YC_OK, _abi.YC_ERR_NOT_CONVERGED):
+            _abi.check(rc)
+        self._adj_steps += 1
+        return info.cg_iters, info.cg_relres, rc
+
+    def adjoint_set_targets(self, lpos, epos, weight):
+        """Targets of the reference's position loss, (max_steps, N, 3) and (max_steps, N, 2):
+        step t's loss is weight (sum |LPos - lpos[t-1]|^2 + sum |EPos - epos[t-1]|^2)."""
+        if lpos is None:
+            _abi.check(_abi.load_library().yc_adjoint_set_targets(self._h, _NULL, _NULL, 0.0))
+            return
+        tx = _arr(lpos, (self._adj_max, self.N, 3))
+        tu = _arr(epos, (self._adj_max, self.N, 2))
+        _abi.check(_abi.load_library().yc_adjoint_set_targets(self._h, _dp(tx), _dp(tu), float(weight)))
+
+    def adjoint_backward(self, h, mode="ref", gstates=None, tol=1e-10, maxit=None, want_state=False, want_ext=False):
+        """Backward pass over the recorded steps.  gstates: per recorded step, the loss
+        gradient with respect to the state after it (a list of get_ClothState-keyed dicts,
+        missing keys zero), or None for the targets' loss.  Returns a dict: "param" (the
+        12 parameter derivatives, PARAMS order), "loss" (the targets' loss), "cg_iters",
+        and on request "state0" (a get_ClothState-keyed dict: the gradient with respect to
+        the recorded initial state) and "ext" ((steps, n): with respect to each step's
+        external force)."""
+        if maxit is None:
+            maxit = 2 * self.n
+        steps, N = self._adj_steps, self.N
+        gs = _NULL
+        if gstates is not None:
+            if len(gstates) != steps:
+                raise ValueError("expected %d state gradients" % steps)
+            flat = np.zeros((steps, 15 * N))
+            offs = {"LPos": (0, 3), "LPosFix": (3, 3), "LVel": (6, 3), "EPos": (9, 2), "EPosBar": (11, 2), "EVel": (13, 2)}
+            for t, g in enumerate(gstates):
+                for k, arr in g.items():
+                    o, w = offs[k]
+                    flat[t, o * N:(o + w) * N] = np.asarray(arr, dtype=np.float64).reshape(N * w)
+            gs = _dp(flat)
+        dparam = np.zeros(len(self.PARAMS))
+        st = np.zeros(15 * N) if want_state else None
+        ext = np.zeros((steps, self.n)) if want_ext else None
+        loss = ctypes.c_double()
+        its = ctypes.c_longlong()
+        rc = _abi.load_library().yc_adjoint_backward(self._h, self.GRAD_MODES[mode], float(h), float(tol), int(maxit), gs,
+                                                      _dp(dparam), _dp(st) if st is not None else _NULL,
+                                                      _dp(ext) if ext is not None else _NULL, ctypes.byref(loss),
+                                                      ctypes.byref(its))
+        _abi.check(rc)
+        out = {"param": dparam, "loss": loss.value, "cg_iters": its.value}
+        if want_state:
+            out["state0"] = {"LPos": st[:3 * N].reshape(N, 3), "LPosFix": st[3 * N:6 * N].reshape(N, 3),
+                             "LVel": st[6 * N:9 * N].reshape(N, 3), "EPos": st[9 * N:11 * N].reshape(N, 2),
+                             "EPosBar": st[11 * N:13 * N].reshape(N, 2), "EVel": st[13 * N:15 * N].reshape(N, 2)}
+        if want_ext:
+            out["ext"] = ext
+        return out
+
+    def adjoint_end(self):
+        _abi.check(_abi.load_library().yc_adjoint_end(self._h))
+
     # CG variants: "reference" (the reference solver's three-kernel order), "fused" (the
     # same recurrences with one kernel and one reduction per iteration) or "persistent"
     # (the fused recurrences in one cooperative launch per solve)

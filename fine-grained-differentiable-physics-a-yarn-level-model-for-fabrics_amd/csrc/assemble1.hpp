@@ -17,7 +17,6 @@ using namespace ys;
 
 template <class T, bool ASM1_REF>
 __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut& O, double h, int tile) {
-    typedef v3<T> V;
     constexpr bool tan = !std::is_same<T, double>::value;
     const int nl = threadIdx.x & (TILE_N - 1);              // node lane in the tile
     const int half = tan ? -1 : (int)(threadIdx.x >> 6);    // -1: every element (tangent run)
@@ -49,12 +48,7 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
     const bool warp_side = half <= 0, weft_side = half != 0;
     if (active) {
         // handle (src/PhysicsCloth.cpp:232-248)
-        if (warp_side && (C.flags[k] & 2)) {
-            V xf = plane3<T>(C.xfix, C.dxfix, C.N, k);
-            T hk = par<T>(C.handle_k, C.dhandle_k);
-            S.FL(k, (-hk) * (pos<T>(C, k) - xf));
-            S.LL(KIND_KX, k, k, I33T<T>(-hk));
-        }
+        if (warp_side && (C.flags[k] & 2)) handle_element<T>(C, S, k);
         // edges (list order: warp rows, then weft columns; src/Cloth.cpp:190-228).  One
         // call site in a loop that is not unrolled: the inlined element code is large, and
         // four or six copies of it would not fit the instruction cache.
@@ -110,6 +104,14 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
     if (active && half <= 0) {
         // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
         T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
+        if constexpr (!tan) if (O.fric_in) {
+            const double fi[FRIC_IN] = {val(S.SBC_L.x), val(S.SBC_L.y), val(S.SBC_L.z),
+                                        val(S.SBC_LE[0].x), val(S.SBC_LE[0].y), val(S.SBC_LE[0].z),
+                                        val(S.SBC_LE[1].x), val(S.SBC_LE[1].y), val(S.SBC_LE[1].z),
+                                        val(S.F[3]), val(S.F[4]), val(S.FEpart[0]), val(S.FEpart[1]),
+                                        val(S.KxEE[0]), val(S.KxEE[1]), val(S.KvEE[0]), val(S.KvEE[1])};
+            for (int i = 0; i < FRIC_IN; ++i) O.fric_in[(size_t)i * C.N + k] = S.inter ? fi[i] : 0.0;
+        }
         if (S.inter) comp = friction<T>(C, S, sl);
         // b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:20, 63)
         int pk = padidx(O.Wp, u, v);

@@ -32,7 +32,15 @@ struct AsmOut {
     const double* bend_rec;
     // value run: -1 for A; 0, 1, 2 for M, K_x or K_v alone (RowSinkT::select_kind)
     int only_kind;
+    // value run, when set: the friction's accumulated inputs of every node (FRIC_IN planes
+    // of N, zero off the interior), which the adjoint reads back
+    double* fric_in;
 };
+
+// The friction's inputs accumulated over a node's elements, as planes of fric_in and of
+// the adjoint weights: the compression sums SBC_L (3) and SBC_LE (2 x 3), the E force
+// before friction (2), its thread partial (2), the K_x and K_v E-E self entries (2 + 2).
+enum FricIn { FI_SBCL = 0, FI_SBCLE = 3, FI_FNS = 9, FI_FEP = 11, FI_KXEE = 13, FI_KVEE = 15, FRIC_IN = 17 };
 
 __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (v + 2) * Wp; }
 

@@ -63,15 +63,42 @@ struct ClothDev {
 template <class T> YS_DEV T par(double v, double d);
 template <> YS_DEV double par<double>(double v, double) { return v; }
 template <> YS_DEV dd par<dd>(double v, double d) { return dd(v, d); }
-YS_DEV double tget(const double* p, int i) { return p ? p[i] : 0.0; }
+
+// Seeded tangents of the adjoint kernels.  There a state field's tangent pointer in
+// ClothDev is not an array but a small field code (SEED_X ... SEED_COMP, below
+// SEED_MAX), and the tangent of that field is the unit seed of the calling thread: 1 at
+// the node, field and components the thread has put into its LDS seed slot
+// (ys_seed_node, ys_seed_fm = field << 4 | component mask), 0 everywhere else.  One
+// kernel thus differentiates the element code along a different input in every thread.
+enum SeedField { SEED_X = 1, SEED_V, SEED_XFIX, SEED_UV, SEED_UVD, SEED_UVBAR, SEED_NRM, SEED_COMP, SEED_MAX = 16 };
+constexpr int SEED_THREADS = 256;
+__shared__ int ys_seed_node[SEED_THREADS];
+__shared__ int ys_seed_fm[SEED_THREADS];
+YS_DEV void set_seed(int node, int field, int mask) {
+    ys_seed_node[threadIdx.x] = node;
+    ys_seed_fm[threadIdx.x] = field << 4 | mask;
+}
+__host__ __device__ inline const double* seed_ptr(int field) { return reinterpret_cast<const double*>((size_t)field); }
+// tangent of component c of node k of the field whose tangent pointer is p (planes of N)
+YS_DEV double tan_at(const double* p, int N, int k, int c) {
+    const size_t f = reinterpret_cast<size_t>(p);
+    if (f < SEED_MAX) {
+        if (f == 0) return 0.0;
+        const int fm = ys_seed_fm[threadIdx.x];
+        return (k == ys_seed_node[threadIdx.x] && (fm >> 4) == (int)f && ((fm >> c) & 1)) ? 1.0 : 0.0;
+    }
+    return p[(size_t)c * N + k];
+}
 
 template <class T> YS_DEV v3<T> plane3(const double* a, const double* da, int N, int k) {
-    return V3<T>(par<T>(a[k], tget(da, k)), par<T>(a[N + k], tget(da, N + k)), par<T>(a[2 * N + k], tget(da, 2 * N + k)));
+    return V3<T>(par<T>(a[k], tan_at(da, N, k, 0)), par<T>(a[N + k], tan_at(da, N, k, 1)),
+                 par<T>(a[2 * N + k], tan_at(da, N, k, 2)));
 }
 template <class T = double> YS_DEV v3<T> pos(const ClothDev& C, int k) { return plane3<T>(C.x, C.dx, C.N, k); }
 template <class T = double> YS_DEV v3<T> vel(const ClothDev& C, int k) { return plane3<T>(C.v, C.dv, C.N, k); }
-template <class T = double> YS_DEV T uvc(const ClothDev& C, int k, int c) { return par<T>(C.uv[c * C.N + k], tget(C.duv, c * C.N + k)); }
-template <class T = double> YS_DEV T uvdc(const ClothDev& C, int k, int c) { return par<T>(C.uvd[c * C.N + k], tget(C.duvd, c * C.N + k)); }
+template <class T = double> YS_DEV T uvc(const ClothDev& C, int k, int c) { return par<T>(C.uv[c * C.N + k], tan_at(C.duv, C.N, k, c)); }
+template <class T = double> YS_DEV T uvdc(const ClothDev& C, int k, int c) { return par<T>(C.uvd[c * C.N + k], tan_at(C.duvd, C.N, k, c)); }
+template <class T = double> YS_DEV T uvbarc(const ClothDev& C, int k, int c) { return par<T>(C.uvbar[c * C.N + k], tan_at(C.duvbar, C.N, k, c)); }
 YS_DEV bool interior(const ClothDev& C, int u, int v) { return u > 0 && u < C.W - 1 && v > 0 && v < C.Lg - 1; }
 YS_DEV bool flip(const ClothDev& C, int k) { return (C.flags[k] & 1) != 0; }
 YS_DEV int crossing(const ClothDev& C, int u, int v) { return (u - 1) + (v - 1) * (C.W - 2); }
@@ -224,8 +251,8 @@ struct RowSinkT {
                      case 3: a[3] += x; break; default: a[4] += x; break; }
     }
     YS_DEV static void add2(T (&a)[2], int i, T x) { if (i == 0) a[0] += x; else a[1] += x; }
-    YS_DEV T velL(int k, int c) { return par<T>(C->v[c * C->N + k], tget(C->dv, c * C->N + k)); }
-    YS_DEV T velE(int k, int c) { return par<T>(C->uvd[c * C->N + k], tget(C->duvd, c * C->N + k)); }
+    YS_DEV T velL(int k, int c) { return par<T>(C->v[c * C->N + k], tan_at(C->dv, C->N, k, c)); }
+    YS_DEV T velE(int k, int c) { return uvdc<T>(*C, k, c); }
     // a block entry in a product with the velocity (value only when detached)
     YS_DEV T blk(T b) const { return (REF || detach) ? T(val(b)) : b; }
 
@@ -295,6 +322,7 @@ struct RowSinkT {
         if (p[(size_t)i * stride] != 0.0) bad = 1;
     }
     YS_DEV void FL(int k, v3<T> f) { if (k == me) { F[0] += f.x; F[1] += f.y; F[2] += f.z; } }
+    YS_DEV void addF(int i, T f) { add5(F, i, f); }   // a force row of the node itself (the friction)
     // E force on the crossing of node k; elem_owner is the static owner thread of the
     // emitting element in its own loop, for the partial sum read by the friction
     YS_DEV void FE(int k, int c, T f, int elem_owner) {
@@ -587,8 +615,10 @@ YS_DEV BendVals<T> bend_values(v3<T> x2, v3<T> x0, v3<T> x1, T D, T kb) {
 }
 
 // bend along yarn direction c centred on node (uc, vc): n2 = previous, n1 = next
+// parts: bit 0 the bend of the crossing nodes, bit 1 its crimp twin (the adjoint passes
+// that seed the normals need only the twin)
 template <class T, class Sink>
-YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int yarn, int owner) {
+YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int yarn, int owner, int parts = 3) {
     typedef v3<T> V;
     typedef mat3<T> M;
     int du = c == 0 ? 1 : 0, dv = c == 1 ? 1 : 0;
@@ -597,7 +627,7 @@ YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int 
     bool i2 = interior(C, u2, v2), i0 = interior(C, uc, vc), i1 = interior(C, u1, v1);
     T kb = par<T>(C.kb[yarn], C.dkb[yarn]);
     // Bending_F / Bending_D on the crossing nodes
-    {
+    if (parts & 1) {
         T D = uvc<T>(C, k1, c) - uvc<T>(C, k2, c);
         BendVals<T> o = bend_values<T>(pos<T>(C, k2), pos<T>(C, k0), pos<T>(C, k1), D, kb);
         S.FL(k2, o.Fx2); S.FL(k0, o.Fx0); S.FL(k1, o.Fx1);
@@ -619,7 +649,7 @@ YS_DEV void bend_element(const ClothDev& C, Sink& S, int uc, int vc, int c, int 
     }
     // Crimp_Bending_F / _D on the +-R n copies: compression inputs only (src/PhysicsCloth.cpp:824-987);
     // the copies keep their construction-time Eulerian coordinates (u L, v L)
-    {
+    if (parts & 2) {
         T D = c == 0 ? (double)u1 * C.L - (double)u2 * C.L : (double)v1 * C.L - (double)v2 * C.L;
         BendVals<T> o = bend_values<T>(crimp_pos<T>(C, u2, v2, c), crimp_pos<T>(C, uc, vc, c), crimp_pos<T>(C, u1, v1, c), D, kb);
         T sg = c == 0 ? 1.0 : -1.0;
@@ -731,6 +761,17 @@ YS_DEV void emit_bend_role(const ClothDev& C, Sink& S, const double* rec, int nb
 }
 
 // ---------------------------------------------------------------------------
+// Constraint_F / Constraint_D of a handle node k (src/PhysicsCloth.cpp:232-248)
+// ---------------------------------------------------------------------------
+template <class T, class Sink>
+YS_DEV void handle_element(const ClothDev& C, Sink& S, int k) {
+    v3<T> xf = plane3<T>(C.xfix, C.dxfix, C.N, k);
+    T hk = par<T>(C.handle_k, C.dhandle_k);
+    S.FL(k, (-hk) * (pos<T>(C, k) - xf));
+    S.LL(KIND_KX, k, k, I33T<T>(-hk));
+}
+
+// ---------------------------------------------------------------------------
 // Wind_F on one face (src/PhysicsCloth.cpp:3091-3106; src/ClothUnit.h:268-273)
 // ---------------------------------------------------------------------------
 template <class T, class Sink>
@@ -763,7 +804,7 @@ YS_DEV T friction(const ClothDev& C, Sink& S, T* is_slide) {
     comp = comp > 0.0 ? comp : 0.0;
     T lim = comp * par<T>(C.mu, C.dmu);
     for (int c = 0; c < 2; ++c) {
-        T dl = uvc<T>(C, k, c) - par<T>(C.uvbar[c * C.N + k], tget(C.duvbar, c * C.N + k));
+        T dl = uvc<T>(C, k, c) - uvbarc<T>(C, k, c);
         T cEE = 0.5 * dot(n, S.SBC_LE[c]);
         T FNS = S.F[3 + c];          // F_E before friction (F_E_No_Slide)
         T Fpt = S.FEpart[c];         // thread-partial F_Es[ID] (Q2)
@@ -777,7 +818,7 @@ YS_DEV T friction(const ClothDev& C, Sink& S, T* is_slide) {
                       (0.5 * (par<T>(C.kf, C.dkf) + (1.0 - pow(th, 2.0)) * par<T>(C.mu, C.dmu) * comp + th * par<T>(C.mu, C.dmu) * cEE));
         T tq = tanh(10.0 * lim - FNS);   // (10*fric_lim - F)[c] (Q3)
         T fvel = (-0.5 * (par<T>(C.kf, C.dkf) * dl - th * par<T>(C.mu, C.dmu) * comp)) * (1.0 - pow(tq, 2.0)) * S.KvEE[c] - par<T>(C.df, C.ddf);
-        S.F[3 + c] += fric;
+        S.addF(3 + c, fric);
         S.EE(KIND_KX, k, k, c, fpos);
         S.EE(KIND_KV, k, k, c, fvel);
     }

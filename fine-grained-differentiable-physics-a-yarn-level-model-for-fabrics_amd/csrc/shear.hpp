@@ -32,7 +32,7 @@ __device__ __forceinline__ void shear_compute(const ClothDev& C, int k0, int k1,
     V d1 = V3<T>(a.x / l1, a.y / l1, a.z / l1), d3v = V3<T>(b3.x / l3, b3.y / l3, b3.z / l3);
     M P1 = projector(d1), P3 = projector(d3v);
     T phi = acos(dot(d1, d3v));
-    T cp = par<T>(comp_arr[k0], tget(dcomp_arr, k0));
+    T cp = par<T>(comp_arr[k0], tan_at(dcomp_arr, 0, k0, 0));
     T kx = (cp + 1.0) * par<T>(C.S, C.dS) * R * R * PI_REF;
     T sn = sin(phi), cs = cos(phi);
     V P1d3 = P1 * d3v, P3d1 = P3 * d1;

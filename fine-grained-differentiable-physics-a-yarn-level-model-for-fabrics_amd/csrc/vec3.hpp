@@ -54,7 +54,8 @@ YS_DEV dd acos(dd a) {
     return dd(::acos(a.v), s > 0.0 ? -a.d / s : 0.0);
 }
 YS_DEV dd tanh(dd a) { double t = ::tanh(a.v); return dd(t, (1.0 - t * t) * a.d); }
-YS_DEV dd exp(dd a) { double e = ::exp(a.v); return dd(e, e * a.d); }
+// a zero tangent stays zero where exp overflows (inf * 0 would be NaN)
+YS_DEV dd exp(dd a) { double e = ::exp(a.v); return dd(e, a.d == 0.0 ? 0.0 : e * a.d); }
 YS_DEV dd fabs(dd a) { return a.v < 0.0 ? -a : a; }
 YS_DEV dd pow(dd a, double p) { double r = ::pow(a.v, p); return dd(r, p * ::pow(a.v, p - 1.0) * a.d); }
 YS_DEV dd fmax(dd a, double b) { return a.v >= b ? a : dd(b); }

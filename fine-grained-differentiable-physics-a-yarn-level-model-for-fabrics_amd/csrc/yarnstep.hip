@@ -1522,6 +1522,8 @@ __global__ void __launch_bounds__(NBLK) k_transpose(const double* A, double* At,
 
 }  // namespace
 
+#include "adjoint.hpp"
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
@@ -1584,7 +1586,25 @@ struct yc_ctx {
     double last_h;                          // step of the last assembly
     double* Aexp;                           // M, K_x or K_v alone, for yc_export_matrix (on first use)
     double* dexp;                           // the diagonal scratch of that assembly (5 P)
+    double* fric_out;                       // when set: the assembly writes the friction inputs here
+    // reverse mode (yc_adjoint_*): checkpoints of the recorded rollout and the backward
+    // pass's buffers
+    int adj_max, adj_K, adj_steps;          // capacity, checkpoint interval, steps recorded
+    double* adj_ckpt;                       // blocks of G_PLANES N: s_0, s_K, s_2K, ...
+    double* adj_final;                      // the state after the last recorded step
+    double* adj_seg;                        // K blocks: a segment's states, recomputed
+    double* adj_ext;                        // per-step external force (checkpoint interval > 1), or null
+    std::vector<char> adj_has_ext;          // whether step t had one
+    double *adj_g0, *adj_g1;                // state adjoints (G_PLANES N each)
+    double *adj_gpar, *adj_gcomp, *adj_ybar, *adj_pbar, *adj_gslide, *adj_fric_in, *adj_lam, *adj_xs;
+    double *adj_gst, *adj_lpart, *adj_sums;
+    double *adj_tx, *adj_tu;                // target positions / Eulerian coordinates (steps x N x 3 / 2), or null
+    double adj_tw;                          // their loss weight
 };
+
+namespace {
+void adjoint_free(yc_ctx* c);
+}
 
 namespace {
 
@@ -1600,6 +1620,7 @@ AsmOut make_out(yc_ctx* c) {
     AsmOut O;
     O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
     O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
+    O.fric_in = c->fric_out;
     O.dext = nullptr;
     O.detach = 0;
     O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
@@ -2256,6 +2277,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
 void yc_destroy(yc_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    adjoint_free(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     void* bufs[] = {c->x, c->v, c->xfix, c->uv, c->uvd, c->uvbar, c->nrm, c->flags, c->A, c->F, c->is_slide, c->comp,
@@ -3427,6 +3449,370 @@ int yc_run_warm(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit
     }
     if (total_iters) *total_iters = tot;
     return worst;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reverse mode: a recorded rollout and its backward pass (adjoint.hpp)
+// ---------------------------------------------------------------------------
+namespace {
+
+void adjoint_free(yc_ctx* c) {
+    double** bufs[] = {&c->adj_ckpt, &c->adj_final, &c->adj_seg, &c->adj_ext, &c->adj_g0, &c->adj_g1, &c->adj_gpar,
+                       &c->adj_gcomp, &c->adj_ybar, &c->adj_pbar, &c->adj_gslide, &c->adj_fric_in, &c->adj_lam, &c->adj_xs,
+                       &c->adj_gst, &c->adj_lpart, &c->adj_sums, &c->adj_tx, &c->adj_tu};
+    for (double** p : bufs) { if (*p) (void)hipFree(*p); *p = nullptr; }
+    c->adj_max = c->adj_K = c->adj_steps = 0;
+    c->adj_has_ext.clear();
+    c->fric_out = nullptr;
+}
+
+// The descriptor of the adjoint kernels: the state's tangent pointers are seed codes
+// (tan_at), the parameters' tangents zero unless a parameter pass sets them.
+ClothDev make_adj_dev(yc_ctx* c, int mode) {
+    ClothDev C = make_dev(c);
+    C.dx = seed_ptr(SEED_X); C.dv = seed_ptr(SEED_V); C.dxfix = seed_ptr(SEED_XFIX); C.duv = seed_ptr(SEED_UV);
+    C.duvd = seed_ptr(SEED_UVD); C.duvbar = seed_ptr(SEED_UVBAR); C.dnrm = seed_ptr(SEED_NRM);
+    C.ref_mode = mode == YC_GRAD_REF;
+    return C;
+}
+
+size_t adj_block(const yc_ctx* c) { return (size_t)G_PLANES * c->N; }
+
+StateDev block_state(yc_ctx* c, double* blk) {
+    StateDev s = make_state(c);
+    const size_t N = c->N;
+    s.x = blk + G_X * N; s.v = blk + G_V * N; s.xfix = blk + G_XFIX * N; s.uv = blk + G_UV * N;
+    s.uvd = blk + G_UVD * N; s.uvbar = blk + G_UVBAR * N; s.nrm = blk + G_NRM * N;
+    return s;
+}
+
+int ckpt_io(yc_ctx* c, double* blk, int save) {
+    hipLaunchKernelGGL(k_ckpt, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), blk, save);
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+// One recorded step, as yc_step with the zero CG guess (a recomputed step repeats it bit for bit).
+int adj_forward_step(yc_ctx* c, double h, double tol, int maxit, int* iters, double* relres) {
+    int rc = do_assemble(c, h);
+    if (!rc) rc = do_solve(c, nullptr, tol, maxit, iters, relres);
+    if (rc == YC_ERR_NONFINITE || rc == YC_ERR_HIP || rc == YC_ERR_ARG) return rc;
+    int rc2 = do_update(c, h);
+    return rc2 ? rc2 : rc;
+}
+
+template <bool REF>
+int adj_launch(yc_ctx* c, const ClothDev& Cadj, double h) {
+    const int nT = c->nT;
+    AdjW w;
+    w.lam = c->adj_lam; w.xs = c->adj_xs; w.ybar = c->adj_ybar; w.Wp = c->Wp; w.P = c->P; w.h = h;
+    AdjOut o;
+    o.g = c->adj_g1; o.gpar = c->adj_gpar; o.gcomp = c->adj_gcomp; o.ybar = c->adj_ybar;
+    ClothDev Cs = Cadj;
+    Cs.S = 1.0;   // k_adj_shear_par: the shear at S = 1 is its S derivative
+    if (c->Nc > 0) {
+        hipLaunchKernelGGL(k_adj_shear<REF>, dim3(nT), dim3(TILE_N), 0, c->stream, Cadj, w, o, (const double*)c->comp);
+        hipLaunchKernelGGL(k_adj_shear_par<REF>, dim3(nT), dim3(TILE_N), 0, c->stream, Cs, w, o, (const double*)c->comp);
+    } else {
+        HIPCHK(hipMemsetAsync(c->adj_gcomp, 0, (size_t)c->N * sizeof(double), c->stream));
+    }
+    hipLaunchKernelGGL(k_adj_fric<REF>, dim3(nT), dim3(TILE_N), 0, c->stream, Cadj, w, o, (const double*)c->adj_fric_in,
+                       (const double*)c->adj_gslide, -1);
+    for (int p = 0; p < 3; ++p) {
+        ClothDev Cp = Cadj;
+        if (p == 0) Cp.dmu = 1.0;
+        else if (p == 1) Cp.dkf = 1.0;
+        else Cp.ddf = 1.0;
+        hipLaunchKernelGGL(k_adj_fric<REF>, dim3(nT), dim3(TILE_N), 0, c->stream, Cp, w, o, (const double*)c->adj_fric_in,
+                           (const double*)c->adj_gslide, p);
+    }
+    hipLaunchKernelGGL(k_adj_elems<REF>, dim3(nT), dim3(TILE_N), 0, c->stream, Cadj, w, o);
+    for (int p = 0; p < 3; ++p) {
+        ClothDev Cp = Cadj;
+        if (p == 0) { Cp.drho[0] = Cp.drho[1] = 1.0; Cp.dkb[0] = Cp.dkb[1] = 1.0; Cp.dhandle_k = 1.0; }
+        else if (p == 1) { Cp.dks[0] = Cp.dks[1] = 1.0; }
+        else Cp.dkc = 1.0;
+        hipLaunchKernelGGL(k_adj_elems_par<REF>, dim3(nT), dim3(TILE_N), 0, c->stream, Cp, w, o, p);
+    }
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+// The loss gradient of the state after recorded step t (1-based), into adj_g1, from the
+// caller's array (host) or the targets.
+int adj_add_loss(yc_ctx* c, int t, const double* gstate, double* state_blk) {
+    const size_t N = c->N;
+    if (gstate) {
+        HIPCHK(hipMemcpyAsync(c->adj_gst, gstate + (size_t)(t - 1) * 15 * N, 15 * N * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        hipLaunchKernelGGL(k_adj_add_gstate, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, c->N,
+                           (const double*)c->adj_gst, c->adj_g1);
+    } else if (c->adj_tx) {
+        hipLaunchKernelGGL(k_adj_target_loss, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, c->N,
+                           (const double*)(state_blk + G_X * N), (const double*)(state_blk + G_UV * N),
+                           (const double*)(c->adj_tx + (size_t)(t - 1) * 3 * N), (const double*)(c->adj_tu + (size_t)(t - 1) * 2 * N),
+                           c->adj_tw, c->adj_g1, c->adj_lpart);
+    }
+    HIPCHK(hipGetLastError());
+    return YC_OK;
+}
+
+// Copies the state adjoint block g (G_PLANES planes) to the yc_get_state layout.
+void adj_to_state_layout(const yc_ctx* c, const std::vector<double>& g, double* out) {
+    const size_t N = c->N;
+    for (size_t k = 0; k < N; ++k) {
+        for (int q = 0; q < 3; ++q) {
+            out[3 * k + q] = g[(G_X + q) * N + k];
+            out[3 * N + 3 * k + q] = g[(G_XFIX + q) * N + k];
+            out[6 * N + 3 * k + q] = g[(G_V + q) * N + k];
+        }
+        for (int q = 0; q < 2; ++q) {
+            out[9 * N + 2 * k + q] = g[(G_UV + q) * N + k];
+            out[11 * N + 2 * k + q] = g[(G_UVBAR + q) * N + k];
+            out[13 * N + 2 * k + q] = g[(G_UVD + q) * N + k];
+        }
+    }
+}
+
+template <bool REF>
+int adjoint_backward(yc_ctx* c, double h, double tol, int maxit, const double* gstate, double* dparam, double* dstate0,
+                     double* dext, double* loss, long long* cg_iters) {
+    const int T = c->adj_steps, K = c->adj_K;
+    const size_t N = c->N, B = adj_block(c), PV = (size_t)5 * c->P;
+    const int mode = REF ? YC_GRAD_REF : YC_GRAD_EXACT;
+    const ClothDev Cadj = make_adj_dev(c, mode);
+    const int nblk_n = (c->N + NBLK - 1) / NBLK;
+    long long iters_total = 0;
+    int rc = ckpt_io(c, c->adj_final, 1);   // the state after the last step, restored at the end
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->adj_g1, 0, B * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->adj_gpar, 0, (size_t)YC_NPARAM * N * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->adj_lpart, 0, N * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->adj_lam, 0, PV * sizeof(double), c->stream));
+    rc = adj_add_loss(c, T, gstate, c->adj_final);
+    if (rc) return rc;
+    std::swap(c->adj_g0, c->adj_g1);   // g0: the adjoint of the state after the step being reversed
+    std::vector<double> hl(PV);
+    std::vector<double> glob(c->n);
+    c->fric_out = c->adj_fric_in;
+    const bool had_ext = c->has_ext;
+    double* ext_save = c->adj_ext ? c->adj_ext + (size_t)c->adj_max * c->n : nullptr;
+    if (ext_save && had_ext)
+        HIPCHK(hipMemcpyAsync(ext_save, c->ext, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    auto state_ptr = [&](int n, int seg0) -> double* {
+        if (n == T) return c->adj_final;
+        if (n % K == 0) return c->adj_ckpt + (size_t)(n / K) * B;
+        return c->adj_seg + (size_t)(n - seg0) * B;
+    };
+    for (int q = (T - 1) / K; q >= 0 && !rc; --q) {
+        const int s0 = q * K, s1 = std::min(s0 + K, T);
+        // recompute the segment's states after its checkpoint
+        if (s1 - s0 > 1) {
+            rc = ckpt_io(c, c->adj_ckpt + (size_t)q * B, 0);
+            for (int n = s0 + 1; n < s1 && !rc; ++n) {
+                if (c->adj_ext && c->adj_has_ext[n - 1]) {
+                    HIPCHK(hipMemcpyAsync(c->ext, c->adj_ext + (size_t)(n - 1) * c->n, (size_t)c->n * sizeof(double),
+                                          hipMemcpyDeviceToDevice, c->stream));
+                    c->has_ext = true;
+                } else if (c->adj_ext) {
+                    c->has_ext = false;
+                }
+                rc = adj_forward_step(c, h, tol, maxit, nullptr, nullptr);
+                if (rc == YC_ERR_NOT_CONVERGED) rc = YC_OK;   // recorded as it went forward
+                if (!rc) rc = ckpt_io(c, c->adj_seg + (size_t)(n - s0) * B, 1);
+            }
+        }
+        for (int n = s1 - 1; n >= s0 && !rc; --n) {
+            double* sn = state_ptr(n, s0);
+            double* sn1 = state_ptr(n + 1, s0);
+            // 1. normals' adjoint at the state after the step
+            hipLaunchKernelGGL(k_adj_nrm, dim3(nblk_n), dim3(NBLK), 0, c->stream, block_state(c, sn1),
+                               (const double*)(c->adj_g0 + G_NRM * N), c->adj_pbar);
+            // 2. recompute the step's system and friction inputs
+            rc = ckpt_io(c, sn, 0);
+            if (!rc) rc = do_assemble(c, h);
+            if (rc) break;
+            hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)(sn1 + G_V * N),
+                               (const double*)(sn1 + G_UVD * N), c->W, c->Lg, c->N, c->Wp, c->P, c->adj_xs);
+            // 3. the update's adjoint: the right-hand side of the backward solve in b
+            hipLaunchKernelGGL(k_adj_update, dim3(nblk_n), dim3(NBLK), 0, c->stream, (const unsigned char*)c->flags, c->W, c->Lg,
+                               (const double*)c->adj_g0, (const double*)c->adj_pbar, (const double*)c->is_slide,
+                               (const double*)c->adj_xs, c->Wp, c->P, h, c->b, c->adj_gslide, c->adj_g1);
+            // 4. the backward solve, from the previous step's lambda
+            const double* Au = c->A;
+            if (!REF) {
+                if (c->bj_of == c->At) c->bj_of = nullptr;   // At is rewritten
+                hipLaunchKernelGGL(k_transpose, dim3(c->nT), dim3(NBLK), 0, c->stream, (const double*)c->A, c->At, c->W, c->Lg, c->ntx);
+                Au = c->At;
+            }
+            HIPCHK(hipMemcpyAsync(c->cx, c->adj_lam, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->cx, c->cy, Au), (double*)nullptr,
+                               (const CGScal*)c->sc, 0);
+            int it = 0;
+            rc = solve_on(c, Au, tol, maxit, &it, true);
+            iters_total += it;
+            if (rc) break;
+            HIPCHK(hipMemcpyAsync(c->adj_lam, c->cx, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            if (dext) {   // the adjoint of the step's external force: h lambda
+                HIPCHK(hipMemcpyAsync(hl.data(), c->adj_lam, PV * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                from_padded(c, hl, glob.data());
+                for (int i = 0; i < c->n; ++i) dext[(size_t)n * c->n + i] = h * glob[i];
+            }
+            // 5. the assembly's vector-Jacobian product, into g1 and the parameter partials
+            rc = adj_launch<REF>(c, Cadj, h);
+            if (rc) break;
+            // 6. the loss gradient of the state before the step (the states after steps 1..T)
+            if (n >= 1) rc = adj_add_loss(c, n, gstate, sn);
+            if (rc) break;
+            std::swap(c->adj_g0, c->adj_g1);
+        }
+    }
+    c->fric_out = nullptr;
+    if (!rc) {   // the initial state's normals come from its x (yc_set_state)
+        hipLaunchKernelGGL(k_adj_nrm, dim3(nblk_n), dim3(NBLK), 0, c->stream, block_state(c, c->adj_ckpt),
+                           (const double*)(c->adj_g0 + G_NRM * N), c->adj_pbar);
+        hipLaunchKernelGGL(k_adj_fold_nrm, dim3(nblk_n), dim3(NBLK), 0, c->stream, c->W, c->Lg, (const double*)c->adj_pbar, c->adj_g0);
+        hipLaunchKernelGGL(k_plane_sums, dim3(YC_NPARAM), dim3(256), 0, c->stream, (const double*)c->adj_gpar, c->N, c->adj_sums);
+        hipLaunchKernelGGL(k_plane_sums, dim3(1), dim3(256), 0, c->stream, (const double*)c->adj_lpart, c->N, c->adj_sums + YC_NPARAM);
+        HIPCHK(hipGetLastError());
+        double sums[YC_NPARAM + 1];
+        HIPCHK(hipMemcpyAsync(sums, c->adj_sums, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> g0;
+        if (dstate0) {
+            g0.resize(B);
+            HIPCHK(hipMemcpyAsync(g0.data(), c->adj_g0, B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // ks = Y pi R^2 and kb = B pi R^2 (src/Yarn.h:14-15); a one-yarn cloth's second
+        // yarn slot is its first yarn
+        const double area = PI_REF * std::pow(c->d.R, 2);
+        sums[YC_P_Y0] *= area; sums[YC_P_Y1] *= area; sums[YC_P_B0] *= area; sums[YC_P_B1] *= area;
+        if (c->d.n_yarns == 1) {
+            sums[YC_P_RHO0] += sums[YC_P_RHO1]; sums[YC_P_Y0] += sums[YC_P_Y1]; sums[YC_P_B0] += sums[YC_P_B1];
+            sums[YC_P_RHO1] = sums[YC_P_Y1] = sums[YC_P_B1] = 0.0;
+        }
+        if (dparam) for (int i = 0; i < YC_NPARAM; ++i) dparam[i] = sums[i];
+        if (loss) *loss = sums[YC_NPARAM];
+        if (dstate0) adj_to_state_layout(c, g0, dstate0);
+    }
+    if (cg_iters) *cg_iters = iters_total;
+    // back to the state after the last recorded step
+    int rc2 = ckpt_io(c, c->adj_final, 0);
+    if (ext_save) {   // the caller's external force as it was
+        c->has_ext = had_ext;
+        if (had_ext && hipMemcpyAsync(c->ext, ext_save, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            rc2 = fail(YC_ERR_HIP, "restoring the external force");
+    }
+    if (!rc) rc = rc2;
+    (void)hipStreamSynchronize(c->stream);
+    c->assembled = false;
+    c->solved = false;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yc_adjoint_end(yc_ctx* c) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    (void)hipStreamSynchronize(c->stream);
+    adjoint_free(c);
+    return YC_OK;
+}
+
+int yc_adjoint_begin(yc_ctx* c, int max_steps, int checkpoint_every) {
+    if (!c || max_steps <= 0 || checkpoint_every <= 0) return fail(YC_ERR_ARG, "bad argument");
+    g_err.clear();
+    yc_adjoint_end(c);
+    const size_t N = c->N, B = adj_block(c), PV = (size_t)5 * c->P;
+    const int K = std::min(checkpoint_every, max_steps);
+    int rc = YC_OK;
+    auto chk = [&](int r) { if (r && !rc) rc = r; };
+    chk(dalloc(&c->adj_ckpt, ((size_t)max_steps / K + 1) * B));
+    chk(dalloc(&c->adj_final, B));
+    if (K > 1) chk(dalloc(&c->adj_seg, (size_t)K * B));
+    chk(dalloc(&c->adj_g0, B)); chk(dalloc(&c->adj_g1, B));
+    chk(dalloc(&c->adj_gpar, (size_t)YC_NPARAM * N)); chk(dalloc(&c->adj_gcomp, N)); chk(dalloc(&c->adj_ybar, (size_t)FRIC_IN * N));
+    chk(dalloc(&c->adj_pbar, 15 * N)); chk(dalloc(&c->adj_gslide, 2 * N)); chk(dalloc(&c->adj_fric_in, (size_t)FRIC_IN * N));
+    chk(dalloc(&c->adj_lam, PV)); chk(dalloc(&c->adj_xs, PV));
+    chk(dalloc(&c->adj_gst, 15 * N)); chk(dalloc(&c->adj_lpart, N)); chk(dalloc(&c->adj_sums, YC_NPARAM + 1));
+    if (rc) { adjoint_free(c); return rc; }
+    c->adj_max = max_steps;
+    c->adj_K = K;
+    c->adj_steps = 0;
+    rc = ckpt_io(c, c->adj_ckpt, 1);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(YC_ERR_HIP, "sync");
+    if (rc) adjoint_free(c);
+    return rc;
+}
+
+int yc_adjoint_step(yc_ctx* c, double h, double tol, int maxit, yc_step_info* info) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (c->adj_max <= 0) return fail(YC_ERR_ARG, "yc_adjoint_step before yc_adjoint_begin");
+    if (c->adj_steps >= c->adj_max) return fail(YC_ERR_ARG, "more steps than yc_adjoint_begin allowed");
+    g_err.clear();
+    const int t = c->adj_steps;
+    if (c->adj_K > 1) {   // a recomputed segment needs the step's external force
+        if (c->has_ext && !c->adj_ext) {   // one slot per step and one for the caller's force during a backward pass
+            int r = dalloc(&c->adj_ext, (size_t)(c->adj_max + 1) * c->n);
+            if (r) return r;
+        }
+        c->adj_has_ext.resize(c->adj_max, 0);
+        c->adj_has_ext[t] = c->has_ext ? 1 : 0;
+        if (c->has_ext)
+            HIPCHK(hipMemcpyAsync(c->adj_ext + (size_t)t * c->n, c->ext, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
+    int it = 0;
+    double rr = 0;
+    int rc = adj_forward_step(c, h, tol, maxit, &it, &rr);
+    if (info) { info->cg_iters = it; info->cg_relres = rr; info->status = rc; }
+    if (rc && rc != YC_ERR_NOT_CONVERGED) return rc;
+    c->adj_steps = t + 1;
+    if (c->adj_steps % c->adj_K == 0 && c->adj_steps < c->adj_max) {
+        int r = ckpt_io(c, c->adj_ckpt + (size_t)(c->adj_steps / c->adj_K) * adj_block(c), 1);
+        if (r) return r;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+int yc_adjoint_set_targets(yc_ctx* c, const double* lpos, const double* epos, double weight) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (c->adj_max <= 0) return fail(YC_ERR_ARG, "yc_adjoint_set_targets before yc_adjoint_begin");
+    if (c->adj_tx) { (void)hipStreamSynchronize(c->stream); (void)hipFree(c->adj_tx); (void)hipFree(c->adj_tu); }
+    c->adj_tx = c->adj_tu = nullptr;
+    if (!lpos && !epos) return YC_OK;
+    if (!lpos || !epos) return fail(YC_ERR_ARG, "targets need both positions and Eulerian coordinates");
+    const size_t N = c->N, T = c->adj_max;
+    int rc = dalloc(&c->adj_tx, T * 3 * N);
+    if (!rc) rc = dalloc(&c->adj_tu, T * 2 * N);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->adj_tx, lpos, T * 3 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->adj_tu, epos, T * 2 * N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->adj_tw = weight;
+    return YC_OK;
+}
+
+int yc_adjoint_backward(yc_ctx* c, int mode, double h, double tol, int maxit, const double* gstate, double* dparam,
+                        double* dstate0, double* dext, double* loss, long long* cg_iters) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (c->adj_max <= 0) return fail(YC_ERR_ARG, "yc_adjoint_backward before yc_adjoint_begin");
+    if (c->adj_steps <= 0) return fail(YC_ERR_ARG, "no recorded step");
+    if (mode != YC_GRAD_REF && mode != YC_GRAD_EXACT) return fail(YC_ERR_ARG, "unknown gradient mode");
+    if (c->precond != YC_PRECOND_JACOBI && !c->coop_ok) return fail(YC_ERR_ARG, "the block-Jacobi preconditioner needs the persistent CG");
+    g_err.clear();
+    if (mode == YC_GRAD_EXACT && !c->At) {   // A^T of the exact adjoint's solves (shared with the tangents)
+        int r = dalloc(&c->At, c->a_size);
+        if (r) return r;
+    }
+    return mode == YC_GRAD_REF ? adjoint_backward<true>(c, h, tol, maxit, gstate, dparam, dstate0, dext, loss, cg_iters)
+                               : adjoint_backward<false>(c, h, tol, maxit, gstate, dparam, dstate0, dext, loss, cg_iters);
 }
 
 }  // extern "C"

@@ -349,6 +349,40 @@ int yc_tangent_get(yc_ctx* ctx, int dir, double* dlpos, double* dlposfix, double
 int yc_tangent_end(yc_ctx* ctx);
 
 /* ---------------------------------------------------------------------------------
+ * Differentiation: reverse mode (adjoint) of a recorded rollout.
+ * Replaces: loss.backward() through a rollout of SimSteps (src/TrainHeter.h:194-231),
+ * i.e. LibTorch autograd through b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:
+ * 15-97) and EigenSolver::apply's backward (src/Solvers.h:35-52, src/Solvers.cpp:33-58).
+ * One backward solve per step whatever the number of parameters: YC_GRAD_REF solves with
+ * A as the reference's backward does, with A, M, K_v and the non-jamming shear angle held
+ * constant; YC_GRAD_EXACT solves with A^T and adds A's dependence (-lambda^T dA x).
+ * The recording stores the state every checkpoint_every steps; the backward pass
+ * recomputes the states in between (bit for bit: the recorded steps use the zero CG
+ * guess) and every step's system from its state.
+ * --------------------------------------------------------------------------------- */
+/* Starts a recording at the current state, for up to max_steps steps. */
+int yc_adjoint_begin(yc_ctx* ctx, int max_steps, int checkpoint_every);
+/* One recorded SimStep (as yc_step with x0 = NULL); the external force in effect is used. */
+int yc_adjoint_step(yc_ctx* ctx, double h, double tol, int maxit, yc_step_info* info);
+/* Device targets of the reference's position loss (src/TrainHeter.h:211-226): lpos
+ * max_steps x N x 3, epos max_steps x N x 2 (node-major, the state after step t in row
+ * t - 1), host arrays copied to the device; the loss of step t is
+ * weight (sum_k |x_k - lpos_k|^2 + sum_k |uv_k - epos_k|^2).  NULL, NULL clears them. */
+int yc_adjoint_set_targets(yc_ctx* ctx, const double* lpos, const double* epos, double weight);
+/* The backward pass over the recorded steps, with the same h (and the CG tol, maxit for
+ * the backward solves).  gstate: the loss gradient with respect to the state after each
+ * step, steps x 15N doubles in the yc_get_state layout (host), or NULL for the targets'
+ * loss (or none).  Outputs (any may be NULL): dparam[YC_NPARAM] the parameter gradient;
+ * dstate0[15N] the gradient with respect to the recorded initial state (yc_set_state
+ * layout); dext[steps x n] the gradient with respect to each step's external force;
+ * loss the targets' loss; cg_iters the backward solves' iterations.  The context ends at
+ * the state after the last recorded step. */
+int yc_adjoint_backward(yc_ctx* ctx, int mode, double h, double tol, int maxit, const double* gstate, double* dparam,
+                        double* dstate0, double* dext, double* loss, long long* cg_iters);
+/* Frees the recording. */
+int yc_adjoint_end(yc_ctx* ctx);
+
+/* ---------------------------------------------------------------------------------
  * Batches: many cloths of uneven size stepped together, each phase of a step one launch
  * over all of them (the persistent CG one cooperative launch per group of cloths that
  * fit on the device at once), replayed from a hipGraph where the runtime can capture
