@@ -191,6 +191,52 @@ def tangent_measure(ys, spec, state, steps, precond):
     return out
 
 
+def adjoint_measure(ys, spec, state, steps, precond):
+    """The reverse mode on the same workload: a rollout of `steps` recorded SimSteps and
+    its backward pass (one backward solve per step) giving the position loss's gradient
+    with respect to all twelve parameters, against targets from the rollout of the cloth
+    with its first yarn 10 % heavier (made beforehand, untimed)."""
+    tspec = dict(spec)
+    tspec["yarns"] = [dict(spec["yarns"][0], rho=spec["yarns"][0]["rho"] * 1.1)] + list(spec["yarns"][1:])
+    t = ys.Cloth(tspec)
+    t.set_cg_variant("persistent")
+    t.set_preconditioner(precond)
+    t.set_ClothState(state)
+    X, U = [], []
+    for _ in range(steps):
+        t.step(spec["h"], tol=1e-8)
+        st = t.get_ClothState()
+        X.append(st["LPos"])
+        U.append(st["EPos"])
+    t.close()
+    X, U = np.array(X), np.array(U)
+    out = {"workload": "the same cloth and state, %d recorded steps + the backward pass, gradient of the position "
+                       "loss with respect to the 12 parameters" % steps}
+    for mode in ("ref", "exact"):
+        c = ys.Cloth(spec)
+        c.set_cg_variant("persistent")
+        c.set_preconditioner(precond)
+        runs = []
+        for rep in range(2):   # the first run is untimed (allocation, kernel loading)
+            c.set_ClothState(state)
+            t0 = time.perf_counter()
+            c.adjoint_begin(steps)
+            for _ in range(steps):
+                c.adjoint_step(spec["h"], tol=1e-8)
+            c.adjoint_set_targets(X, U, 1.0 / c.N)
+            t1 = time.perf_counter()
+            r = c.adjoint_backward(spec["h"], mode, tol=1e-8)
+            t2 = time.perf_counter()
+            c.adjoint_end()
+            runs.append((t1 - t0, t2 - t1, r))
+        c.close()
+        fwd, bwd, r = runs[-1]
+        out[mode] = {"ms_per_gradient_step": 1e3 * (fwd + bwd) / steps, "forward_ms_per_step": 1e3 * fwd / steps,
+                     "backward_ms_per_step": 1e3 * bwd / steps, "backward_cg_iters_per_step": r["cg_iters"] / steps,
+                     "gradient_steps_per_s": steps / (fwd + bwd)}
+    return out
+
+
 def gradient_measure(ys, steps, precond):
     """Batched differentiable learning (64 cloths of uneven size, the batch workload):
     one loss-and-gradient evaluation of the eight material parameters per cloth, each a
@@ -201,17 +247,28 @@ def gradient_measure(ys, steps, precond):
     import yarnstep.learn as ln
     items = ys.cloth_spec.batch_workload()
     frames = ln.batch_trajectories(items, steps)
-    t0 = time.perf_counter()
-    ok = 0
-    for i, ((spec, _), f) in enumerate(zip(items, frames)):
-        sp = ln.perturbed_spec(spec, ln.NAMES, 500 + i)
-        _, g, ok_ = ln.sample_loss_terms(sp, f, ln.NAMES, mode="ref", tol=1e-8, precond=precond)
-        ok += ok_
-    dt = time.perf_counter() - t0
-    return {"workload": "64 cloths, sides 10..64 (seed 3), %d-step rollouts, gradients of %s" % (steps, ", ".join(ln.NAMES)),
-            "form": "forward-mode tangents, 8 directions + the primal solve per step, one cloth at a time",
-            "cloth_steps": ok, "cloth_gradient_steps_per_s": ok / dt, "ms_per_cloth_step": 1e3 * dt / max(ok, 1),
-            "s_per_epoch": dt}
+    out = {"workload": "64 cloths, sides 10..64 (seed 3), %d-step rollouts, gradients of %s" % (steps, ", ".join(ln.NAMES))}
+    grads = {}
+    for form, fn in (("tangents", ln.sample_loss_terms), ("adjoint", ln.sample_loss_terms_adjoint)):
+        t0 = time.perf_counter()
+        ok = 0
+        gs = []
+        for i, ((spec, _), f) in enumerate(zip(items, frames)):
+            sp = ln.perturbed_spec(spec, ln.NAMES, 500 + i)
+            _, g, ok_ = fn(sp, f, ln.NAMES, mode="ref", tol=1e-8, precond=precond)
+            ok += ok_
+            gs.append(g)
+        dt = time.perf_counter() - t0
+        grads[form] = np.array(gs)
+        out[form] = {"form": ("forward-mode tangents, 8 directions + the primal solve per step" if form == "tangents" else
+                              "reverse mode: recorded rollout, one backward solve per step, all parameters at once") +
+                             ", one cloth at a time",
+                     "cloth_steps": ok, "cloth_gradient_steps_per_s": ok / dt, "ms_per_cloth_step": 1e3 * dt / max(ok, 1),
+                     "s_per_epoch": dt}
+    a, t = grads["adjoint"], grads["tangents"]
+    out["adjoint_over_tangents"] = out["adjoint"]["cloth_gradient_steps_per_s"] / out["tangents"]["cloth_gradient_steps_per_s"]
+    out["max_rel_gradient_difference"] = float(np.max(np.abs(a - t)) / max(np.max(np.abs(t)), 1e-300))
+    return out
 
 
 def hbm_stream_measure(launches=20):
@@ -274,6 +331,9 @@ def main():
                     help="rollout steps of the 64-cloth parameter-gradient line (0: skip it)")
     ap.add_argument("--tangent-only", action="store_true",
                     help="time only the differentiation line and print it (profiling runs)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="time only the reverse-mode line on the workload and the 64-cloth gradient line, and print "
+                         "them (profiling runs)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run 5 launches of a 256 MiB -> 256 MiB FP64 negation, the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counter runs")
@@ -298,6 +358,11 @@ def main():
     tol = 1e-8
     if args.tangent_only:
         print(json.dumps({"tangents": tangent_measure(ys, spec, state, args.tangent_steps, args.precond)}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_measure(ys, spec, state, args.tangent_steps, args.precond)}), flush=True)
+        if args.gradient_steps > 0:
+            print(json.dumps({"batch_gradient": gradient_measure(ys, args.gradient_steps, args.precond)}), flush=True)
         return
     if args.plain_only:
         net = None
@@ -454,6 +519,7 @@ def main():
         out["config"]["homo256"] = large_measure(ys, args.large_steps)
     if args.tangent_steps > 0 and not args.plain_only and args.cg == "persistent":
         out["config"]["tangents"] = tangent_measure(ys, spec, state, args.tangent_steps, args.precond)
+        out["config"]["adjoint"] = adjoint_measure(ys, spec, state, args.tangent_steps, args.precond)
     if args.gradient_steps > 0 and not args.plain_only and args.cg == "persistent":
         out["config"]["batch_gradient"] = gradient_measure(ys, args.gradient_steps, args.precond)
     if not args.no_cpu:

@@ -334,15 +334,29 @@ class Cloth:
         self._adj_steps = 0
 
     def adjoint_step(self, h, tol=1e-10, maxit=None):
-        """A recorded SimStep (zero CG guess); returns (iters, relres, status)."""
+        """A recorded SimStep (zero CG guess); returns (iters, relres, status).  A step whose
+        solve does not converge raises (NotConverged, NonFinite) and is neither taken nor
+        recorded."""
         if maxit is None:
             maxit = 2 * self.n
         info = _abi.YcStepInfo()
         rc = _abi.load_library().yc_adjoint_step(self._h, float(h), float(tol), int(maxit), ctypes.byref(info))
-        if rc not in (_abi.YC_OK, _abi.YC_ERR_NOT_CONVERGED):
-            _abi.check(rc)
+        _abi.check(rc)
         self._adj_steps += 1
         return info.cg_iters, info.cg_relres, rc
+
+    def adjoint_run(self, h, steps, tol=1e-10, maxit=None):
+        """steps recorded SimSteps queued without host round trips; returns the CG
+        iterations.  A failure raises and voids the recording."""
+        if maxit is None:
+            maxit = 2 * self.n
+        tot = ctypes.c_longlong()
+        rc = _abi.load_library().yc_adjoint_run(self._h, float(h), int(steps), float(tol), int(maxit), ctypes.byref(tot))
+        if rc != _abi.YC_OK:
+            self._adj_steps = 0
+        _abi.check(rc)
+        self._adj_steps += int(steps)
+        return tot.value
 
     def adjoint_set_targets(self, lpos, epos, weight):
         """Targets of the reference's position loss, (max_steps, N, 3) and (max_steps, N, 2):

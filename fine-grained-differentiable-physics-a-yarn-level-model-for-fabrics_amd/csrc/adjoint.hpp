@@ -140,7 +140,9 @@ struct AdjSink {
         if (row(kF)) {
             if (!REF) acc -= coef[kind] * lam(kF, 3 + c) * v.d * xs(kD, 3 + c);
             if (kind == KIND_M) acc += lam(kF, 3 + c) * bprod(v, velE(kD, c));
-            if (kD == kF && !fric) {
+            // (REF: the friction reads these entries only into A's E-E entries and, for K_v,
+            // into K_v^T v as a constant block, so their adjoints are zero)
+            if (!REF && kD == kF && !fric) {
                 if (kind == KIND_KX) acc += yb(kF, FI_KXEE + c) * v.d;
                 if (kind == KIND_KV) acc += yb(kF, FI_KVEE + c) * v.d;
             }

@@ -3494,13 +3494,14 @@ int ckpt_io(yc_ctx* c, double* blk, int save) {
     return YC_OK;
 }
 
-// One recorded step, as yc_step with the zero CG guess (a recomputed step repeats it bit for bit).
+// One recorded step, as yc_step with the zero CG guess (a recomputed step repeats it bit
+// for bit); a step whose solve fails or stops above tolerance is not taken, as in
+// yc_step_tangent.
 int adj_forward_step(yc_ctx* c, double h, double tol, int maxit, int* iters, double* relres) {
     int rc = do_assemble(c, h);
     if (!rc) rc = do_solve(c, nullptr, tol, maxit, iters, relres);
-    if (rc == YC_ERR_NONFINITE || rc == YC_ERR_HIP || rc == YC_ERR_ARG) return rc;
-    int rc2 = do_update(c, h);
-    return rc2 ? rc2 : rc;
+    if (rc) return rc;
+    return do_update(c, h);
 }
 
 template <bool REF>
@@ -3587,6 +3588,18 @@ int adjoint_backward(yc_ctx* c, double h, double tol, int maxit, const double* g
     long long iters_total = 0;
     int rc = ckpt_io(c, c->adj_final, 1);   // the state after the last step, restored at the end
     if (rc) return rc;
+    // with the persistent CG the whole pass is queued without a host round trip: the
+    // assemblies' flags and the solves' outcomes accumulate on the device and are read once
+    const bool queued = c->coop_ok != 0;
+    if (queued) {
+        HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+        HIPCHK(hipMemsetAsync((char*)c->sc + offsetof(CGScal, total_iters), 0, sizeof(long long) + 2 * sizeof(int), c->stream));
+    }
+    double* lamlog = nullptr;   // every step's lambda, for dext (device, converted at the end)
+    if (dext) {
+        rc = dalloc(&lamlog, (size_t)T * PV);
+        if (rc) return rc;
+    }
     HIPCHK(hipMemsetAsync(c->adj_g1, 0, B * sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(c->adj_gpar, 0, (size_t)YC_NPARAM * N * sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(c->adj_lpart, 0, N * sizeof(double), c->stream));
@@ -3620,7 +3633,6 @@ int adjoint_backward(yc_ctx* c, double h, double tol, int maxit, const double* g
                     c->has_ext = false;
                 }
                 rc = adj_forward_step(c, h, tol, maxit, nullptr, nullptr);
-                if (rc == YC_ERR_NOT_CONVERGED) rc = YC_OK;   // recorded as it went forward
                 if (!rc) rc = ckpt_io(c, c->adj_seg + (size_t)(n - s0) * B, 1);
             }
         }
@@ -3632,7 +3644,7 @@ int adjoint_backward(yc_ctx* c, double h, double tol, int maxit, const double* g
                                (const double*)(c->adj_g0 + G_NRM * N), c->adj_pbar);
             // 2. recompute the step's system and friction inputs
             rc = ckpt_io(c, sn, 0);
-            if (!rc) rc = do_assemble(c, h);
+            if (!rc) rc = do_assemble(c, h, queued);
             if (rc) break;
             hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)(sn1 + G_V * N),
                                (const double*)(sn1 + G_UVD * N), c->W, c->Lg, c->N, c->Wp, c->P, c->adj_xs);
@@ -3650,17 +3662,18 @@ int adjoint_backward(yc_ctx* c, double h, double tol, int maxit, const double* g
             HIPCHK(hipMemcpyAsync(c->cx, c->adj_lam, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             hipLaunchKernelGGL(k_spmv, dim3(c->part.nS), dim3(BLK), 0, c->stream, make_spmv(c, c->cx, c->cy, Au), (double*)nullptr,
                                (const CGScal*)c->sc, 0);
-            int it = 0;
-            rc = solve_on(c, Au, tol, maxit, &it, true);
-            iters_total += it;
+            if (queued) {
+                hipLaunchKernelGGL(k_cg_init, dim3(c->part.nV), dim3(BLK), 0, c->stream, make_vec(c), c->part, 1);
+                rc = launch_cgp(c, tol, maxit, Au);
+            } else {
+                int it = 0;
+                rc = solve_on(c, Au, tol, maxit, &it, true);
+                iters_total += it;
+            }
             if (rc) break;
             HIPCHK(hipMemcpyAsync(c->adj_lam, c->cx, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-            if (dext) {   // the adjoint of the step's external force: h lambda
-                HIPCHK(hipMemcpyAsync(hl.data(), c->adj_lam, PV * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(hipStreamSynchronize(c->stream));
-                from_padded(c, hl, glob.data());
-                for (int i = 0; i < c->n; ++i) dext[(size_t)n * c->n + i] = h * glob[i];
-            }
+            if (lamlog)   // the adjoint of the step's external force is h lambda
+                HIPCHK(hipMemcpyAsync(lamlog + (size_t)n * PV, c->cx, PV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             // 5. the assembly's vector-Jacobian product, into g1 and the parameter partials
             rc = adj_launch<REF>(c, Cadj, h);
             if (rc) break;
@@ -3671,6 +3684,35 @@ int adjoint_backward(yc_ctx* c, double h, double tol, int maxit, const double* g
         }
     }
     c->fric_out = nullptr;
+    if (!rc && queued) {   // the outcomes of the queued assemblies and solves
+        HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_flag + 2, c->cg_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        iters_total = c->h_sc->total_iters;
+        if (c->h_flag[2]) {
+            (void)hipMemsetAsync(c->cg_err, 0, sizeof(int), c->stream);
+            rc = fail(YC_ERR_HIP, "persistent CG: a block waited past its spin limit");
+        } else if (c->h_flag[0]) {
+            rc = fail(YC_ERR_HIP, "assembly: stencil slot out of range");
+        } else if (c->h_flag[1] || c->h_sc->nonfinite) {
+            rc = fail(YC_ERR_NONFINITE, "adjoint: NaN/Inf in a recomputed system or a backward solve");
+        } else if (c->h_sc->nonconv) {
+            rc = fail(YC_ERR_NOT_CONVERGED, "adjoint: a backward solve stopped at maxit above tolerance");
+        }
+    }
+    if (!rc && dext) {
+        for (int n = 0; n < T && !rc; ++n) {
+            if (hipMemcpyAsync(hl.data(), lamlog + (size_t)n * PV, PV * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = fail(YC_ERR_HIP, "dext download");
+                break;
+            }
+            from_padded(c, hl, glob.data());
+            for (int i = 0; i < c->n; ++i) dext[(size_t)n * c->n + i] = h * glob[i];
+        }
+    }
+    if (lamlog) { (void)hipStreamSynchronize(c->stream); (void)hipFree(lamlog); }
     if (!rc) {   // the initial state's normals come from its x (yc_set_state)
         hipLaunchKernelGGL(k_adj_nrm, dim3(nblk_n), dim3(NBLK), 0, c->stream, block_state(c, c->adj_ckpt),
                            (const double*)(c->adj_g0 + G_NRM * N), c->adj_pbar);
@@ -3771,7 +3813,7 @@ int yc_adjoint_step(yc_ctx* c, double h, double tol, int maxit, yc_step_info* in
     double rr = 0;
     int rc = adj_forward_step(c, h, tol, maxit, &it, &rr);
     if (info) { info->cg_iters = it; info->cg_relres = rr; info->status = rc; }
-    if (rc && rc != YC_ERR_NOT_CONVERGED) return rc;
+    if (rc) return rc;   // not taken, not recorded
     c->adj_steps = t + 1;
     if (c->adj_steps % c->adj_K == 0 && c->adj_steps < c->adj_max) {
         int r = ckpt_io(c, c->adj_ckpt + (size_t)(c->adj_steps / c->adj_K) * adj_block(c), 1);
@@ -3779,6 +3821,66 @@ int yc_adjoint_step(yc_ctx* c, double h, double tol, int maxit, yc_step_info* in
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return rc;
+}
+
+int yc_adjoint_run(yc_ctx* c, double h, int steps, double tol, int maxit, long long* total_iters) {
+    if (!c) return fail(YC_ERR_ARG, "null context");
+    if (c->adj_max <= 0) return fail(YC_ERR_ARG, "yc_adjoint_run before yc_adjoint_begin");
+    if (steps < 0 || c->adj_steps + steps > c->adj_max) return fail(YC_ERR_ARG, "more steps than yc_adjoint_begin allowed");
+    if (!(c->cg_variant == YC_CG_PERSISTENT && c->coop_ok)) {   // step by step, each checked
+        long long tot = 0;
+        for (int s = 0; s < steps; ++s) {
+            yc_step_info info;
+            int rc = yc_adjoint_step(c, h, tol, maxit, &info);
+            tot += info.cg_iters;
+            if (rc) { if (total_iters) *total_iters = tot; return rc; }
+        }
+        if (total_iters) *total_iters = tot;
+        return YC_OK;
+    }
+    g_err.clear();
+    if (c->adj_K > 1) {
+        if (c->has_ext && !c->adj_ext) {
+            int r = dalloc(&c->adj_ext, (size_t)(c->adj_max + 1) * c->n);
+            if (r) return r;
+        }
+        c->adj_has_ext.resize(c->adj_max, 0);
+    }
+    HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync((char*)c->sc + offsetof(CGScal, total_iters), 0, sizeof(long long) + 2 * sizeof(int), c->stream));
+    for (int s = 0; s < steps; ++s) {
+        const int t = c->adj_steps + s;
+        if (c->adj_K > 1) {
+            c->adj_has_ext[t] = c->has_ext ? 1 : 0;
+            if (c->has_ext)
+                HIPCHK(hipMemcpyAsync(c->adj_ext + (size_t)t * c->n, c->ext, (size_t)c->n * sizeof(double), hipMemcpyDeviceToDevice,
+                                      c->stream));
+        }
+        int rc = do_assemble(c, h, true);
+        if (!rc) rc = do_solve(c, nullptr, tol, maxit, nullptr, nullptr, false, true);
+        if (!rc) rc = do_update(c, h);
+        if (!rc && (t + 1) % c->adj_K == 0 && t + 1 < c->adj_max) rc = ckpt_io(c, c->adj_ckpt + (size_t)((t + 1) / c->adj_K) * adj_block(c), 1);
+        if (rc) { c->adj_steps = 0; return rc; }
+    }
+    HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->h_flag + 2, c->cg_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (total_iters) *total_iters = c->h_sc->total_iters;
+    int rc = YC_OK;
+    if (c->h_flag[2]) {
+        (void)hipMemsetAsync(c->cg_err, 0, sizeof(int), c->stream);
+        rc = fail(YC_ERR_HIP, "persistent CG: a block waited past its spin limit");
+    } else if (c->h_flag[0]) {
+        rc = fail(YC_ERR_HIP, "assembly: stencil slot out of range");
+    } else if (c->h_flag[1] || c->h_sc->nonfinite) {
+        rc = fail(YC_ERR_NONFINITE, "NaN/Inf in A, b or a solution");
+    } else if (c->h_sc->nonconv) {
+        rc = fail(YC_ERR_NOT_CONVERGED, "a CG solve stopped at maxit above tolerance");
+    }
+    if (rc) { c->adj_steps = 0; return rc; }   // the recording is void: steps after a failed one ran on its result
+    c->adj_steps += steps;
+    return YC_OK;
 }
 
 int yc_adjoint_set_targets(yc_ctx* c, const double* lpos, const double* epos, double weight) {

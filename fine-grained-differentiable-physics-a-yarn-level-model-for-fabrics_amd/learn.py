@@ -9,10 +9,11 @@ over nodes of |LPos - LPos_gt|^2 and of |EPos - EPos_gt|^2, and the batch loss d
 the sum by (successful steps - 1) (src/TrainHeter.h:196-226).  A step with a NaN/Inf
 gradient is skipped (src/TrainHeter.h:246-256).
 
-Here the rollouts run on the GPU and the parameter gradient comes from the step's
-forward-mode tangents (``Cloth.step_tangent``): one direction per learned parameter,
-propagated with the rollout, so the gradient is ready when the rollout ends and no
-step is stored or recomputed.  ``mode="ref"`` gives the reference's gradient (A, M,
+Here the rollouts run on the GPU.  The parameter gradient comes from the reverse mode
+by default (``Cloth.adjoint_*``: the rollout is recorded, and one backward pass with one
+backward solve per step gives every parameter's derivative), or from the step's
+forward-mode tangents (``Cloth.step_tangent``: one direction per learned parameter,
+propagated with the rollout, nothing stored).  ``mode="ref"`` gives the reference's gradient (A, M,
 K_v and the non-jamming shear angle held constant, backward solve with A),
 ``mode="exact"`` the derivative of the step as computed.
 """
@@ -113,14 +114,63 @@ def sample_loss_terms(spec, frames, names, mode="ref", tol=1e-10, cg="persistent
     return loss, grad, ok
 
 
+def sample_loss_terms_adjoint(spec, frames, names, mode="ref", tol=1e-10, cg="persistent", precond="block2",
+                              checkpoint_every=1):
+    """``sample_loss_terms`` by the reverse mode: the rollout is recorded (a state every
+    checkpoint_every steps), the position loss's targets sit on the device, and one
+    backward pass with one backward solve per step returns the loss and the derivatives of
+    all parameters at once, whatever their number."""
+    c = Cloth(spec)
+    nsteps = frames["LPos"].shape[0] - 1
+    try:
+        configure_solver(c, cg, precond)
+        c.set_ClothState({k: frames[k][0] for k in frames})
+        c.adjoint_begin(nsteps, checkpoint_every)
+        ok = 0
+        try:   # every step queued at once; on a failure, again step by step to find the last good one
+            c.adjoint_run(spec["h"], nsteps, tol=tol)
+            s = c.get_ClothState()
+            if np.isfinite(s["LPos"]).all() and np.isfinite(s["EPos"]).all():
+                ok = nsteps
+        except (_abi.NonFinite, _abi.NotConverged):
+            pass
+        if ok < nsteps:
+            c.set_ClothState({k: frames[k][0] for k in frames})
+            c.adjoint_begin(nsteps, checkpoint_every)
+        for _ in range(nsteps - ok):
+            try:
+                c.adjoint_step(spec["h"], tol=tol)
+            except (_abi.NonFinite, _abi.NotConverged):
+                break
+            s = c.get_ClothState()
+            if not (np.isfinite(s["LPos"]).all() and np.isfinite(s["EPos"]).all()):
+                break
+            ok += 1
+        if ok == 0:
+            c.adjoint_end()
+            return 0.0, np.zeros(len(names)), 0
+        c.adjoint_set_targets(frames["LPos"][1:], frames["EPos"][1:], 1.0 / c.N)
+        res = c.adjoint_backward(spec["h"], mode, tol=tol)
+        c.adjoint_end()
+    finally:
+        c.close()
+    grad = np.array([res["param"][Cloth.PARAMS.index(n)] for n in names])
+    return res["loss"], grad, ok
+
+
 class ParameterLearner:
     """SGD on the boxed parameters of one cloth, as the reference's training loop.
 
     spec: the cloth (its parameter values are the starting point); names: the learned
     parameters (default: all eight of the reference); lrs / ranges: per-parameter
-    learning rates and (upper, lower) boxes, the reference's by default."""
+    learning rates and (upper, lower) boxes, the reference's by default; form: "adjoint"
+    (reverse mode, one backward solve per step for all parameters) or "tangent" (one
+    forward tangent per learned parameter)."""
 
-    def __init__(self, spec, names=NAMES, lrs=None, ranges=None, mode="ref", tol=1e-10):
+    def __init__(self, spec, names=NAMES, lrs=None, ranges=None, mode="ref", tol=1e-10, form="adjoint"):
+        if form not in ("adjoint", "tangent"):
+            raise ValueError("form must be 'adjoint' or 'tangent'")
+        self.form = form
         if len(spec["yarns"]) == 1:
             names = tuple(n for n in names if n not in ("rho1", "Y1", "B1"))
         self.spec = spec
@@ -149,7 +199,8 @@ class ParameterLearner:
         spec = self.current_spec()
         tot, g, ok = 0.0, np.zeros(len(self.names)), 0
         for frames in samples:
-            l_, g_, ok_ = sample_loss_terms(spec, frames, self.names, self.mode, self.tol)
+            fn = sample_loss_terms_adjoint if self.form == "adjoint" else sample_loss_terms
+            l_, g_, ok_ = fn(spec, frames, self.names, self.mode, self.tol)
             tot += l_
             g += g_
             ok += ok_

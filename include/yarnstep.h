@@ -362,8 +362,14 @@ int yc_tangent_end(yc_ctx* ctx);
  * --------------------------------------------------------------------------------- */
 /* Starts a recording at the current state, for up to max_steps steps. */
 int yc_adjoint_begin(yc_ctx* ctx, int max_steps, int checkpoint_every);
-/* One recorded SimStep (as yc_step with x0 = NULL); the external force in effect is used. */
+/* One recorded SimStep (as yc_step with x0 = NULL); the external force in effect is used.
+ * A step whose solve stops above tolerance (YC_ERR_NOT_CONVERGED) or fails is neither
+ * taken nor recorded, as in yc_step_tangent. */
 int yc_adjoint_step(yc_ctx* ctx, double h, double tol, int maxit, yc_step_info* info);
+/* steps recorded SimSteps queued without a host round trip (persistent CG; other forms
+ * step one by one).  When any of them fails the recording is void (it holds no step
+ * afterwards; start again with yc_adjoint_begin) and the status says why. */
+int yc_adjoint_run(yc_ctx* ctx, double h, int steps, double tol, int maxit, long long* total_iters);
 /* Device targets of the reference's position loss (src/TrainHeter.h:211-226): lpos
  * max_steps x N x 3, epos max_steps x N x 2 (node-major, the state after step t in row
  * t - 1), host arrays copied to the device; the loss of step t is

@@ -51,9 +51,10 @@ def oracle_loss(spec, frames):
     return tot / max(ok - 1, 1)
 
 
-def test_exact_loss_gradient_matches_oracle_fd(ys, cs):
+@pytest.mark.parametrize("form", ["tangent", "adjoint"])
+def test_exact_loss_gradient_matches_oracle_fd(ys, cs, form):
     truth, start, frames = _problem(ys, cs)
-    lrn = ys.learn.ParameterLearner(start, mode="exact", tol=1e-13)
+    lrn = ys.learn.ParameterLearner(start, mode="exact", tol=1e-13, form=form)
     loss, g = lrn.loss_and_grad([frames])
     say("  loss %.6e (oracle %.6e)" % (loss, oracle_loss(lrn.current_spec(), frames)))
     assert loss == pytest.approx(oracle_loss(lrn.current_spec(), frames), rel=1e-6)
@@ -69,7 +70,7 @@ def test_exact_loss_gradient_matches_oracle_fd(ys, cs):
         fd1 = (vals[0] - vals[1]) / (2 * eps)
         fd2 = (vals[2] - vals[3]) / (4 * eps)
         fd = (4 * fd1 - fd2) / 3   # Richardson
-        say("  %-4s grad %+.6e fd %+.6e" % (n, g[i], fd))
+        say("  %s %-4s grad %+.6e fd %+.6e" % (form, n, g[i], fd))
         assert abs(g[i] - fd) <= 1e-4 * abs(fd) + 1e-6 * gmax
 
 
@@ -77,6 +78,13 @@ def test_exact_loss_gradient_matches_oracle_fd(ys, cs):
 def test_gradient_step_lowers_the_loss(ys, cs, mode):
     truth, start, frames = _problem(ys, cs)
     lrn = ys.learn.ParameterLearner(start, mode=mode, tol=1e-13)
+    tg = ys.learn.ParameterLearner(start, mode=mode, tol=1e-13, form="tangent")
+    # the reverse mode's gradient is the tangents' one
+    la, ga = lrn.loss_and_grad([frames])
+    lt, gt = tg.loss_and_grad([frames])
+    say("  %s adjoint vs tangent: loss %.3e / %.3e, |dg| / |g| %.2e" % (mode, la, lt, np.linalg.norm(ga - gt) / np.linalg.norm(gt)))
+    assert la == pytest.approx(lt, rel=1e-12)
+    assert np.linalg.norm(ga - gt) <= 1e-8 * np.linalg.norm(gt)
     l0, g = lrn.loss_and_grad([frames])
     lr = 0.2 * l0 / float(g @ g)
     lrn.lrs = {n: lr for n in lrn.names}
