@@ -265,6 +265,22 @@ def gradient_measure(ys, steps, precond):
                              ", one cloth at a time",
                      "cloth_steps": ok, "cloth_gradient_steps_per_s": ok / dt, "ms_per_cloth_step": 1e3 * dt / max(ok, 1),
                      "s_per_epoch": dt}
+    # the reverse mode of all 64 cloths at once: recorded together, backward passes together
+    specs = [ln.perturbed_spec(spec, ln.NAMES, 500 + i) for i, (spec, _) in enumerate(items)]
+    ln.batch_loss_terms_adjoint(specs[:2], frames[:2], ln.NAMES, mode="ref", tol=1e-8)   # untimed: kernels, allocations
+    t0 = time.perf_counter()
+    res = ln.batch_loss_terms_adjoint(specs, frames, ln.NAMES, mode="ref", tol=1e-8)
+    dt = time.perf_counter() - t0
+    ok = sum(r[2] for r in res)
+    grads["batched_adjoint"] = np.array([r[1] for r in res])
+    out["batched_adjoint"] = {"form": "reverse mode of all 64 cloths together: one launch per phase over all cloths, the "
+                                      "solves (forward and backward) one cooperative launch, Jacobi preconditioner",
+                              "cloth_steps": ok, "cloth_gradient_steps_per_s": ok / dt, "ms_per_cloth_step": 1e3 * dt / max(ok, 1),
+                              "s_per_epoch": dt}
+    out["batched_adjoint_over_tangents"] = out["batched_adjoint"]["cloth_gradient_steps_per_s"] / \
+        out["tangents"]["cloth_gradient_steps_per_s"]
+    out["batched_max_rel_gradient_difference"] = float(np.max(np.abs(grads["batched_adjoint"] - grads["tangents"])) /
+                                                       max(np.max(np.abs(grads["tangents"])), 1e-300))
     a, t = grads["adjoint"], grads["tangents"]
     out["adjoint_over_tangents"] = out["adjoint"]["cloth_gradient_steps_per_s"] / out["tangents"]["cloth_gradient_steps_per_s"]
     out["max_rel_gradient_difference"] = float(np.max(np.abs(a - t)) / max(np.max(np.abs(t)), 1e-300))

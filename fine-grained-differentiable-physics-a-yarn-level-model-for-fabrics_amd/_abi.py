@@ -59,7 +59,8 @@ EXPORTS = ["yc_create", "yc_destroy", "yc_last_error", "yc_sizes", "yc_set_state
            "yc_batch_info", "yc_time_assemble", "yc_tangent_set_ext", "yc_set_preconditioner",
            "yc_set_state_device", "yc_get_state_device", "yc_export_precond", "yc_export_matrix",
            "yc_load_system", "yc_update_vel", "yc_run_guess", "yc_mlp_forward_noalloc", "yc_warm_guess", "yc_time_warm_guess",
-           "yc_adjoint_begin", "yc_adjoint_step", "yc_adjoint_run", "yc_adjoint_set_targets", "yc_adjoint_backward", "yc_adjoint_end"]
+           "yc_adjoint_begin", "yc_adjoint_step", "yc_adjoint_run", "yc_adjoint_set_targets", "yc_adjoint_backward", "yc_adjoint_end",
+           "yc_batch_adjoint_run", "yc_batch_adjoint_backward"]
 
 
 def lib_path():
@@ -131,6 +132,10 @@ def load_library():
     L.yc_adjoint_backward.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp, dp,
                                       ctypes.POINTER(ctypes.c_longlong)]
     L.yc_adjoint_end.argtypes = [vp]
+    L.yc_batch_adjoint_run.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int)]
+    L.yc_batch_adjoint_backward.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp,
+                                            ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int)]
     L.yc_time_solve.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, ip]
     L.yc_time_assemble.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp]
     L.yc_cg_iteration_bytes.argtypes = [vp]

@@ -89,6 +89,37 @@ class ClothBatch:
         return np.array(it[:], dtype=np.int64), np.array(st[:], dtype=np.int32), rc
 
 
+    def adjoint_run(self, h, steps, tol=1e-8, maxit=None):
+        """`steps` recorded SimSteps of every cloth (each cloth's adjoint_begin first,
+        checkpoint interval 1).  Returns (iters, status); raises on failure, which voids
+        every member's recording."""
+        n = len(self.cloths)
+        it = (ctypes.c_longlong * n)()
+        st = (ctypes.c_int * n)()
+        rc = _abi.load_library().yc_batch_adjoint_run(self._h, float(h), int(steps), float(tol),
+                                                      -1 if maxit is None else int(maxit), it, st)
+        for c in self.cloths:
+            c._adj_steps = 0 if rc != _abi.YC_OK else c._adj_steps + int(steps)
+        _abi.check(rc)
+        return np.array(it[:], dtype=np.int64), np.array(st[:], dtype=np.int32)
+
+    def adjoint_backward(self, h, mode="ref", tol=1e-8, maxit=None):
+        """The backward passes of every cloth's recording against its targets
+        (Cloth.adjoint_set_targets).  Returns (dparam (n, 12) in Cloth.PARAMS order,
+        loss (n,), backward CG iterations (n,))."""
+        from .cloth import Cloth
+        n = len(self.cloths)
+        dparam = np.zeros((n, len(Cloth.PARAMS)))
+        loss = np.zeros(n)
+        it = (ctypes.c_longlong * n)()
+        st = (ctypes.c_int * n)()
+        dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+        rc = _abi.load_library().yc_batch_adjoint_backward(self._h, Cloth.GRAD_MODES[mode], float(h), float(tol),
+                                                           -1 if maxit is None else int(maxit), dp(dparam), dp(loss), it, st)
+        _abi.check(rc)
+        return dparam, loss, np.array(it[:], dtype=np.int64)
+
+
 def batch_signature(cloths):
     return "batch:" + ",".join("%dx%d" % (c.width, c.length) for c in cloths)
 

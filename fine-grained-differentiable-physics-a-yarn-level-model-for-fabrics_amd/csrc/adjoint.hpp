@@ -165,8 +165,9 @@ __device__ __forceinline__ void add_g(const AdjOut& O, int N, int plane, int k, 
     O.g[(size_t)plane * N + k] += x;
 }
 
-// The node of thread t of the node-gather kernels: one 64-thread block per 8 x 8 tile.
-__device__ __forceinline__ int adj_node(const ClothDev& C) { return tile_node(C, blockIdx.x, threadIdx.x); }
+// The node-gather bodies below take the tile of their block: one 64-thread block per 8 x 8
+// tile (tile_node), as the single-cloth kernels at the end of this file launch them, or one
+// block per tile of every cloth of a batch (batch_adjoint.hpp).
 
 // ---------------------------------------------------------------------------
 // Elements other than the shear (handle, edges, bends with their crimp twins, faces):
@@ -178,8 +179,8 @@ __device__ __forceinline__ int adj_node(const ClothDev& C) { return tile_node(C,
 constexpr int ADJ_STATE_PASSES = 14;
 
 template <bool REF>
-__global__ void __launch_bounds__(TILE_N) k_adj_elems(ClothDev C, AdjW Wt, AdjOut O) {
-    const int k = adj_node(C);
+__device__ __forceinline__ void adj_elems_tile(const ClothDev& C, const AdjW& Wt, const AdjOut& O, int tile) {
+    const int k = tile_node(C, tile, threadIdx.x);
     if (k < 0) return;
     const int u = k % C.W, v = k / C.W;
     const bool wind = C.wind_density != 0.0 || C.wind_drag != 0.0;
@@ -261,8 +262,8 @@ __global__ void __launch_bounds__(TILE_N) k_adj_elems(ClothDev C, AdjW Wt, AdjOu
 // of one pass (rho, kb and the handle stiffness; or ks; or kc), and each element's
 // tangent goes to its own parameter, yarn by yarn.  Own rows only.
 template <bool REF>
-__global__ void __launch_bounds__(TILE_N) k_adj_elems_par(ClothDev C, AdjW Wt, AdjOut O, int pass) {
-    const int k = adj_node(C);
+__device__ __forceinline__ void adj_elems_par_tile(const ClothDev& C, const AdjW& Wt, const AdjOut& O, int pass, int tile) {
+    const int k = tile_node(C, tile, threadIdx.x);
     if (k < 0) return;
     const int u = k % C.W, v = k / C.W;
     const int ne_warp = C.Lg * (C.W - 1), nb_warp = C.Lg * (C.W - 2);
@@ -344,8 +345,8 @@ __device__ __forceinline__ void shear_adj_segment(const ClothDev& C, Sink& S, co
 }
 
 template <bool REF>
-__global__ void __launch_bounds__(TILE_N) k_adj_shear(ClothDev C, AdjW Wt, AdjOut O, const double* comp) {
-    const int k = adj_node(C);
+__device__ __forceinline__ void adj_shear_tile(const ClothDev& C, const AdjW& Wt, const AdjOut& O, const double* comp, int tile) {
+    const int k = tile_node(C, tile, threadIdx.x);
     if (k < 0) return;
     AdjSink<REF> S;
     S.init(&C, Wt, k, false);
@@ -372,8 +373,8 @@ __global__ void __launch_bounds__(TILE_N) k_adj_shear(ClothDev C, AdjW Wt, AdjOu
 // that differ from its host build and from -O1 on the same inputs, while the other
 // exact-mode passes matched; this form needs a fraction of the registers.)
 template <bool REF>
-__global__ void __launch_bounds__(TILE_N) k_adj_shear_par(ClothDev C, AdjW Wt, AdjOut O, const double* comp) {
-    const int k = adj_node(C);
+__device__ __forceinline__ void adj_shear_par_tile(const ClothDev& C, const AdjW& Wt, const AdjOut& O, const double* comp, int tile) {
+    const int k = tile_node(C, tile, threadIdx.x);
     if (k < 0) return;
     const int W = C.W;
     auto lam = [&](int q, int c) { return Wt.lam[(size_t)c * Wt.P + padidx(Wt.Wp, q % W, q / W)]; };
@@ -429,9 +430,9 @@ __device__ __forceinline__ double fric_adj_pass(const ClothDev& C, AdjSink<REF>&
 }
 
 template <bool REF>
-__global__ void __launch_bounds__(TILE_N) k_adj_fric(ClothDev C, AdjW Wt, AdjOut O, const double* fric_in, const double* gslide,
-                                                     int fpar) {
-    const int k = adj_node(C);
+__device__ __forceinline__ void adj_fric_tile(const ClothDev& C, const AdjW& Wt, const AdjOut& O, const double* fric_in,
+                                              const double* gslide, int fpar, int tile) {
+    const int k = tile_node(C, tile, threadIdx.x);
     if (k < 0) return;
     const int N = C.N;
     if (!interior(C, k % C.W, k / C.W)) {
@@ -475,8 +476,7 @@ __global__ void __launch_bounds__(TILE_N) k_adj_fric(ClothDev C, AdjW Wt, AdjOut
 // d-bar_i = (Cbar + Cbar^T) d_i, and p-bar_i = d-bar_i - mean(d-bar).  Written per cross
 // point: pbar[(i * 3 + c) N + k], i over (k, k-1, k+1, k-W, k+W).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NBLK) k_adj_nrm(StateDev s, const double* gnrm, double* pbar) {
-    const int k = blockIdx.x * NBLK + threadIdx.x;
+__device__ __forceinline__ void adj_nrm_node(const StateDev& s, const double* gnrm, double* pbar, int k) {
     if (k >= s.N) return;
     const int N = s.N, u = k % s.W, v = k / s.W;
     if (!(u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1)) {
@@ -524,11 +524,10 @@ __global__ void __launch_bounds__(NBLK) k_adj_nrm(StateDev s, const double* gnrm
 // u-dot v-dot on the boundary, where the update leaves them) goes to gout, which the
 // assembly's adjoint then adds to.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NBLK) k_adj_update(const unsigned char* flags, int W, int Lg, const double* gin,
-                                                     const double* pbar, const double* is_slide, const double* sol, int Wp, int P,
-                                                     double h, double* rhs, double* gslide, double* gout) {
+__device__ __forceinline__ void adj_update_node(const unsigned char* flags, int W, int Lg, const double* gin, const double* pbar,
+                                                const double* is_slide, const double* sol, int Wp, int P, double h, double* rhs,
+                                                double* gslide, double* gout, int k) {
     const int N = W * Lg;
-    const int k = blockIdx.x * NBLK + threadIdx.x;
     if (k >= N) return;
     const int u = k % W, v = k / W;
     const size_t pk = (size_t)padidx(Wp, u, v);
@@ -571,9 +570,8 @@ __global__ void __launch_bounds__(NBLK) k_adj_update(const unsigned char* flags,
 
 // x-bar += pbar gathered over the crosses (the last normals' adjoint, for the initial
 // state, whose normals yc_set_state computes from x)
-__global__ void __launch_bounds__(NBLK) k_adj_fold_nrm(int W, int Lg, const double* pbar, double* g) {
+__device__ __forceinline__ void adj_fold_nrm_node(int W, int Lg, const double* pbar, double* g, int k) {
     const int N = W * Lg;
-    const int k = blockIdx.x * NBLK + threadIdx.x;
     if (k >= N) return;
     const int u = k % W, v = k / W;
     for (int c = 0; c < 3; ++c) {
@@ -607,9 +605,8 @@ __global__ void __launch_bounds__(NBLK) k_adj_add_gstate(int N, const double* gs
 // against device targets (node-major (N,3) positions, (N,2) Eulerian coordinates):
 // w (sum |x - X|^2 + sum |uv - U|^2), its gradient added to g, the node's terms to
 // lpart (summed afterwards in a fixed order).
-__global__ void __launch_bounds__(NBLK) k_adj_target_loss(int N, const double* x, const double* uv, const double* X,
-                                                          const double* U, double w, double* g, double* lpart) {
-    const int k = blockIdx.x * NBLK + threadIdx.x;
+__device__ __forceinline__ void adj_target_loss_node(int N, const double* x, const double* uv, const double* X, const double* U,
+                                                     double w, double* g, double* lpart, int k) {
     if (k >= N) return;
     double l = 0.0;
     for (int c = 0; c < 3; ++c) {
@@ -643,8 +640,7 @@ __global__ void __launch_bounds__(256) k_plane_sums(const double* a, int n, doub
 
 // The 18 state planes of a context (x, v, xfix, uv, uvd, uvbar, nrm) to and from one
 // contiguous checkpoint block.
-__global__ void __launch_bounds__(NBLK) k_ckpt(StateDev s, double* blk, int save) {
-    const int k = blockIdx.x * NBLK + threadIdx.x;
+__device__ __forceinline__ void ckpt_node(const StateDev& s, double* blk, int save, int k) {
     if (k >= s.N) return;
     const size_t N = s.N;
     double* planes[7] = {s.x, s.v, s.xfix, s.uv, s.uvd, s.uvbar, s.nrm};
@@ -656,5 +652,44 @@ __global__ void __launch_bounds__(NBLK) k_ckpt(StateDev s, double* blk, int save
             else planes[f][(size_t)c * N + k] = blk[(size_t)q * N + k];
         }
 }
+
+// ---------------------------------------------------------------------------
+// Single-cloth kernels
+// ---------------------------------------------------------------------------
+template <bool REF>
+__global__ void __launch_bounds__(TILE_N) k_adj_elems(ClothDev C, AdjW Wt, AdjOut O) { adj_elems_tile<REF>(C, Wt, O, blockIdx.x); }
+template <bool REF>
+__global__ void __launch_bounds__(TILE_N) k_adj_elems_par(ClothDev C, AdjW Wt, AdjOut O, int pass) {
+    adj_elems_par_tile<REF>(C, Wt, O, pass, blockIdx.x);
+}
+template <bool REF>
+__global__ void __launch_bounds__(TILE_N) k_adj_shear(ClothDev C, AdjW Wt, AdjOut O, const double* comp) {
+    adj_shear_tile<REF>(C, Wt, O, comp, blockIdx.x);
+}
+template <bool REF>
+__global__ void __launch_bounds__(TILE_N) k_adj_shear_par(ClothDev C, AdjW Wt, AdjOut O, const double* comp) {
+    adj_shear_par_tile<REF>(C, Wt, O, comp, blockIdx.x);
+}
+template <bool REF>
+__global__ void __launch_bounds__(TILE_N) k_adj_fric(ClothDev C, AdjW Wt, AdjOut O, const double* fric_in, const double* gslide,
+                                                     int fpar) {
+    adj_fric_tile<REF>(C, Wt, O, fric_in, gslide, fpar, blockIdx.x);
+}
+__global__ void __launch_bounds__(NBLK) k_adj_nrm(StateDev s, const double* gnrm, double* pbar) {
+    adj_nrm_node(s, gnrm, pbar, blockIdx.x * NBLK + threadIdx.x);
+}
+__global__ void __launch_bounds__(NBLK) k_adj_update(const unsigned char* flags, int W, int Lg, const double* gin,
+                                                     const double* pbar, const double* is_slide, const double* sol, int Wp, int P,
+                                                     double h, double* rhs, double* gslide, double* gout) {
+    adj_update_node(flags, W, Lg, gin, pbar, is_slide, sol, Wp, P, h, rhs, gslide, gout, blockIdx.x * NBLK + threadIdx.x);
+}
+__global__ void __launch_bounds__(NBLK) k_adj_fold_nrm(int W, int Lg, const double* pbar, double* g) {
+    adj_fold_nrm_node(W, Lg, pbar, g, blockIdx.x * NBLK + threadIdx.x);
+}
+__global__ void __launch_bounds__(NBLK) k_adj_target_loss(int N, const double* x, const double* uv, const double* X,
+                                                          const double* U, double w, double* g, double* lpart) {
+    adj_target_loss_node(N, x, uv, X, U, w, g, lpart, blockIdx.x * NBLK + threadIdx.x);
+}
+__global__ void __launch_bounds__(NBLK) k_ckpt(StateDev s, double* blk, int save) { ckpt_node(s, blk, save, blockIdx.x * NBLK + threadIdx.x); }
 
 }  // namespace

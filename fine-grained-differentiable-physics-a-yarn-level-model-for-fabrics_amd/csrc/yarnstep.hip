@@ -427,10 +427,8 @@ __device__ __forceinline__ void stencil_rows(int g, S slot, X xin, double* y0, d
 
 __device__ __forceinline__ int row_slot0(int g) { return g < 3 ? g * 45 : SLOT_EU; }
 
-// y = A x; optionally partial sums of x . y per block (one block per tile)
-__global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, const CGScal* sc, int check_done) {
-    if (check_done && ld_done(sc)) return;
-    int tile = blockIdx.x;
+// y = A x on one tile; optionally partial sums of x . y per block (one block per tile)
+__device__ __forceinline__ void spmv_tile(const SpmvArgs& a, double* partials, int tile) {
     int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     int u = (tile % a.ntx) * TILE + (lane & 7), v = (tile / a.ntx) * TILE + (lane >> 3);
     double dotp = 0.0;
@@ -455,8 +453,13 @@ __global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, cons
     }
     if (partials) {
         double s = block_sum(dotp);
-        if (threadIdx.x == 0) partials[blockIdx.x] = s;
+        if (threadIdx.x == 0) partials[tile] = s;
     }
+}
+
+__global__ void __launch_bounds__(BLK) k_spmv(SpmvArgs a, double* partials, const CGScal* sc, int check_done) {
+    if (check_done && ld_done(sc)) return;
+    spmv_tile(a, partials, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1502,9 +1505,9 @@ __global__ void __launch_bounds__(NBLK) k_update_tan(StateDev s, TanState t, con
 
 // At = A^T in the same stencil layout (the stencil is symmetric in structure): the
 // entry of row (n, rr), column (m, cc) of At is A's entry of row (m, cc), column (n, rr).
-__global__ void __launch_bounds__(NBLK) k_transpose(const double* A, double* At, int W, int Lg, int ntx) {
-    int u = (blockIdx.x % ntx) * TILE + (threadIdx.x & (TILE - 1));
-    int v = (blockIdx.x / ntx) * TILE + (threadIdx.x / TILE);
+__device__ __forceinline__ void transpose_tile(const double* A, double* At, int W, int Lg, int ntx, int tile) {
+    int u = (tile % ntx) * TILE + (threadIdx.x & (TILE - 1));
+    int v = (tile / ntx) * TILE + (threadIdx.x / TILE);
     if (u >= W || v >= Lg) return;
     double* out = At + a_base(ntx, u, v);
     for (int sidx = 0; sidx < NSLOT; ++sidx) {
@@ -1518,6 +1521,10 @@ __global__ void __launch_bounds__(NBLK) k_transpose(const double* A, double* At,
         }
         out[(size_t)sidx * TILE_N] = x;
     }
+}
+
+__global__ void __launch_bounds__(NBLK) k_transpose(const double* A, double* At, int W, int Lg, int ntx) {
+    transpose_tile(A, At, W, Lg, ntx, blockIdx.x);
 }
 
 }  // namespace
@@ -3920,3 +3927,4 @@ int yc_adjoint_backward(yc_ctx* c, int mode, double h, double tol, int maxit, co
 }  // extern "C"
 
 #include "batch.hpp"
+#include "batch_adjoint.hpp"

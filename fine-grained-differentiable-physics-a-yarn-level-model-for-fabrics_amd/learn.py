@@ -158,6 +158,45 @@ def sample_loss_terms_adjoint(spec, frames, names, mode="ref", tol=1e-10, cg="pe
     return res["loss"], grad, ok
 
 
+def batch_loss_terms_adjoint(specs, frames_list, names, mode="ref", tol=1e-10, precond="jacobi"):
+    """``sample_loss_terms_adjoint`` for many cloths at once: the rollouts are recorded
+    together and their backward passes run together (``ClothBatch.adjoint_run`` /
+    ``adjoint_backward``: one launch per phase over all cloths, the solves in one
+    cooperative launch).  Returns a list of (loss_sum, grad_sum, steps_ok) per cloth.  If
+    a step of the batch fails (NaN/Inf, no convergence), every cloth is redone alone, so
+    each stops at its own first failed step as the reference's rollouts do."""
+    from .batch import ClothBatch
+    nsteps = frames_list[0]["LPos"].shape[0] - 1
+    if any(f["LPos"].shape[0] - 1 != nsteps for f in frames_list):
+        raise ValueError("the batched rollouts need trajectories of one length")
+    cloths, b = [], None
+    try:
+        for spec, f in zip(specs, frames_list):
+            c = Cloth(spec)
+            cloths.append(c)
+            c.set_cg_variant("persistent")
+            c.set_preconditioner(precond)
+            c.set_ClothState({k: f[k][0] for k in f})
+            c.adjoint_begin(nsteps)
+        b = ClothBatch(cloths)
+        try:
+            b.adjoint_run(specs[0]["h"], nsteps, tol=tol)
+        except (_abi.NonFinite, _abi.NotConverged):
+            b.close()
+            b = None
+            return [sample_loss_terms_adjoint(sp, f, names, mode, tol, precond=precond) for sp, f in zip(specs, frames_list)]
+        for c, f in zip(cloths, frames_list):
+            c.adjoint_set_targets(f["LPos"][1:], f["EPos"][1:], 1.0 / c.N)
+        dparam, loss, _ = b.adjoint_backward(specs[0]["h"], mode, tol=tol)
+    finally:
+        if b is not None:
+            b.close()
+        for c in cloths:
+            c.close()
+    idx = [Cloth.PARAMS.index(n) for n in names]
+    return [(float(loss[i]), dparam[i, idx], nsteps) for i in range(len(cloths))]
+
+
 class ParameterLearner:
     """SGD on the boxed parameters of one cloth, as the reference's training loop.
 

@@ -419,6 +419,21 @@ int yc_batch_run(yc_batch* b, double h, int steps, double tol, int maxit, long l
 /* Layout facts: cooperative groups, whether the last run replayed a graph, tiles of all
  * cloths, tiles resident at once. */
 int yc_batch_info(const yc_batch* b, int* n_groups, int* graph_used, int* total_tiles, int* capacity);
+/* Reverse mode of a batch: the members' adjoint recordings (yc_adjoint_begin on each, with
+ * checkpoint interval 1 and the same number of steps recorded) advanced together and
+ * their backward passes run together, each phase one launch over all cloths and the
+ * backward solves one cooperative launch; each cloth's gradient is its single-cloth
+ * backward pass's.  No counterpart in the reference, whose training drivers differentiate
+ * one cloth per pass (src/TrainHeter.h:161-231).
+ * yc_batch_adjoint_run: `steps` recorded SimSteps of every member (as yc_batch_run); a
+ * failed step voids every member's recording.
+ * yc_batch_adjoint_backward: the members' target losses (yc_adjoint_set_targets on each)
+ * differentiated with respect to their parameters: dparam n x YC_NPARAM and loss[n] (may
+ * be NULL), iters[i] the backward solves' iterations, status[i] as yc_batch_run.  The
+ * members end at their last recorded states. */
+int yc_batch_adjoint_run(yc_batch* b, double h, int steps, double tol, int maxit, long long* iters, int* status);
+int yc_batch_adjoint_backward(yc_batch* b, int mode, double h, double tol, int maxit, double* dparam, double* loss,
+                              long long* iters, int* status);
 
 #ifdef __cplusplus
 }

@@ -206,3 +206,67 @@ def test_adjoint_target_loss(ys, cs):
     # the backward pass leaves the context at the last recorded state
     for k in KEYS:
         assert np.array_equal(after[k], traj[-1][k])
+
+
+@pytest.mark.parametrize("mode,precond", [("ref", "jacobi"), ("exact", "block2")])
+def test_batch_adjoint_equals_single_cloths(ys, cs, mode, precond):
+    """Cloths of uneven size recorded and differentiated together (yc_batch_adjoint_*):
+    each cloth's loss and parameter gradient equal its own single-cloth adjoint's."""
+    from yarnstep.batch import ClothBatch
+    sizes = [(8, 8, "plain", 21), (12, 10, "twill", 22), (6, 9, "satin", 23), (16, 16, "twill", 24)]
+    steps = 5
+    specs, states, targets = [], [], []
+    for W, Lg, pat, seed in sizes:
+        sp = _spec(cs, W, Lg, pat, seed)
+        st = _natural(ys, sp, 3)
+        tsp = dict(sp)
+        tsp["yarns"] = [dict(sp["yarns"][0], rho=sp["yarns"][0]["rho"] * 1.2), sp["yarns"][1]]
+        t = ys.Cloth(tsp)
+        t.set_ClothState(st)
+        X, U = [], []
+        for _ in range(steps):
+            t.step(sp["h"], tol=1e-12)
+            s = t.get_ClothState()
+            X.append(s["LPos"])
+            U.append(s["EPos"])
+        t.close()
+        specs.append(sp)
+        states.append(st)
+        targets.append((np.array(X), np.array(U)))
+    h = specs[0]["h"]
+    single = []
+    for sp, st, (X, U) in zip(specs, states, targets):
+        c = ys.Cloth(sp)
+        c.set_cg_variant("persistent")
+        c.set_preconditioner(precond)
+        c.set_ClothState(st)
+        c.adjoint_begin(steps)
+        c.adjoint_run(h, steps, tol=1e-12)
+        c.adjoint_set_targets(X, U, 1.0 / c.N)
+        r = c.adjoint_backward(h, mode, tol=1e-12)
+        single.append((r["loss"], r["param"]))
+        c.adjoint_end()
+        c.close()
+    cloths = []
+    for sp, st, (X, U) in zip(specs, states, targets):
+        c = ys.Cloth(sp)
+        c.set_cg_variant("persistent")
+        c.set_preconditioner(precond)
+        c.set_ClothState(st)
+        c.adjoint_begin(steps)
+        cloths.append(c)
+    b = ClothBatch(cloths)
+    b.adjoint_run(h, steps, tol=1e-12)
+    for c, (X, U) in zip(cloths, targets):
+        c.adjoint_set_targets(X, U, 1.0 / c.N)
+    dparam, loss, iters = b.adjoint_backward(h, mode, tol=1e-12)
+    b.close()
+    for c in cloths:
+        c.adjoint_end()
+        c.close()
+    for i, (l1, g1) in enumerate(single):
+        rel = np.max(np.abs(dparam[i] - g1)) / max(np.max(np.abs(g1)), 1e-300)
+        say("  cloth %d (%dx%d) loss %.6e / %.6e, max rel gradient difference %.2e, backward iterations %d" %
+            (i, sizes[i][0], sizes[i][1], loss[i], l1, rel, iters[i]))
+        assert loss[i] == pytest.approx(l1, rel=1e-12)
+        assert rel <= 1e-12
