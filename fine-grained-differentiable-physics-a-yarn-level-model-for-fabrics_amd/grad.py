@@ -6,8 +6,10 @@ stiffness and friction by backpropagating a position loss through SimStep
 EigenSolver::apply, src/Solvers.h:35-52).  Here the step's tangents are propagated
 forward on the GPU (Cloth.step_tangent, one direction per parameter), so the Jacobian of
 the whole trajectory with respect to the parameters is available after the forward
-rollout and the backward pass is a contraction with it: no checkpoint, no recompute
-and O(1) memory in the rollout length.
+rollout and the backward pass is a contraction with it.  The autograd Function
+(make_autograd_function) uses the reverse mode by default: the forward pass records the
+rollout, the backward pass gives every parameter's derivative with one backward solve per
+step (Cloth.adjoint_*).
 
 Two gradients are offered: "ref", the reference's own (M, K_v, A and the non-jamming
 shear angle held constant, backward solve with A instead of A^T), and "exact", the
@@ -82,26 +84,52 @@ def loss_and_grad(spec, state, steps, names, target, mode="ref", **kw):
     return loss, g
 
 
-def make_autograd_function():
-    """torch.autograd.Function over the rollout: forward(theta) -> positions
-    (steps, N, 3) as float64 on theta's device; backward contracts the upstream gradient
-    with the forward-mode Jacobian saved in the forward pass and returns it with
-    theta's device and dtype."""
+def make_autograd_function(form="adjoint"):
+    """torch.autograd.Function over the rollout: forward(theta, spec, state, steps, names,
+    mode) -> positions (steps, N, 3) as float64 on theta's device; backward returns the
+    gradient with theta's device and dtype (EigenSolver::apply's role in the reference,
+    src/Solvers.h:35-52, for the whole rollout).
+    form "adjoint": the forward pass records the rollout on the GPU and the backward pass
+    runs the reverse mode (one backward solve per step, every parameter at once), with the
+    upstream gradient as the loss gradient of each step's positions.
+    form "tangent": the forward pass propagates one tangent per parameter and the backward
+    pass contracts the upstream gradient with that Jacobian."""
     import torch
 
     class YarnRollout(torch.autograd.Function):
         @staticmethod
         def forward(ctx, theta, spec, state, steps, names, mode):
             vals = theta.detach().cpu().numpy().astype(np.float64)
-            pos, jac = rollout_jacobian(with_params(spec, names, vals), state, steps, names, mode=mode)
             ctx.theta_device, ctx.theta_dtype = theta.device, theta.dtype
-            ctx.save_for_backward(torch.from_numpy(jac).to(theta.device))
+            if form == "tangent":
+                pos, jac = rollout_jacobian(with_params(spec, names, vals), state, steps, names, mode=mode)
+                ctx.save_for_backward(torch.from_numpy(jac).to(theta.device))
+                return torch.from_numpy(pos).to(theta.device)
+            c = Cloth(with_params(spec, names, vals))
+            configure_solver(c)
+            c.set_ClothState(state)
+            c.adjoint_begin(steps)
+            pos = np.zeros((steps, c.N, 3))
+            for t in range(steps):
+                c.adjoint_step(spec["h"], tol=1e-10)
+                pos[t] = c.get_ClothState()["LPos"]
+            ctx.cloth, ctx.names, ctx.mode, ctx.h = c, list(names), mode, spec["h"]
             return torch.from_numpy(pos).to(theta.device)
 
         @staticmethod
         def backward(ctx, g):
-            (jac,) = ctx.saved_tensors
-            grad = (jac * g.to(jac.dtype).unsqueeze(0)).reshape(jac.shape[0], -1).sum(dim=1)
+            if form == "tangent":
+                (jac,) = ctx.saved_tensors
+                grad = (jac * g.to(jac.dtype).unsqueeze(0)).reshape(jac.shape[0], -1).sum(dim=1)
+            else:
+                c = ctx.cloth
+                gn = g.detach().to(dtype=torch.float64).cpu().numpy()
+                try:
+                    res = c.adjoint_backward(ctx.h, ctx.mode, gstates=[{"LPos": gn[t]} for t in range(gn.shape[0])], tol=1e-10)
+                finally:
+                    c.adjoint_end()
+                    c.close()
+                grad = torch.from_numpy(np.array([res["param"][PARAMS.index(n)] for n in ctx.names]))
             return grad.to(device=ctx.theta_device, dtype=ctx.theta_dtype), None, None, None, None, None
 
     return YarnRollout

@@ -281,7 +281,8 @@ def test_ref_and_exact_modes_differ(ys, cs):
     assert d > 1e-3
 
 
-def test_autograd_rollout_gradient_matches_fd(ys, cs):
+@pytest.mark.parametrize("form", ["adjoint", "tangent"])
+def test_autograd_rollout_gradient_matches_fd(ys, cs, form):
     """The torch.autograd.Function over an 8-step rollout: the gradient of the position
     loss against a target trajectory (seeded ground-truth parameters, as the parameter-
     learning drivers use) matches the oracle's central difference of the same loss along
@@ -296,7 +297,7 @@ def test_autograd_rollout_gradient_matches_fd(ys, cs):
     steps = 8
     target, _ = grad.rollout_jacobian(spec, state, steps, names, mode="exact", tol=1e-14, maxit=20000)
     theta0 = truth * np.array([1.3, 0.8, 1.2, 0.7])
-    fn = grad.make_autograd_function()
+    fn = grad.make_autograd_function(form)
     theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
     pos = fn.apply(theta, grad.with_params(spec, names, theta0), state, steps, names, "exact")
     loss = torch.mean((pos - torch.from_numpy(target)) ** 2)
@@ -319,7 +320,8 @@ def test_autograd_rollout_gradient_matches_fd(ys, cs):
     e = 1e-4
     fd = (oracle_loss(theta0 + e * dirn) - oracle_loss(theta0 - e * dirn)) / (2 * e)
     an = float(g @ dirn)
-    say("  loss %.4e  directional derivative: autograd %.8e  oracle fd %.8e  rel %.2e" % (float(loss), an, fd, abs(an - fd) / abs(fd)))
+    say("  %s: loss %.4e  directional derivative: autograd %.8e  oracle fd %.8e  rel %.2e" %
+        (form, float(loss), an, fd, abs(an - fd) / abs(fd)))
     assert abs(an - fd) <= 1e-5 * abs(fd)
 
 
