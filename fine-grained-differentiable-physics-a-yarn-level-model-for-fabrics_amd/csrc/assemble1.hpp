@@ -60,7 +60,12 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
             else if (ei == 1) { if (u == C.W - 1) continue; eu = u; ev = v; ec = 0; idx = v * (C.W - 1) + u; }
             else if (ei == 2) { if (v == 0) continue; eu = u; ev = v - 1; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v - 1; }
             else { if (v == C.Lg - 1) continue; eu = u; ev = v; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v; }
+            // value run: the edge's entries in the node's rows summed in registers, then into
+            // the slots (the tangent run has no registers to spare for the dual sums)
+            const int uo = eu == u && ev == v ? eu + (ec == 0) : eu, vo = eu == u && ev == v ? ev + (ec == 1) : ev;
+            if constexpr (!tan) S.edge_begin();
             edge_element<T>(C, S, eu, ev, ec, ec == 0 ? (v & 1) : (u & 1), omp_owner(C.n_edges, idx));
+            if constexpr (!tan) S.edge_end(uo + vo * C.W, S.inter, S.inter && interior(C, uo, vo), ec);
         }
         // bends centred on u-1, u, u+1 (warp) and v-1, v, v+1 (weft) (src/Cloth.cpp:230-270):
         // a bend centred d after the node holds it as n2 (role 0), centred on it as n0

@@ -49,11 +49,9 @@ __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (
 // (455 us against 360 us for the node kernel), so the records are off.
 constexpr bool USE_BEND_REC = false;
 
-// LDS of the value assembly kernels: one tile's slot planes (81,408 bytes, above the
-// 64 KiB default, allowed per kernel at context creation)
-constexpr size_t ASM_LDS = (size_t)NSLOT * TILE_N * sizeof(double);
-// the first assembly kernel's value run: the slot planes, then the second half's L-L self
-// block (9 planes) and accumulators (RowSinkT::NACC planes) -- 101,888 bytes
+// LDS of the first assembly kernel's value run: one tile's slot planes (81,408 bytes), then
+// the second half's L-L self block (9 planes) and accumulators (RowSinkT::NACC planes) --
+// 101,888 bytes, above the 64 KiB default, allowed per kernel at context creation
 constexpr size_t ASM1_LDS = (size_t)(NSLOT + 9 + RowSinkT<double>::NACC) * TILE_N * sizeof(double);
 constexpr int ASM1_THREADS = 2 * TILE_N;
 

@@ -68,9 +68,9 @@ __global__ void __launch_bounds__(ASM1_THREADS, 1) k_b_assemble1(const BatchItem
     assemble1_block<double, false>(bi.C, bi.O, h, blockIdx.x - bi.t0);
 }
 
-__global__ void __launch_bounds__(NBLK, 1) k_b_assemble2(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
+__global__ void __launch_bounds__(ASM2_THREADS) k_b_assemble2(const BatchItem* __restrict__ it, const int* __restrict__ map, double h) {
     const BatchItem& b = it[map[blockIdx.x]];
-    assemble2_tile<double>(b.C, b.O, h, blockIdx.x - b.t0);
+    assemble2_value(b.C, b.O, h, blockIdx.x - b.t0);
 }
 
 template <int NB>
@@ -300,7 +300,7 @@ int batch_step(yc_batch* b, double h) {
     HIPCHK(hipGetLastError());
     int rc = batch_node_phase(b, h, 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_b_assemble2, dim3(b->n_tile), dim3(NBLK), ASM_LDS, b->stream, it, (const int*)b->map_tile, h);
+    hipLaunchKernelGGL(k_b_assemble2, dim3(b->n_tile), dim3(ASM2_THREADS), 0, b->stream, it, (const int*)b->map_tile, h);
     hipLaunchKernelGGL(k_b_cg_init, dim3(b->n_vec), dim3(BLK), 0, b->stream, it, (const int*)b->map_vec);
     if (b->bj)   // each cloth's block-Jacobi rows from its new A, on the batch stream
         for (yc_ctx* c : b->m) {
@@ -373,8 +373,7 @@ int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out) {
     auto bail = [&](int rc) { yc_batch_destroy(b); (void)hipSetDevice(cur); return rc; };
     if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(b->device) != hipSuccess) return bail(fail(YC_ERR_HIP, "batch: device"));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(YC_ERR_HIP, "batch: stream"));
-    if (hipFuncSetAttribute((const void*)k_b_assemble1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM1_LDS) != hipSuccess ||
-        hipFuncSetAttribute((const void*)k_b_assemble2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess)
+    if (hipFuncSetAttribute((const void*)k_b_assemble1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM1_LDS) != hipSuccess)
         return bail(fail(YC_ERR_HIP, "batch: assembly LDS size not supported"));
     if (hipMalloc((void**)&b->d_status, sizeof(long long) * BSTAT * n) != hipSuccess ||
         hipHostMalloc((void**)&b->h_status, sizeof(long long) * BSTAT * n) != hipSuccess)

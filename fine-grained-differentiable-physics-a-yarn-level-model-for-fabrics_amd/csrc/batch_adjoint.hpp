@@ -350,7 +350,7 @@ int yc_batch_adjoint_backward(yc_batch* b, int mode, double h, double tol, int m
             nodes(k_ba_ckpt, s, T, 0);
             hipLaunchKernelGGL(k_b_assemble1, dim3(b->n_tile), dim3(ASM1_THREADS), ASM1_LDS, b->stream, it2, mt, h);
             if ((rc = batch_node_phase(b, h, 0))) break;   // the shear records of every member
-            hipLaunchKernelGGL(k_b_assemble2, dim3(b->n_tile), dim3(NBLK), ASM_LDS, b->stream, it2, mt, h);
+            hipLaunchKernelGGL(k_b_assemble2, dim3(b->n_tile), dim3(ASM2_THREADS), 0, b->stream, it2, mt, h);
             if (exact) tiles(k_ba_transpose, NBLK);
             if (b->bj)
                 for (yc_ctx* c : b->m) {

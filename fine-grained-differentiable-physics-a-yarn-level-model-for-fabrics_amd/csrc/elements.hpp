@@ -209,6 +209,12 @@ struct RowSinkT {
     v3<T> SBC_LE[2];
     int fric_owner;
     int bad;
+    // While racc_on, the entries of A in this node's rows that an edge emits are summed in
+    // registers, per column (0: the node itself, 1: the edge's other end) and entry (0-8 the
+    // L-L block row-major, 9-11 L rows x E column, 12-14 E row x L columns, 15 E-E), and
+    // edge_end() adds each sum to its slot once, instead of one slot update per term.
+    bool racc_on;
+    T racc[2][16];
 
     YS_DEV void init(const ClothDev* c, int k, double* a, double h, double* da = nullptr, bool det = false) {
         C = c; me = k; mu_ = k % c->W; mv_ = k / c->W; inter = interior(*c, mu_, mv_);
@@ -220,6 +226,7 @@ struct RowSinkT {
         SBC_L = V3<T>(T(0.0), T(0.0), T(0.0)); SBC_LE[0] = SBC_L; SBC_LE[1] = SBC_L;
         fric_owner = inter ? omp_owner(c->Nc, crossing(*c, mu_, mv_)) : -1;
         bad = 0;
+        racc_on = false;
     }
     // Rows of one matrix of the system alone (kind 0: M, 1: K_x, 2: K_v, each with
     // coefficient 1): the reference's get_SpaM / get_SpaFPos / get_SpaFVel
@@ -233,6 +240,30 @@ struct RowSinkT {
         int s = slot_of(rc, kcol % C->W - mu_, kcol / C->W - mv_, cc);
         if (s < 0) bad = 1;
         return s;
+    }
+    YS_DEV void accA(int kcol, int i, T x) {
+        if (kcol == me) racc[0][i] += x;
+        else racc[1][i] += x;
+    }
+    YS_DEV void edge_begin() {
+        racc_on = true;
+        for (int i = 0; i < 16; ++i) { racc[0][i] = T(0.0); racc[1][i] = T(0.0); }
+    }
+    // the sums into the slots: the L-L blocks of both columns; the E entries of the node's
+    // own column when it is interior (e_me), of the other end's when both ends are (e_o),
+    // the entries the edge's terms emit under the same conditions
+    YS_DEV void edge_end(int kother, bool e_me, bool e_o, int c) {
+        racc_on = false;
+        for (int hc = 0; hc < 2; ++hc) {
+            const int col = hc == 0 ? me : kother;
+            for (int rc = 0; rc < 3; ++rc)
+                for (int cc = 0; cc < 3; ++cc) addA(rc, col, cc, racc[hc][rc * 3 + cc]);
+            if (hc == 0 ? e_me : e_o) {
+                for (int rc = 0; rc < 3; ++rc) addA(rc, col, 3 + c, racc[hc][9 + rc]);
+                for (int cc = 0; cc < 3; ++cc) addA(3 + c, col, cc, racc[hc][12 + cc]);
+                addA(3 + c, col, 3 + c, racc[hc][15]);
+            }
+        }
     }
     YS_DEV void addA(int rc, int kcol, int cc, T x) {
         int s = slot(rc, kcol, cc);
@@ -261,7 +292,8 @@ struct RowSinkT {
         if (kF == me) {
             for (int u = 0; u < 3; ++u)
                 for (int v = 0; v < 3; ++v) {
-                    addA(v, kD, u, coef[kind] * B.m[u][v]);
+                    if (racc_on) accA(kD, v * 3 + u, coef[kind] * B.m[u][v]);
+                    else addA(v, kD, u, coef[kind] * B.m[u][v]);
                     if (kind == KIND_M) Mv[v] += blk(B.m[u][v]) * velL(kD, u);
                 }
         }
@@ -274,7 +306,8 @@ struct RowSinkT {
         T b[3] = {B.x, B.y, B.z};
         if (kL == me)
             for (int v = 0; v < 3; ++v) {
-                addA(v, kE, 3 + c, coef[kind] * b[v]);
+                if (racc_on) accA(kE, 9 + v, coef[kind] * b[v]);
+                else addA(v, kE, 3 + c, coef[kind] * b[v]);
                 if (kind == KIND_M) add5(Mv, v, blk(b[v]) * velE(kE, c));
             }
         if (kind == KIND_KV && kE == me)
@@ -285,7 +318,8 @@ struct RowSinkT {
         T b[3] = {B.x, B.y, B.z};
         if (kE == me)
             for (int u = 0; u < 3; ++u) {
-                addA(3 + c, kL, u, coef[kind] * b[u]);
+                if (racc_on) accA(kL, 12 + u, coef[kind] * b[u]);
+                else addA(3 + c, kL, u, coef[kind] * b[u]);
                 if (kind == KIND_M) add5(Mv, 3 + c, blk(b[u]) * velL(kL, u));
             }
         if (kind == KIND_KV && kL == me)
@@ -294,7 +328,8 @@ struct RowSinkT {
     // EE: A[3N+2e(kF)+c, 3N+2e(kD)+c] = val
     YS_DEV void EE(int kind, int kF, int kD, int c, T v) {
         if (kF == me) {
-            addA(3 + c, kD, 3 + c, coef[kind] * v);
+            if (racc_on) accA(kD, 15, coef[kind] * v);
+            else addA(3 + c, kD, 3 + c, coef[kind] * v);
             if (kind == KIND_M) add5(Mv, 3 + c, blk(v) * velE(kD, c));
             if (kD == me) {
                 if (kind == KIND_KX) add2(KxEE, c, v);

@@ -209,27 +209,122 @@ __device__ void emit_shear_role(Sink& S, const int (&k)[3], int r, const v3<T>& 
 __device__ __forceinline__ int shear_seg(const ClothDev& C, int e, int q);
 __device__ __forceinline__ void shear_load(const double* rec, int nseg, int s, int r, d3& F, m33 (&B)[3]);
 
+// The up to 12 segments touching a node, in the order of the crossings (u-1, v), (u, v-1),
+// (u, v), (u, v+1), (u+1, v) and, per crossing, of the reference's shear_segs ((left,bot),
+// (left,top), (right,bot), (right,top); src/Cloth.cpp:170-181): the crossing's offset from
+// the node, the segment q, and the node's role in it (0: centre, 1: warp neighbour, 2: weft
+// neighbour).
+constexpr int SEG_DU[12] = {-1, -1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1};
+constexpr int SEG_DV[12] = {0, 0, -1, -1, 0, 0, 0, 0, 1, 1, 0, 0};
+constexpr int SEG_Q[12] = {2, 3, 0, 2, 0, 1, 2, 3, 1, 3, 0, 1};
+constexpr int SEG_R[12] = {1, 1, 2, 2, 0, 0, 0, 0, 2, 2, 1, 1};
+// Index in SHEAR_O (the 3 x 3 neighbourhood's stencil offsets 0-4, 9-12) of the node at
+// (du, dv) from the row's node.
+constexpr int shear_oi(int du, int dv) {
+    return dv == 0 ? (du == 0 ? 0 : du < 0 ? 1 : 2)
+                   : du == 0 ? (dv < 0 ? 3 : 4) : (dv < 0 ? (du < 0 ? 5 : 6) : (du < 0 ? 7 : 8));
+}
+// The node column of block jj (0: crossing, 1: warp neighbour, 2: weft neighbour) of the
+// node's segment j, as a SHEAR_O index.
+constexpr int seg_block_oi(int j, int jj) {
+    return shear_oi(SEG_DU[j] + (jj == 1 ? ((SEG_Q[j] >> 1) == 0 ? -1 : 1) : 0),
+                    SEG_DV[j] + (jj == 2 ? ((SEG_Q[j] & 1) == 0 ? 1 : -1) : 0));
+}
+constexpr int SHEAR_O[9] = {0, 1, 2, 3, 4, 9, 10, 11, 12};
+constexpr int ASM2_THREADS = 3 * TILE_N;
+
+// The value run of the second assembly kernel for one tile: three threads per node, thread
+// rc taking the node's L row rc.  The shear writes only the L-L blocks of the 3 x 3 node
+// neighbourhood, so row rc receives 27 slots (9 column nodes x 3 components); they start
+// from the first kernel's values and collect the segments' contributions in registers, in
+// the segments' order (the same sums, in the same order, as one update per contribution),
+// with each segment's record loaded while the one before is added.  Thread rc also adds
+// the shear force to F and b of its component and writes its Jacobi diagonal entry; thread
+// 0 the E rows' entries.  A[3F+v, 3D+u] = B[u][v] (src/MatrixMp.cpp:65-76): row component
+// rc takes B[u][rc], u = 0..2.
+__device__ __forceinline__ void assemble2_value(const ClothDev& C, const AsmOut& O, double h, int tile) {
+    const int nl = threadIdx.x & (TILE_N - 1), rc = threadIdx.x / TILE_N;
+    const int k = tile_node(C, tile, nl);
+    if (k < 0) return;
+    const int u = k % C.W, v = k / C.W;
+    const bool inter = interior(C, u, v);
+    // -h^2: K_x's coefficient in A (or 1 / 0 for the rows of one matrix alone)
+    const double coef = O.only_kind < 0 ? -(h * h) : (O.only_kind == KIND_KX ? 1.0 : 0.0);
+    double* gA = O.A + a_base(C.ntx, u, v);
+    double acc[27];
+#pragma unroll
+    for (int oi = 0; oi < 9; ++oi)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) acc[oi * 3 + cc] = gA[(size_t)(rc * 45 + SHEAR_O[oi] * 3 + cc) * TILE_N];
+    bool ok[12];
+    int sidx[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        const int cu = u + SEG_DU[j], cv = v + SEG_DV[j];
+        ok[j] = interior(C, cu, cv);
+        sidx[j] = ok[j] ? shear_seg(C, crossing(C, cu, cv), SEG_Q[j]) : 0;
+    }
+    // a record's part for row rc: F[rc] and B[jj][cc][rc] (planes rec[(r 30 + f) nseg + s])
+    double Fb[2], Bb[2][9];
+    auto load = [&](int j, int buf) {
+        const double* base = O.shear_rec + (size_t)SEG_R[j] * 30 * O.nseg + sidx[j];
+        Fb[buf] = base[(size_t)rc * O.nseg];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) Bb[buf][jj * 3 + cc] = base[(size_t)(3 + jj * 9 + cc * 3 + rc) * O.nseg];
+    };
+    if (ok[0]) load(0, 0);
+    double fsum = 0.0;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        if (j + 1 < 12 && ok[j + 1]) load(j + 1, (j + 1) & 1);
+        if (!ok[j]) continue;
+        fsum += Fb[j & 1];
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) acc[seg_block_oi(j, jj) * 3 + cc] += coef * Bb[j & 1][jj * 3 + cc];
+    }
+    const int pk = padidx(O.Wp, u, v);
+    O.F[rc * C.N + k] += fsum;
+    const double bv = O.b[rc * O.P + pk] + h * fsum;
+    O.b[rc * O.P + pk] = bv;
+    bool bad = !isfinite(bv);
+    // Jacobi diagonal (the reference solver's DiagonalPreconditioner: 1/a_ii, or 1 if zero)
+    // self column (SHEAR_O index 0), component rc (selected: a run-time index would put
+    // the accumulators in scratch memory)
+    const double d = rc == 0 ? acc[0] : (rc == 1 ? acc[1] : acc[2]);
+    O.invdiag[rc * O.P + pk] = d != 0.0 ? 1.0 / d : 1.0;
+    if (rc == 0)
+        for (int c = 3; c < 5; ++c) {
+            double inv = 0.0;
+            if (inter) {   // E-E self entries, untouched by the shear
+                const double de = gA[(size_t)((c == 3 ? SLOT_EU : SLOT_EV) + 10) * TILE_N];
+                inv = de != 0.0 ? 1.0 / de : 1.0;
+            }
+            O.invdiag[c * O.P + pk] = inv;
+            if (!isfinite(O.b[c * O.P + pk])) bad = true;
+        }
+#pragma unroll
+    for (int oi = 0; oi < 9; ++oi)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            const double a = acc[oi * 3 + cc];
+            if (!isfinite(a)) bad = true;
+            gA[(size_t)(rc * 45 + SHEAR_O[oi] * 3 + cc) * TILE_N] = a;
+        }
+    if (bad) atomicOr(&O.flag[1], 1);
+}
+
+// The tangent run (T = dd): one thread per node replays the segments' tangent records
+// (k_shear_elem_tan) through the row sink into dF, db and, in exact mode, dA.
 template <class T, bool REF = false>
 __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& O, double h, int k) {
-    constexpr bool tan = !std::is_same<T, double>::value;
+    static_assert(!std::is_same<T, double>::value, "the value run is assemble2_value");
     RowSinkT<T, REF> S;
-    S.init(&C, k, tan ? nullptr : O.A, h, tan ? O.dA : nullptr, tan && O.detach);
-    if (!tan) S.select_kind(O.only_kind);
+    S.init(&C, k, nullptr, h, O.dA, O.detach);
     int u = S.mu_, v = S.mv_;
-    extern __shared__ double lds_rows[];
-    double* gA = S.A;
-    // the shear writes only the L-L blocks of the 3 x 3 node neighbourhood (offsets 0-4 and
-    // 9-12 of the stencil: 81 slots); only those go through LDS
-    constexpr int SHEAR_O[9] = {0, 1, 2, 3, 4, 9, 10, 11, 12};
-    if (!tan) {   // those slots into LDS, back to HBM at the end
-        S.A = lds_rows + threadIdx.x;
-        for (int rc = 0; rc < 3; ++rc)
-            for (int oi = 0; oi < 9; ++oi)
-                for (int cc = 0; cc < 3; ++cc) {
-                    const size_t sl = (size_t)(rc * 45 + SHEAR_O[oi] * 3 + cc) * TILE_N;
-                    S.A[sl] = gA[sl];
-                }
-    }
     auto K = [&](int uu, int vv) { return uu + vv * C.W; };
     // shear_segs order per crossing: (left,bot), (left,top), (right,bot), (right,top)
 #pragma unroll 1
@@ -244,60 +339,28 @@ __device__ __forceinline__ void assemble2_node(const ClothDev& C, const AsmOut& 
             int k1 = a == 0 ? K(cu - 1, cv) : K(cu + 1, cv);
             int k3 = bq == 0 ? K(cu, cv + 1) : K(cu, cv - 1);
             if (kc != k && k1 != k && k3 != k) continue;
-            // replay the segment's record for this node's role: its values (value run) or
-            // their tangents (tangent run, k_shear_elem_tan; the tangent outputs take only
-            // the tangent parts of the shear's forces and K_x blocks, so the values are 0)
+            // replay the segment's tangent record for this node's role (the tangent outputs
+            // take only the tangent parts of the shear's forces and K_x blocks, so the
+            // values are 0)
             int sidx = shear_seg(C, crossing(C, cu, cv), 2 * a + bq);
             int r = kc == k ? 0 : (k1 == k ? 1 : 2);
             const int kk[3] = {kc, k1, k3};
             d3 F;
             m33 B[3];
             shear_load(O.shear_rec, O.nseg, sidx, r, F, B);
-            if constexpr (tan) {
-                v3<T> Ft = V3<T>(T(0.0, F.x), T(0.0, F.y), T(0.0, F.z));
-                mat3<T> Bt[3];
-                for (int j = 0; j < 3; ++j)
-                    for (int aa = 0; aa < 3; ++aa)
-                        for (int bb = 0; bb < 3; ++bb) Bt[j].m[aa][bb] = T(0.0, B[j].m[aa][bb]);
-                emit_shear_role<T>(S, kk, r, Ft, Bt);
-            } else {
-                emit_shear_role<double>(S, kk, r, F, B);
-            }
+            v3<T> Ft = V3<T>(T(0.0, F.x), T(0.0, F.y), T(0.0, F.z));
+            mat3<T> Bt[3];
+            for (int j = 0; j < 3; ++j)
+                for (int aa = 0; aa < 3; ++aa)
+                    for (int bb = 0; bb < 3; ++bb) Bt[j].m[aa][bb] = T(0.0, B[j].m[aa][bb]);
+            emit_shear_role<T>(S, kk, r, Ft, Bt);
         }
     }
     int pk = padidx(O.Wp, u, v);
-    if (tan) {
-        for (int c = 0; c < 3; ++c) {
-            O.dF[c * C.N + k] += tan_of(S.F[c]);
-            O.db[c * O.P + pk] += h * tan_of(S.F[c]);
-        }
-        if (S.bad) atomicOr(&O.flag[0], 1);
-        return;
-    }
     for (int c = 0; c < 3; ++c) {
-        O.F[c * C.N + k] += val(S.F[c]);
-        O.b[c * O.P + pk] += h * val(S.F[c]);
+        O.dF[c * C.N + k] += tan_of(S.F[c]);
+        O.db[c * O.P + pk] += h * tan_of(S.F[c]);
     }
-    // Jacobi diagonal (the reference solver's DiagonalPreconditioner: 1/a_ii, or 1 if zero)
-    bool bad = false;
-    for (int c = 0; c < 5; ++c) {
-        double d = 0.0;
-        if (c < 3) d = S.A[(size_t)(c * 45 + c) * TILE_N];
-        else if (S.inter) d = gA[(size_t)((c == 3 ? SLOT_EU : SLOT_EV) + 10) * TILE_N];   // untouched by the shear
-        double inv = 0.0;
-        if (c < 3 || S.inter) inv = d != 0.0 ? 1.0 / d : 1.0;
-        O.invdiag[c * O.P + pk] = inv;
-        if (!isfinite(O.b[c * O.P + pk])) bad = true;
-    }
-    for (int rc = 0; rc < 3; ++rc)
-        for (int oi = 0; oi < 9; ++oi)
-            for (int cc = 0; cc < 3; ++cc) {
-                const size_t sl = (size_t)(rc * 45 + SHEAR_O[oi] * 3 + cc) * TILE_N;
-                const double a = S.A[sl];
-                if (!isfinite(a)) bad = true;
-                gA[sl] = a;
-            }
-    if (bad) atomicOr(&O.flag[1], 1);
     if (S.bad) atomicOr(&O.flag[0], 1);
 }
 
@@ -307,9 +370,8 @@ __device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& 
     if (k >= 0) assemble2_node<T, REF>(C, O, h, k);
 }
 
-template <class T>
-__global__ void __launch_bounds__(NBLK, 1) k_assemble2(ClothDev C, AsmOut O, double h) {
-    assemble2_tile<T>(C, O, h, blockIdx.x);
+__global__ void __launch_bounds__(ASM2_THREADS) k_assemble2(ClothDev C, AsmOut O, double h) {
+    assemble2_value(C, O, h, blockIdx.x);
 }
 
 // The second kernel's tangent run for every direction in one launch (blockIdx.y); REF as
@@ -1872,7 +1934,7 @@ int do_assemble(yc_ctx* c, double h, bool async = false) {
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
                            c->shear_rec, 4 * c->Nc);
-    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble2, dim3(c->nT), dim3(ASM2_THREADS), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     c->bj_of = nullptr;   // A is new
     if (c->precond == YC_PRECOND_BLOCK2) {
@@ -2244,8 +2306,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
     if (rc) { yc_destroy(c); return rc; }
     c->coop_ok = cgp_fits<false>(c) ? 1 : 0;
     c->coop_bj_ok = cgp_fits<true>(c) ? 1 : 0;
-    if (hipFuncSetAttribute((const void*)k_assemble1<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM1_LDS) != hipSuccess ||
-        hipFuncSetAttribute((const void*)k_assemble2<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM_LDS) != hipSuccess) {
+    if (hipFuncSetAttribute((const void*)k_assemble1<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ASM1_LDS) != hipSuccess) {
         yc_destroy(c);
         return fail(YC_ERR_HIP, "assembly LDS size not supported");
     }
@@ -2503,7 +2564,7 @@ int assemble_kind(yc_ctx* c, int kind) {
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
                            c->shear_rec, 4 * c->Nc);
-    hipLaunchKernelGGL(k_assemble2<double>, dim3(c->nT), dim3(NBLK), ASM_LDS, c->stream, c->cd, O, c->last_h);
+    hipLaunchKernelGGL(k_assemble2, dim3(c->nT), dim3(ASM2_THREADS), 0, c->stream, c->cd, O, c->last_h);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
