@@ -3,13 +3,19 @@
 // tests/asm_host.cpp.  Included after elements.hpp and assembly.hpp.
 //
 // T = double: the step's rows of A, F, b (src/PhysicsCloth.cpp:46-156,
-// src/TrainHelper.cpp:19-20), two threads per node (blocks of 2 x 64 threads): half 0
-// evaluates the warp-direction elements (handle, warp edges and bends), half 1 the
-// weft-direction ones and the faces.  The two halves write disjoint slots of the node's
-// rows except the L-L self block, which half 1 keeps apart; after a barrier half 0 adds
-// it and half 1's accumulators in, then does the friction and b.  T = dd: their
-// directional derivatives along the tangents held in C, into the d* outputs of O, one
-// thread per node (blocks of 64 threads).  ASM1_REF: the reference gradient's tangent run.
+// src/TrainHelper.cpp:19-20), four threads per node (blocks of 4 x 64 threads): group 0
+// evaluates the warp edges, group 1 the warp bends, group 2 the weft edges and the faces,
+// group 3 the weft bends.  Warp and weft elements write disjoint slots of the node's rows
+// except the L-L self block, which the weft groups keep apart.  The bend groups write the
+// slots as they go; the edge groups sum each edge's entries in registers and add them to
+// private planes (the node and its two neighbours along the direction), which they fold
+// into the slots after a barrier, so that an edge group and the bend group of its
+// direction never update a slot at the same time.  After another barrier group 0 adds the
+// other groups' accumulators and the weft self block in, then does the handle, the
+// friction and b.  The order of every sum is fixed, so the result does not depend on
+// timing.  T = dd: the directional derivatives along the tangents held in C, into the d*
+// outputs of O, one thread per node (blocks of 64 threads).  ASM1_REF: the reference
+// gradient's tangent run.
 #pragma once
 
 namespace {
@@ -19,7 +25,7 @@ template <class T, bool ASM1_REF>
 __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut& O, double h, int tile) {
     constexpr bool tan = !std::is_same<T, double>::value;
     const int nl = threadIdx.x & (TILE_N - 1);              // node lane in the tile
-    const int half = tan ? -1 : (int)(threadIdx.x >> 6);    // -1: every element (tangent run)
+    const int grp = tan ? -1 : (int)(threadIdx.x >> 6);     // -1: every element (tangent run)
     const int k = tile_node(C, tile, nl);
     const bool active = k >= 0;
     RowSinkT<T, ASM1_REF> S;
@@ -28,51 +34,56 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
     const int u = S.mu_, v = S.mv_;
     // value run: the node's 159 slots accumulate in LDS (the tile's slot planes, same
     // layout as in HBM) and go to HBM once at the end, instead of a global
-    // read-modify-write per contribution; behind them half 1's self block (9 planes) and
-    // accumulators (ASM1_ACC planes)
+    // read-modify-write per contribution; behind them the weft groups' self block (9
+    // planes) and the edge groups' private planes (2 x ASM1_PRV), which later carry the
+    // accumulators of groups 1-3 (3 x NACC planes)
     extern __shared__ double lds_rows[];
     double* gA = S.A;
     double* self_w = lds_rows + (size_t)NSLOT * TILE_N;
-    double* acc_w = self_w + (size_t)9 * TILE_N;
+    double* prv = self_w + (size_t)9 * TILE_N;
+    double* prv_g = prv + (size_t)(grp == 2 ? ASM1_PRV : 0) * TILE_N + nl;
     if constexpr (!tan) {
         S.A = lds_rows + nl;
-        for (int s = half; s < NSLOT; s += 2) S.A[(size_t)s * TILE_N] = 0.0;
-        if (half == 1) {
-            S.Aself = self_w + nl;
+        for (int s = grp; s < NSLOT; s += 4) S.A[(size_t)s * TILE_N] = 0.0;
+        if (grp >= 2) S.Aself = self_w + nl;
+        if (grp == 3)
             for (int i = 0; i < 9; ++i) S.Aself[(size_t)i * TILE_N] = 0.0;
-        }
+        if (grp == 0 || grp == 2)
+            for (int i = 0; i < ASM1_PRV; ++i) prv_g[(size_t)i * TILE_N] = 0.0;
         __syncthreads();
     } else {
         if (active && S.dA) for (int s = 0; s < NSLOT; ++s) S.dA[(size_t)s * TILE_N] = 0.0;
     }
-    const bool warp_side = half <= 0, weft_side = half != 0;
     if (active) {
-        // handle (src/PhysicsCloth.cpp:232-248)
-        if (warp_side && (C.flags[k] & 2)) handle_element<T>(C, S, k);
+        // handle (src/PhysicsCloth.cpp:232-248); the value run's group 0 adds it after the
+        // barrier below
+        if (tan && (C.flags[k] & 2)) handle_element<T>(C, S, k);
         // edges (list order: warp rows, then weft columns; src/Cloth.cpp:190-228).  One
         // call site in a loop that is not unrolled: the inlined element code is large, and
         // four or six copies of it would not fit the instruction cache.
         int ne_warp = C.Lg * (C.W - 1);
+        const int e0 = grp <= 0 ? 0 : 2, e1 = (grp < 0 || grp == 2) ? 4 : (grp == 0 ? 2 : 0);
     #pragma unroll 1
-        for (int ei = warp_side ? 0 : 2; ei < (weft_side ? 4 : 2); ++ei) {
+        for (int ei = e0; ei < e1; ++ei) {
             int eu, ev, ec, idx;
             if (ei == 0) { if (u == 0) continue; eu = u - 1; ev = v; ec = 0; idx = v * (C.W - 1) + u - 1; }
             else if (ei == 1) { if (u == C.W - 1) continue; eu = u; ev = v; ec = 0; idx = v * (C.W - 1) + u; }
             else if (ei == 2) { if (v == 0) continue; eu = u; ev = v - 1; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v - 1; }
             else { if (v == C.Lg - 1) continue; eu = u; ev = v; ec = 1; idx = ne_warp + u * (C.Lg - 1) + v; }
-            // value run: the edge's entries in the node's rows summed in registers, then into
-            // the slots (the tangent run has no registers to spare for the dual sums)
-            const int uo = eu == u && ev == v ? eu + (ec == 0) : eu, vo = eu == u && ev == v ? ev + (ec == 1) : ev;
+            // value run: the edge's entries in the node's rows summed in registers, then
+            // added to the private planes (the other end precedes the node for ei = 0, 2)
+            // (the tangent run has no registers to spare for the dual sums)
             if constexpr (!tan) S.edge_begin();
             edge_element<T>(C, S, eu, ev, ec, ec == 0 ? (v & 1) : (u & 1), omp_owner(C.n_edges, idx));
-            if constexpr (!tan) S.edge_end(uo + vo * C.W, S.inter, S.inter && interior(C, uo, vo), ec);
+            if constexpr (!tan) S.edge_end_planes(prv_g, TILE_N, (ei & 1) ? 2 : 1);
         }
         // bends centred on u-1, u, u+1 (warp) and v-1, v, v+1 (weft) (src/Cloth.cpp:230-270):
         // a bend centred d after the node holds it as n2 (role 0), centred on it as n0
         // (role 1), centred d before it as n1 (role 2)
         int nb_warp = C.Lg * (C.W - 2);
+        const int b0 = (grp < 0 || grp == 1) ? 0 : 3, b1 = (grp < 0 || grp == 3) ? 6 : (grp == 1 ? 3 : 0);
     #pragma unroll 1
-        for (int bi6 = warp_side ? 0 : 3; bi6 < (weft_side ? 6 : 3); ++bi6) {
+        for (int bi6 = b0; bi6 < b1; ++bi6) {
             int d = bi6 % 3 - 1, dir = bi6 / 3;
             int uc = dir == 0 ? u + d : u, vc = dir == 0 ? v : v + d;
             if (dir == 0 && (uc < 1 || uc > C.W - 2)) continue;
@@ -82,7 +93,7 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
             else bend_element<T>(C, S, uc, vc, dir, dir == 0 ? (v & 1) : (u & 1), omp_owner(C.n_bends, bi));
         }
         // faces of the up to four cells around the node (src/Cloth.cpp:272-286)
-        if (weft_side && (C.wind_density != 0.0 || C.wind_drag != 0.0)) {
+        if ((grp < 0 || grp == 2) && (C.wind_density != 0.0 || C.wind_drag != 0.0)) {
     #pragma unroll 1
             for (int fi = 0; fi < 8; ++fi) {
                 int cu = u - 1 + ((fi >> 2) & 1), cv = v - 1 + ((fi >> 1) & 1), q = fi & 1;
@@ -96,17 +107,43 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
         }
     }
     if constexpr (!tan) {
-        // half 1 hands its accumulators over; half 0 adds them and the self block
-        if (half == 1 && active) S.store_acc(acc_w + nl, TILE_N);
+        // the edge groups fold their private planes into the slots: the node's own column
+        // (E entries when it is interior) and its neighbours' along the direction (E
+        // entries when both are interior), the entries the edges' terms emit; group 2's
+        // L-L self block goes to the weft self planes
         __syncthreads();
-        if (half == 0 && active) {
-            S.add_acc(acc_w + nl, TILE_N);
+        if (active && (grp == 0 || grp == 2)) {
+            const int c = grp == 0 ? 0 : 1;
+            const int up = c == 0 ? u - 1 : u, vp = c == 0 ? v : v - 1, un = c == 0 ? u + 1 : u, vn = c == 0 ? v : v + 1;
+            const bool has_p = up >= 0 && vp >= 0, has_n = un < C.W && vn < C.Lg;
+            const int kcol[3] = {k, up + vp * C.W, un + vn * C.W};
+            const bool has[3] = {has_p || has_n, has_p, has_n};
+            const bool ee[3] = {S.inter, S.inter && interior(C, up, vp), S.inter && interior(C, un, vn)};
+#pragma unroll
+            for (int col = 0; col < 3; ++col) {
+                if (!has[col]) continue;
+                const double* P = prv_g + (size_t)col * 16 * TILE_N;
+                for (int rc = 0; rc < 3; ++rc)
+                    for (int cc = 0; cc < 3; ++cc) S.addA(rc, kcol[col], cc, T(P[(size_t)(rc * 3 + cc) * TILE_N]));
+                if (!ee[col]) continue;
+                for (int rc = 0; rc < 3; ++rc) S.addA(rc, kcol[col], 3 + c, T(P[(size_t)(9 + rc) * TILE_N]));
+                for (int cc = 0; cc < 3; ++cc) S.addA(3 + c, kcol[col], cc, T(P[(size_t)(12 + cc) * TILE_N]));
+                S.addA(3 + c, kcol[col], 3 + c, T(P[(size_t)15 * TILE_N]));
+            }
+            if (grp == 0 && (C.flags[k] & 2)) handle_element<T>(C, S, k);
+        }
+        // groups 1-3 hand their accumulators over; group 0 adds them and the self block
+        __syncthreads();
+        if (grp > 0 && active) S.store_acc(prv + (size_t)(grp - 1) * RowSinkT<T>::NACC * TILE_N + nl, TILE_N);
+        __syncthreads();
+        if (grp == 0 && active) {
+            for (int g = 0; g < 3; ++g) S.add_acc(prv + (size_t)g * RowSinkT<T>::NACC * TILE_N + nl, TILE_N);
             for (int rc = 0; rc < 3; ++rc)
                 for (int cc = 0; cc < 3; ++cc)
                     S.A[(size_t)(rc * 45 + cc) * TILE_N] += self_w[(size_t)(rc * 3 + cc) * TILE_N + nl];
         }
     }
-    if (active && half <= 0) {
+    if (active && grp <= 0) {
         // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
         T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
         if constexpr (!tan) if (O.fric_in) {
@@ -151,12 +188,12 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
         }
     }
     if constexpr (!tan) {
-        // the rows to HBM, both halves, once the friction's entries are in, with the NaN/Inf
+        // the rows to HBM, all four groups, once the friction's entries are in, with the NaN/Inf
         // screen of the slots the shear replay does not touch (it screens its own)
         __syncthreads();
         if (active) {
             bool bad = false;
-            for (int s = half; s < NSLOT; s += 2) {
+            for (int s = grp; s < NSLOT; s += 4) {
                 const double a = S.A[(size_t)s * TILE_N];
                 gA[(size_t)s * TILE_N] = a;
                 if (!isfinite(a)) bad = true;

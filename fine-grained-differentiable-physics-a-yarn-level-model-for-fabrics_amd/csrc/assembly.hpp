@@ -50,10 +50,14 @@ __device__ __forceinline__ int padidx(int Wp, int u, int v) { return (u + 2) + (
 constexpr bool USE_BEND_REC = false;
 
 // LDS of the first assembly kernel's value run: one tile's slot planes (81,408 bytes), then
-// the second half's L-L self block (9 planes) and accumulators (RowSinkT::NACC planes) --
-// 101,888 bytes, above the 64 KiB default, allowed per kernel at context creation
-constexpr size_t ASM1_LDS = (size_t)(NSLOT + 9 + RowSinkT<double>::NACC) * TILE_N * sizeof(double);
-constexpr int ASM1_THREADS = 2 * TILE_N;
+// the weft groups' L-L self block (9 planes) and the two edge groups' private planes
+// (ASM1_PRV each: 3 node columns x 16 entries), which afterwards hold the accumulators of
+// three groups (3 x RowSinkT::NACC planes) -- 135,168 bytes, above the 64 KiB default,
+// allowed per kernel at context creation
+constexpr int ASM1_PRV = 48;
+static_assert(3 * RowSinkT<double>::NACC <= 2 * ASM1_PRV, "accumulator planes");
+constexpr size_t ASM1_LDS = (size_t)(NSLOT + 9 + 2 * ASM1_PRV) * TILE_N * sizeof(double);
+constexpr int ASM1_THREADS = 4 * TILE_N;
 
 // Node of thread t in the assembly kernels: one block per 8 x 8 matrix tile, so that a
 // wavefront's read-modify-writes of one slot are 512 contiguous bytes.

@@ -7,6 +7,7 @@
 // instead of explicit 8x8 products) but keeps the reference's values, including its
 // non-derivative quirks; each function cites the reference lines it reproduces.
 #pragma once
+#include <type_traits>
 #include "vec3.hpp"
 
 #pragma clang fp contract(off)
@@ -212,9 +213,11 @@ struct RowSinkT {
     // While racc_on, the entries of A in this node's rows that an edge emits are summed in
     // registers, per column (0: the node itself, 1: the edge's other end) and entry (0-8 the
     // L-L block row-major, 9-11 L rows x E column, 12-14 E row x L columns, 15 E-E), and
-    // edge_end() adds each sum to its slot once, instead of one slot update per term.
+    // edge_end_planes() adds each sum once, instead of one slot update per term.
+    // (value run only: the tangent run has no registers to spare for the dual sums)
+    static constexpr bool kRacc = std::is_same<T, double>::value;
     bool racc_on;
-    T racc[2][16];
+    double racc[kRacc ? 2 : 1][kRacc ? 16 : 1];
 
     YS_DEV void init(const ClothDev* c, int k, double* a, double h, double* da = nullptr, bool det = false) {
         C = c; me = k; mu_ = k % c->W; mv_ = k / c->W; inter = interior(*c, mu_, mv_);
@@ -241,28 +244,30 @@ struct RowSinkT {
         if (s < 0) bad = 1;
         return s;
     }
-    YS_DEV void accA(int kcol, int i, T x) {
-        if (kcol == me) racc[0][i] += x;
-        else racc[1][i] += x;
+    // true when x went to the register sums (value run, inside an edge)
+    YS_DEV bool accA(int kcol, int i, const T& x) {
+        if constexpr (kRacc) {
+            if (racc_on) {
+                if (kcol == me) racc[0][i] += x;
+                else racc[1][i] += x;
+                return true;
+            }
+        }
+        return false;
     }
     YS_DEV void edge_begin() {
+        static_assert(kRacc, "value run only");
         racc_on = true;
-        for (int i = 0; i < 16; ++i) { racc[0][i] = T(0.0); racc[1][i] = T(0.0); }
+        for (int i = 0; i < 16; ++i) { racc[0][i] = 0.0; racc[1][i] = 0.0; }
     }
-    // the sums into the slots: the L-L blocks of both columns; the E entries of the node's
-    // own column when it is interior (e_me), of the other end's when both ends are (e_o),
-    // the entries the edge's terms emit under the same conditions
-    YS_DEV void edge_end(int kother, bool e_me, bool e_o, int c) {
+    // the sums added to private planes P (`stride` apart): 16 planes per node column,
+    // column 0 the node itself, other_col (1 or 2) the edge's other end
+    YS_DEV void edge_end_planes(double* P, int stride, int other_col) {
+        static_assert(kRacc, "value run only");
         racc_on = false;
-        for (int hc = 0; hc < 2; ++hc) {
-            const int col = hc == 0 ? me : kother;
-            for (int rc = 0; rc < 3; ++rc)
-                for (int cc = 0; cc < 3; ++cc) addA(rc, col, cc, racc[hc][rc * 3 + cc]);
-            if (hc == 0 ? e_me : e_o) {
-                for (int rc = 0; rc < 3; ++rc) addA(rc, col, 3 + c, racc[hc][9 + rc]);
-                for (int cc = 0; cc < 3; ++cc) addA(3 + c, col, cc, racc[hc][12 + cc]);
-                addA(3 + c, col, 3 + c, racc[hc][15]);
-            }
+        for (int i = 0; i < 16; ++i) {
+            P[(size_t)i * stride] += racc[0][i];
+            P[(size_t)(other_col * 16 + i) * stride] += racc[1][i];
         }
     }
     YS_DEV void addA(int rc, int kcol, int cc, T x) {
@@ -292,8 +297,7 @@ struct RowSinkT {
         if (kF == me) {
             for (int u = 0; u < 3; ++u)
                 for (int v = 0; v < 3; ++v) {
-                    if (racc_on) accA(kD, v * 3 + u, coef[kind] * B.m[u][v]);
-                    else addA(v, kD, u, coef[kind] * B.m[u][v]);
+                    if (!accA(kD, v * 3 + u, coef[kind] * B.m[u][v])) addA(v, kD, u, coef[kind] * B.m[u][v]);
                     if (kind == KIND_M) Mv[v] += blk(B.m[u][v]) * velL(kD, u);
                 }
         }
@@ -306,8 +310,7 @@ struct RowSinkT {
         T b[3] = {B.x, B.y, B.z};
         if (kL == me)
             for (int v = 0; v < 3; ++v) {
-                if (racc_on) accA(kE, 9 + v, coef[kind] * b[v]);
-                else addA(v, kE, 3 + c, coef[kind] * b[v]);
+                if (!accA(kE, 9 + v, coef[kind] * b[v])) addA(v, kE, 3 + c, coef[kind] * b[v]);
                 if (kind == KIND_M) add5(Mv, v, blk(b[v]) * velE(kE, c));
             }
         if (kind == KIND_KV && kE == me)
@@ -318,8 +321,7 @@ struct RowSinkT {
         T b[3] = {B.x, B.y, B.z};
         if (kE == me)
             for (int u = 0; u < 3; ++u) {
-                if (racc_on) accA(kL, 12 + u, coef[kind] * b[u]);
-                else addA(3 + c, kL, u, coef[kind] * b[u]);
+                if (!accA(kL, 12 + u, coef[kind] * b[u])) addA(3 + c, kL, u, coef[kind] * b[u]);
                 if (kind == KIND_M) add5(Mv, 3 + c, blk(b[u]) * velL(kL, u));
             }
         if (kind == KIND_KV && kL == me)
@@ -328,8 +330,7 @@ struct RowSinkT {
     // EE: A[3N+2e(kF)+c, 3N+2e(kD)+c] = val
     YS_DEV void EE(int kind, int kF, int kD, int c, T v) {
         if (kF == me) {
-            if (racc_on) accA(kD, 15, coef[kind] * v);
-            else addA(3 + c, kD, 3 + c, coef[kind] * v);
+            if (!accA(kD, 15, coef[kind] * v)) addA(3 + c, kD, 3 + c, coef[kind] * v);
             if (kind == KIND_M) add5(Mv, 3 + c, blk(v) * velE(kD, c));
             if (kD == me) {
                 if (kind == KIND_KX) add2(KxEE, c, v);
