@@ -369,7 +369,13 @@ def main():
         del x, y
     import yarnstep.warmstart as ws
     cs = ys.cloth_spec
-    spec, state, ext, _ = cs.workload(args.workload)
+    spec, state, ext, stable_steps = cs.workload(args.workload)
+    if args.warmup + args.steps > stable_steps:
+        # every timed run starts from rest; beyond this many steps the cloth's implicit step
+        # loses definiteness (compressed yarn segments), in the oracle as on the GPU
+        print("bench.py: the %s workload stays stable for %d steps from rest; --warmup + --steps must not exceed "
+              "that" % (args.workload, stable_steps), file=sys.stderr)
+        sys.exit(2)
     h = spec["h"]
     tol = 1e-8
     if args.tangent_only:

@@ -144,11 +144,17 @@ def workload(name):
         st = initial_state(spec)
         p = float(np.random.default_rng(2).uniform(0.0, 2.0))
         return spec, st, pressure_force(spec, st, p), 50
-    if name == "twill128":       # 128x128 blend twill with seeded wind, 500 steps
+    if name == "twill128":       # 128x128 blend twill with seeded wind (6.38 m/s), 140 steps
+        # The reference's stretch Jacobian (src/PhysicsCloth.cpp:473) gives a compressed
+        # segment the transverse stiffness ks (1/l - 1/D) with no clamp; at h = 1 ms it
+        # outweighs the segment's mass after a few percent of compression, so once the
+        # hanging cloth folds A stops being definite and the step diverges: near step 154
+        # with this wind, in the oracle and on the GPU alike (near step 100 without wind).
+        # The length returned is the span that stays stable from rest.
         spec = heter_spec(128, "twill", 4)
         rng = np.random.default_rng(40)
         spec["wind_vel"] = [0.0, float(rng.uniform(2.0, 8.0)), 0.0]
-        return spec, initial_state(spec), None, 500
+        return spec, initial_state(spec), None, 140
     raise KeyError(name)
 
 

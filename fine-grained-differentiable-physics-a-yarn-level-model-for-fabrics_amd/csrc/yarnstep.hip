@@ -1638,6 +1638,9 @@ struct yc_ctx {
     double* bj;                             // block-Jacobi rows, 5 P x 20 (when selected)
     const double* bj_of;                    // the matrix (A or At) bj was computed from, or null
     hipEvent_t io_event;                    // orders device state transfers with a caller's stream
+    hipStream_t side;                       // queued steps: the block-Jacobi rows beside the guess
+    hipEvent_t ev_side[2];                  // ... after the assembly / rows done
+    bool side_pending;                      // rows launched on `side`, not yet joined into `stream`
     hipGraphExec_t graph1;
     // forward-mode tangents of the step (directional derivatives)
     int ndir;
@@ -1830,11 +1833,17 @@ const void* cgp_kernel(int kp) {
 // The block-Jacobi rows of the matrix Aptr (k_bj_factor), unless they are already those
 // of Aptr: c->bj_of names the matrix the rows were computed from (the assembly computes
 // them for the new A; everything else that rewrites A or At clears it).
-int bj_factor(yc_ctx* c, const double* Aptr) {
-    if (c->bj_of == Aptr) return YC_OK;
+int bj_factor(yc_ctx* c, const double* Aptr, hipStream_t st = nullptr) {
+    if (c->bj_of == Aptr) {
+        if (c->side_pending) {   // rows computed on the side stream: the context's stream waits for them
+            HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side[1], 0));
+            c->side_pending = false;
+        }
+        return YC_OK;
+    }
     const int ngx = (c->W + 1) / 2, ngy = (c->Lg + 1) / 2, ng = ngx * ngy;
-    hipLaunchKernelGGL(k_bj_factor, dim3((ng + BJF_GROUPS - 1) / BJF_GROUPS), dim3(64), 0, c->stream, Aptr, c->bj, c->W, c->Lg,
-                       c->Wp, c->P, c->ntx, ngx, ng);
+    hipLaunchKernelGGL(k_bj_factor, dim3((ng + BJF_GROUPS - 1) / BJF_GROUPS), dim3(64), 0, st ? st : c->stream, Aptr, c->bj, c->W,
+                       c->Lg, c->Wp, c->P, c->ntx, ngx, ng);
     HIPCHK(hipGetLastError());
     c->bj_of = Aptr;
     return YC_OK;
@@ -1921,7 +1930,10 @@ void from_padded(const yc_ctx* c, const std::vector<double>& pad, double* glob) 
 }
 
 // async: no host synchronisation; the flags are sticky and read by the caller later
-int do_assemble(yc_ctx* c, double h, bool async = false) {
+// side: with the block-Jacobi preconditioner, the rows are computed on c->side after the
+// assembly (c->ev_side[1] marks them done) instead of in order on the context's stream;
+// the first bj_factor() call for them (the solve's) joins them into the context's stream.
+int do_assemble(yc_ctx* c, double h, bool async = false, bool side = false) {
     if (!async) HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
     c->last_h = h;
@@ -1938,8 +1950,16 @@ int do_assemble(yc_ctx* c, double h, bool async = false) {
     HIPCHK(hipGetLastError());
     c->bj_of = nullptr;   // A is new
     if (c->precond == YC_PRECOND_BLOCK2) {
-        int rc = bj_factor(c, c->A);
+        if (side) {
+            HIPCHK(hipEventRecord(c->ev_side[0], c->stream));
+            HIPCHK(hipStreamWaitEvent(c->side, c->ev_side[0], 0));
+        }
+        int rc = bj_factor(c, c->A, side ? c->side : nullptr);
         if (rc) return rc;
+        if (side) {
+            HIPCHK(hipEventRecord(c->ev_side[1], c->side));
+            c->side_pending = true;
+        }
     }
     hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
                        c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
@@ -2359,6 +2379,10 @@ void yc_destroy(yc_ctx* c) {
     if (c->tdev) (void)hipFree(c->tdev);
     if (c->tout) (void)hipFree(c->tout);
     if (c->io_event) (void)hipEventDestroy(c->io_event);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    for (hipEvent_t e : c->ev_side)
+        if (e) (void)hipEventDestroy(e);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2789,8 +2813,19 @@ int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, 
     HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync((char*)c->sc + offsetof(CGScal, total_iters), 0, sizeof(long long) + 2 * sizeof(int), c->stream));
     const bool x0 = m != nullptr || guess != YC_GUESS_ZERO;
+    // with the network and the block-Jacobi preconditioner, the preconditioner's rows (from
+    // A) and the network's guess (from A, b and the state) are independent: the rows go to a
+    // second stream of the same (default) priority, and the solve's launch waits for them.
+    // The next step's rows wait for its assembly, so nothing else runs beside the
+    // cooperative solve.  (The join costs about as much as a cheap guess saves, so the
+    // other guesses keep the rows in order.)
+    const bool side = m != nullptr && c->precond == YC_PRECOND_BLOCK2;
+    if (side && !c->side) {
+        HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        for (hipEvent_t& e : c->ev_side) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     for (int s = 0; s < steps; ++s) {
-        int rc = do_assemble(c, h, true);
+        int rc = do_assemble(c, h, true, side);
         if (!rc && m) rc = warm_guess(c, m);
         if (!rc && !m && guess != YC_GUESS_ZERO) {
             hipLaunchKernelGGL(k_guess_x0, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->vpad,
@@ -2799,7 +2834,11 @@ int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, 
         }
         if (!rc) rc = do_solve(c, nullptr, tol, maxit, nullptr, nullptr, x0, true);
         if (!rc) rc = do_update(c, h);
-        if (rc) return rc;
+        if (rc) {   // no side work outlives the run
+            if (c->side) (void)hipStreamSynchronize(c->side);
+            c->side_pending = false;
+            return rc;
+        }
     }
     HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
