@@ -321,7 +321,7 @@ def test_autograd_rollout_gradient_matches_fd(ys, cs, form):
     fd = (oracle_loss(theta0 + e * dirn) - oracle_loss(theta0 - e * dirn)) / (2 * e)
     an = float(g @ dirn)
     say("  %s: loss %.4e  directional derivative: autograd %.8e  oracle fd %.8e  rel %.2e" %
-        (form, float(loss), an, fd, abs(an - fd) / abs(fd)))
+        (form, float(loss.detach()), an, fd, abs(an - fd) / abs(fd)))
     assert abs(an - fd) <= 1e-5 * abs(fd)
 
 
