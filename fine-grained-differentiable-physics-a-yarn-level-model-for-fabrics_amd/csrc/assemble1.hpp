@@ -21,6 +21,39 @@
 namespace {
 using namespace ys;
 
+// Adds an edge group's private planes P (3 node columns x 16 entries, TILE_N apart) to the
+// node's slots A, for the yarn direction CD (0: warp, along u; 1: weft, along v): column 0
+// the node itself, 1 / 2 its neighbour before / after it along CD; has[col]: the column's
+// edge exists, ee[col]: its E entries exist.  The slots are slot_of's, as constants, so the
+// updates need no index arithmetic and do not wait on each other.  The weft group's L-L
+// self block goes to its own planes Aself.
+template <int CD>
+__device__ __forceinline__ void fold_edge_planes(double* A, double* Aself, const double* P, const bool (&has)[3],
+                                                 const bool (&ee)[3]) {
+    constexpr int OLL[3] = {0, CD == 0 ? 1 : 3, CD == 0 ? 2 : 4};   // L-L stencil offsets
+    constexpr int DLT[3] = {0, -1, 1};
+    constexpr int ER = CD == 0 ? SLOT_EU : SLOT_EV;                 // the direction's E row
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        if (!has[col]) continue;
+        const double* Pc = P + (size_t)col * 16 * TILE_N;
+#pragma unroll
+        for (int rc = 0; rc < 3; ++rc)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const double x = Pc[(size_t)(rc * 3 + cc) * TILE_N];
+                if (CD == 1 && col == 0) Aself[(size_t)(rc * 3 + cc) * TILE_N] += x;
+                else A[(size_t)(rc * 45 + OLL[col] * 3 + cc) * TILE_N] += x;
+            }
+        if (!ee[col]) continue;
+#pragma unroll
+        for (int rc = 0; rc < 3; ++rc) A[(size_t)(rc * 45 + SLOT_LE + 3 * CD + DLT[col] + 1) * TILE_N] += Pc[(size_t)(9 + rc) * TILE_N];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) A[(size_t)(ER + (DLT[col] + 1) * 3 + cc) * TILE_N] += Pc[(size_t)(12 + cc) * TILE_N];
+        A[(size_t)(ER + 10 + DLT[col]) * TILE_N] += Pc[(size_t)15 * TILE_N];
+    }
+}
+
 template <class T, bool ASM1_REF>
 __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut& O, double h, int tile) {
     constexpr bool tan = !std::is_same<T, double>::value;
@@ -116,19 +149,11 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
             const int c = grp == 0 ? 0 : 1;
             const int up = c == 0 ? u - 1 : u, vp = c == 0 ? v : v - 1, un = c == 0 ? u + 1 : u, vn = c == 0 ? v : v + 1;
             const bool has_p = up >= 0 && vp >= 0, has_n = un < C.W && vn < C.Lg;
-            const int kcol[3] = {k, up + vp * C.W, un + vn * C.W};
             const bool has[3] = {has_p || has_n, has_p, has_n};
             const bool ee[3] = {S.inter, S.inter && interior(C, up, vp), S.inter && interior(C, un, vn)};
-#pragma unroll
-            for (int col = 0; col < 3; ++col) {
-                if (!has[col]) continue;
-                const double* P = prv_g + (size_t)col * 16 * TILE_N;
-                for (int rc = 0; rc < 3; ++rc)
-                    for (int cc = 0; cc < 3; ++cc) S.addA(rc, kcol[col], cc, T(P[(size_t)(rc * 3 + cc) * TILE_N]));
-                if (!ee[col]) continue;
-                for (int rc = 0; rc < 3; ++rc) S.addA(rc, kcol[col], 3 + c, T(P[(size_t)(9 + rc) * TILE_N]));
-                for (int cc = 0; cc < 3; ++cc) S.addA(3 + c, kcol[col], cc, T(P[(size_t)(12 + cc) * TILE_N]));
-                S.addA(3 + c, kcol[col], 3 + c, T(P[(size_t)15 * TILE_N]));
+            if constexpr (!tan) {
+                if (c == 0) fold_edge_planes<0>(S.A, nullptr, prv_g, has, ee);
+                else fold_edge_planes<1>(S.A, S.Aself, prv_g, has, ee);
             }
             if (grp == 0 && (C.flags[k] & 2)) handle_element<T>(C, S, k);
         }
