@@ -35,6 +35,9 @@ struct AsmOut {
     // value run, when set: the friction's accumulated inputs of every node (FRIC_IN planes
     // of N, zero off the interior), which the adjoint reads back
     double* fric_in;
+    // value run, when set: the step's starting velocity in the padded 5-plane layout (x, y,
+    // z, u, v; zero u, v off the interior), written by the second kernel
+    double* vpad;
 };
 
 // The friction's inputs accumulated over a node's elements, as planes of fric_in and of

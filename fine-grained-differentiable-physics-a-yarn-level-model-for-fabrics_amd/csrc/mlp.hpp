@@ -470,16 +470,6 @@ __global__ void __launch_bounds__(256) k_guess_x0(const double* vpad, const doub
     x0[i] = x;
 }
 
-// dv of the last solve relative to the velocity it started from; the previous one moves
-// to dprev2
-__global__ void __launch_bounds__(256) k_dprev(const double* x, const double* vpad, int N5, double* dprev, double* dprev2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N5) {
-        dprev2[i] = dprev[i];
-        dprev[i] = x[i] - vpad[i];
-    }
-}
-
 }  // namespace
 
 struct yc_mlp {

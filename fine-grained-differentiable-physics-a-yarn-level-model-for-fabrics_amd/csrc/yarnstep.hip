@@ -287,6 +287,11 @@ __device__ __forceinline__ void assemble2_value(const ClothDev& C, const AsmOut&
             for (int cc = 0; cc < 3; ++cc) acc[seg_block_oi(j, jj) * 3 + cc] += coef * Bb[j & 1][jj * 3 + cc];
     }
     const int pk = padidx(O.Wp, u, v);
+    if (O.vpad) {   // the padded starting velocity (the solve's guesses and residuals read it)
+        O.vpad[(size_t)rc * O.P + pk] = C.v[(size_t)rc * C.N + k];
+        if (rc == 0)
+            for (int c = 0; c < 2; ++c) O.vpad[(size_t)(3 + c) * O.P + pk] = inter ? C.uvd[(size_t)c * C.N + k] : 0.0;
+    }
     O.F[rc * C.N + k] += fsum;
     const double bv = O.b[rc * O.P + pk] + h * fsum;
     O.b[rc * O.P + pk] = bv;
@@ -1398,8 +1403,21 @@ __device__ __forceinline__ void update_node(const StateDev& s, const double* sol
     }
 }
 
-__global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h) {
-    update_node(s, sol, is_slide, Wp, P, h, blockIdx.x * NBLK + threadIdx.x);
+// With dprev set, also the node's velocity change of this solve relative to the velocity it
+// started from (dprev = sol - vpad, padded layout); the previous change moves to dprev2.
+// Padding entries are never written: sol and vpad are zero there, so are dprev and dprev2.
+__global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h,
+                                                 const double* vpad = nullptr, double* dprev = nullptr, double* dprev2 = nullptr) {
+    const int k = blockIdx.x * NBLK + threadIdx.x;
+    if (dprev && k < s.N) {
+        const int pk = padidx(Wp, k % s.W, k / s.W);
+        for (int c = 0; c < 5; ++c) {
+            const size_t i = (size_t)c * P + pk;
+            dprev2[i] = dprev[i];
+            dprev[i] = sol[i] - vpad[i];
+        }
+    }
+    update_node(s, sol, is_slide, Wp, P, h, k);
 }
 
 // Cloth::NodeNorm (src/Cloth.cpp:440-463): smallest-eigenvalue eigenvector of the 3x3
@@ -1693,6 +1711,7 @@ AsmOut make_out(yc_ctx* c) {
     O.A = c->A; O.F = c->F; O.b = c->b; O.is_slide = c->is_slide; O.comp = c->comp; O.fepart = c->fepart;
     O.dA = O.dF = O.db = O.dis_slide = O.dcomp = nullptr;
     O.fric_in = c->fric_out;
+    O.vpad = c->vpad;
     O.dext = nullptr;
     O.detach = 0;
     O.shear_rec = c->shear_rec; O.nseg = 4 * c->Nc;
@@ -1961,8 +1980,6 @@ int do_assemble(yc_ctx* c, double h, bool async = false, bool side = false) {
             c->side_pending = true;
         }
     }
-    hipLaunchKernelGGL(k_pack_vel, dim3(grid_for(c->N)), dim3(BLK), 0, c->stream, (const double*)c->v, (const double*)c->uvd,
-                       c->W, c->Lg, c->N, c->Wp, c->P, c->vpad);
     if (async) {
         HIPCHK(hipGetLastError());
         c->assembled = true;
@@ -2051,10 +2068,8 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
 
 int do_update(yc_ctx* c, double h) {
     if (!c->solved) return fail(YC_ERR_ARG, "yc_update before yc_solve");
-    hipLaunchKernelGGL(k_dprev, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->cx, (const double*)c->vpad,
-                       5 * c->P, c->dprev, c->dprev2);
     hipLaunchKernelGGL(k_update, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), (const double*)c->cx,
-                       (const double*)c->is_slide, c->Wp, c->P, h);
+                       (const double*)c->is_slide, c->Wp, c->P, h, (const double*)c->vpad, c->dprev, c->dprev2);
     HIPCHK(hipGetLastError());
     int rc = refresh_geometry(c);
     c->assembled = false;
