@@ -10,10 +10,12 @@
 // slots as they go; the edge groups sum each edge's entries in registers and add them to
 // private planes (the node and its two neighbours along the direction), which they fold
 // into the slots after a barrier, so that an edge group and the bend group of its
-// direction never update a slot at the same time.  After another barrier group 0 adds the
-// other groups' accumulators and the weft self block in, then does the handle, the
-// friction and b.  The order of every sum is fixed, so the result does not depend on
-// timing.  T = dd: the directional derivatives along the tangents held in C, into the d*
+// direction never update a slot at the same time; group 0 adds the handle.  After another
+// barrier group 0 adds the other groups' accumulators and the weft self block in and does
+// the warp component of the friction and b of the L and E_u rows; group 2 adds the weft
+// groups' E_v sums, forms the compression sum in group 0's order and does the weft
+// component and b of the E_v row.  The order of every sum is fixed, so the result does not
+// depend on timing.  T = dd: the directional derivatives along the tangents held in C, into the d*
 // outputs of O, one thread per node (blocks of 64 threads).  ASM1_REF: the reference
 // gradient's tangent run.
 #pragma once
@@ -157,38 +159,64 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
             }
             if (grp == 0 && (C.flags[k] & 2)) handle_element<T>(C, S, k);
         }
-        // groups 1-3 hand their accumulators over; group 0 adds them and the self block
+        // groups 1-3 hand their accumulators over, group 0 its compression sum; group 0 adds
+        // them and the self block, group 2 the weft component's sums and the compression
+        // sum in group 0's order ((g0 + g1) + g2) + g3, so that both hold the same bits
         __syncthreads();
-        if (grp > 0 && active) S.store_acc(prv + (size_t)(grp - 1) * RowSinkT<T>::NACC * TILE_N + nl, TILE_N);
+        constexpr int NA = RowSinkT<T>::NACC;
+        double* sbcl0 = prv + (size_t)3 * NA * TILE_N + nl;   // group 0's SBC_L, 3 planes
+        if (grp > 0 && active) S.store_acc(prv + (size_t)(grp - 1) * NA * TILE_N + nl, TILE_N);
+        if (grp == 0 && active) {
+            sbcl0[0] = val(S.SBC_L.x); sbcl0[TILE_N] = val(S.SBC_L.y); sbcl0[2 * TILE_N] = val(S.SBC_L.z);
+        }
         __syncthreads();
         if (grp == 0 && active) {
-            for (int g = 0; g < 3; ++g) S.add_acc(prv + (size_t)g * RowSinkT<T>::NACC * TILE_N + nl, TILE_N);
+            for (int g = 0; g < 3; ++g) S.add_acc(prv + (size_t)g * NA * TILE_N + nl, TILE_N);
             for (int rc = 0; rc < 3; ++rc)
                 for (int cc = 0; cc < 3; ++cc)
                     S.A[(size_t)(rc * 45 + cc) * TILE_N] += self_w[(size_t)(rc * 3 + cc) * TILE_N + nl];
         }
-    }
-    if (active && grp <= 0) {
-        // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
-        T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
-        if constexpr (!tan) if (O.fric_in) {
-            const double fi[FRIC_IN] = {val(S.SBC_L.x), val(S.SBC_L.y), val(S.SBC_L.z),
-                                        val(S.SBC_LE[0].x), val(S.SBC_LE[0].y), val(S.SBC_LE[0].z),
-                                        val(S.SBC_LE[1].x), val(S.SBC_LE[1].y), val(S.SBC_LE[1].z),
-                                        val(S.F[3]), val(S.F[4]), val(S.FEpart[0]), val(S.FEpart[1]),
-                                        val(S.KxEE[0]), val(S.KxEE[1]), val(S.KvEE[0]), val(S.KvEE[1])};
-            for (int i = 0; i < FRIC_IN; ++i) O.fric_in[(size_t)i * C.N + k] = S.inter ? fi[i] : 0.0;
+        if (grp == 2 && active) {
+            auto sbcl = [&](const double* p) {
+                return V3<T>(T(p[0]), T(p[(size_t)TILE_N]), T(p[(size_t)2 * TILE_N]));
+            };
+            const double* p1 = prv + (size_t)RowSinkT<T>::ACC_SBCL * TILE_N + nl;                   // group 1's
+            const double* p3 = prv + (size_t)(2 * NA + RowSinkT<T>::ACC_SBCL) * TILE_N + nl;        // group 3's
+            S.SBC_L = ((sbcl(sbcl0) + sbcl(p1)) + S.SBC_L) + sbcl(p3);
+            S.add_acc_weft(prv + (size_t)2 * NA * TILE_N + nl, TILE_N);
         }
-        if (S.inter) comp = friction<T>(C, S, sl);
-        // b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:20, 63)
-        int pk = padidx(O.Wp, u, v);
-        int e = S.inter ? crossing(C, u, v) : 0;
-        if constexpr (!tan) {
-            for (int c = 0; c < 5; ++c) O.F[c * C.N + k] = val(S.F[c]);
-            O.comp[k] = val(comp);
-            O.is_slide[k] = val(sl[0]); O.is_slide[C.N + k] = val(sl[1]);
-            O.fepart[k] = val(S.FEpart[0]); O.fepart[C.N + k] = val(S.FEpart[1]);
-            for (int c = 0; c < 5; ++c) {
+        // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123): the warp
+        // component by group 0, the weft one by group 2; each writes its component's outputs.
+        // The component is a compile-time constant (a run-time index into the sink's
+        // arrays would put them in scratch memory).
+        auto fric_part = [&](auto CF) {
+            constexpr int cf = decltype(CF)::value;
+            T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
+            if (O.fric_in) {
+                const size_t N = C.N;
+                auto put = [&](int i, T x) { O.fric_in[(size_t)i * N + k] = S.inter ? val(x) : 0.0; };
+                if (cf == 0) {
+                    put(FI_SBCL, S.SBC_L.x); put(FI_SBCL + 1, S.SBC_L.y); put(FI_SBCL + 2, S.SBC_L.z);
+                }
+                put(FI_SBCLE + 3 * cf, S.SBC_LE[cf].x); put(FI_SBCLE + 3 * cf + 1, S.SBC_LE[cf].y);
+                put(FI_SBCLE + 3 * cf + 2, S.SBC_LE[cf].z);
+                put(FI_FNS + cf, S.F[3 + cf]); put(FI_FEP + cf, S.FEpart[cf]);
+                put(FI_KXEE + cf, S.KxEE[cf]); put(FI_KVEE + cf, S.KvEE[cf]);
+            }
+            if (S.inter) comp = friction<T>(C, S, sl, cf, cf + 1);
+            // b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:20, 63)
+            const int pk = padidx(O.Wp, u, v);
+            const int e = S.inter ? crossing(C, u, v) : 0;
+            if (cf == 0) {
+                O.comp[k] = val(comp);
+                for (int c = 0; c < 4; ++c) O.F[c * C.N + k] = val(S.F[c]);
+            } else {
+                O.F[4 * C.N + k] = val(S.F[4]);
+            }
+            O.is_slide[cf * C.N + k] = val(sl[cf]);
+            O.fepart[cf * C.N + k] = val(S.FEpart[cf]);
+#pragma unroll
+            for (int c = cf == 0 ? 0 : 4; c < (cf == 0 ? 4 : 5); ++c) {
                 double bv = 0.0;
                 if (c < 3 || S.inter) {
                     double ext = 0.0;
@@ -197,19 +225,27 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
                 }
                 O.b[c * O.P + pk] = bv;
             }
-        } else {
-            for (int c = 0; c < 5; ++c) O.dF[c * C.N + k] = tan_of(S.F[c]);
-            O.dcomp[k] = tan_of(comp);
-            O.dis_slide[k] = tan_of(sl[0]); O.dis_slide[C.N + k] = tan_of(sl[1]);
-            for (int c = 0; c < 5; ++c) {
-                double bv = 0.0;   // f_ext enters through its tangent, when one is set
-                if (c < 3 || S.inter) {
-                    double dext = 0.0;
-                    if (O.dext) dext = c < 3 ? O.dext[3 * k + c] : O.dext[3 * C.N + 2 * e + (c - 3)];
-                    bv = h * (tan_of(S.F[c]) + dext - tan_of(S.KvTv[c])) + tan_of(S.Mv[c]);
-                }
-                O.db[c * O.P + pk] = bv;
+        };
+        if (active && grp == 0) fric_part(std::integral_constant<int, 0>{});
+        if (active && grp == 2) fric_part(std::integral_constant<int, 1>{});
+    } else if (active) {
+        // friction at the node's own crossing (src/PhysicsCloth.cpp:114-123)
+        T comp = T(0.0), sl[2] = {T(0.0), T(0.0)};
+        if (S.inter) comp = friction<T>(C, S, sl);
+        // b = h (F + f_ext - K_v^T v) + M v (src/TrainHelper.cpp:20, 63)
+        int pk = padidx(O.Wp, u, v);
+        int e = S.inter ? crossing(C, u, v) : 0;
+        for (int c = 0; c < 5; ++c) O.dF[c * C.N + k] = tan_of(S.F[c]);
+        O.dcomp[k] = tan_of(comp);
+        O.dis_slide[k] = tan_of(sl[0]); O.dis_slide[C.N + k] = tan_of(sl[1]);
+        for (int c = 0; c < 5; ++c) {
+            double bv = 0.0;   // f_ext enters through its tangent, when one is set
+            if (c < 3 || S.inter) {
+                double dext = 0.0;
+                if (O.dext) dext = c < 3 ? O.dext[3 * k + c] : O.dext[3 * C.N + 2 * e + (c - 3)];
+                bv = h * (tan_of(S.F[c]) + dext - tan_of(S.KvTv[c])) + tan_of(S.Mv[c]);
             }
+            O.db[c * O.P + pk] = bv;
         }
     }
     if constexpr (!tan) {

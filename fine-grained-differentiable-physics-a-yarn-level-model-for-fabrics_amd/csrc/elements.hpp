@@ -339,8 +339,11 @@ struct RowSinkT {
         }
         if (kind == KIND_KV && kD == me) add5(KvTv, 3 + c, blk(v) * velE(kF, c));
     }
-    // the accumulators as NACC planes `stride` apart (value run): store, and add in
+    // the accumulators as NACC planes `stride` apart (value run): store, and add in.  Plane
+    // of F[c], M v[c], K_v^T v[c]: 3c, 3c + 1, 3c + 2; of KxEE[c], KvEE[c], FEpart[c]:
+    // 15 + 3c, 16 + 3c, 17 + 3c; SBC_L: 21-23; SBC_LE[c]: 24 + 3c ..; bad: 30.
     static constexpr int NACC = 31;
+    static constexpr int ACC_SBCL = 21;
     YS_DEV void store_acc(double* p, int stride) const {
         int i = 0;
         for (int c = 0; c < 5; ++c) { p[(size_t)(i++) * stride] = val(F[c]); p[(size_t)(i++) * stride] = val(Mv[c]); p[(size_t)(i++) * stride] = val(KvTv[c]); }
@@ -356,6 +359,13 @@ struct RowSinkT {
         SBC_L.x += p[(size_t)(i++) * stride]; SBC_L.y += p[(size_t)(i++) * stride]; SBC_L.z += p[(size_t)(i++) * stride];
         for (int c = 0; c < 2; ++c) { SBC_LE[c].x += p[(size_t)(i++) * stride]; SBC_LE[c].y += p[(size_t)(i++) * stride]; SBC_LE[c].z += p[(size_t)(i++) * stride]; }
         if (p[(size_t)i * stride] != 0.0) bad = 1;
+    }
+    // the weft component's sums (F, M v, K_v^T v of E_v; KxEE, KvEE, FEpart, SBC_LE of the
+    // weft) of another group's accumulator planes, for the weft friction (value run)
+    YS_DEV void add_acc_weft(const double* p, int stride) {
+        F[4] += p[(size_t)12 * stride]; Mv[4] += p[(size_t)13 * stride]; KvTv[4] += p[(size_t)14 * stride];
+        KxEE[1] += p[(size_t)18 * stride]; KvEE[1] += p[(size_t)19 * stride]; FEpart[1] += p[(size_t)20 * stride];
+        SBC_LE[1].x += p[(size_t)27 * stride]; SBC_LE[1].y += p[(size_t)28 * stride]; SBC_LE[1].z += p[(size_t)29 * stride];
     }
     YS_DEV void FL(int k, v3<T> f) { if (k == me) { F[0] += f.x; F[1] += f.y; F[2] += f.z; } }
     YS_DEV void addF(int i, T f) { add5(F, i, f); }   // a force row of the node itself (the friction)
@@ -831,15 +841,17 @@ YS_DEV void face_element(const ClothDev& C, Sink& S, int ka, int kb2, int kc) {
 // ConSlideFriction at the thread's own crossing (src/PhysicsCloth.cpp:2579-2641),
 // run after every edge / bend of the node has been accumulated.  Returns comp.
 // ---------------------------------------------------------------------------
+// Components c_lo .. c_hi - 1 (both by default; the first assembly kernel's value run
+// gives the warp and the weft component to two threads).
 template <class T, class Sink>
-YS_DEV T friction(const ClothDev& C, Sink& S, T* is_slide) {
+YS_DEV T friction(const ClothDev& C, Sink& S, T* is_slide, int c_lo = 0, int c_hi = 2) {
     typedef v3<T> V;
     int k = S.me;
     V n = plane3<T>(C.nrm, C.dnrm, C.N, k);
     T comp = 0.5 * dot(n, S.SBC_L);
     comp = comp > 0.0 ? comp : 0.0;
     T lim = comp * par<T>(C.mu, C.dmu);
-    for (int c = 0; c < 2; ++c) {
+    for (int c = c_lo; c < c_hi; ++c) {
         T dl = uvc<T>(C, k, c) - uvbarc<T>(C, k, c);
         T cEE = 0.5 * dot(n, S.SBC_LE[c]);
         T FNS = S.F[3 + c];          // F_E before friction (F_E_No_Slide)
