@@ -124,8 +124,19 @@ __device__ __forceinline__ void assemble1_block(const ClothDev& C, const AsmOut&
             if (dir == 0 && (uc < 1 || uc > C.W - 2)) continue;
             if (dir == 1 && (vc < 1 || vc > C.Lg - 2)) continue;
             int bi = dir == 0 ? v * (C.W - 2) + uc - 1 : nb_warp + u * (C.Lg - 2) + vc - 1;
+            // value run: the bend centred on the node leaves its crimp twin (compression sums
+            // only) to the edge group of its direction, which has less to do
             if constexpr (USE_BEND_REC && !tan) emit_bend_role(C, S, O.bend_rec, C.n_bends, bi, 1 - d, omp_owner(C.n_bends, bi));
-            else bend_element<T>(C, S, uc, vc, dir, dir == 0 ? (v & 1) : (u & 1), omp_owner(C.n_bends, bi));
+            else bend_element<T>(C, S, uc, vc, dir, dir == 0 ? (v & 1) : (u & 1), omp_owner(C.n_bends, bi), (!tan && d == 0) ? 1 : 3);
+        }
+        if constexpr (!tan) {
+            if (grp == 0 || grp == 2) {
+                const int dir = grp == 0 ? 0 : 1;
+                if (dir == 0 ? (u >= 1 && u <= C.W - 2) : (v >= 1 && v <= C.Lg - 2)) {
+                    const int bi = dir == 0 ? v * (C.W - 2) + u - 1 : nb_warp + u * (C.Lg - 2) + v - 1;
+                    bend_element<T>(C, S, u, v, dir, dir == 0 ? (v & 1) : (u & 1), omp_owner(C.n_bends, bi), 2);
+                }
+            }
         }
         // faces of the up to four cells around the node (src/Cloth.cpp:272-286)
         if ((grp < 0 || grp == 2) && (C.wind_density != 0.0 || C.wind_drag != 0.0)) {
