@@ -242,9 +242,20 @@ constexpr int ASM2_THREADS = 3 * TILE_N;
 // the shear force to F and b of its component and writes its Jacobi diagonal entry; thread
 // 0 the E rows' entries.  A[3F+v, 3D+u] = B[u][v] (src/MatrixMp.cpp:65-76): row component
 // rc takes B[u][rc], u = 0..2.
+// ROWS: the block takes 64 consecutive nodes of one grid row (block b: row b / ceil(W / 64),
+// nodes from 64 (b mod ceil(W / 64))) instead of an 8 x 8 tile, so that a wavefront's reads of
+// a record field (crossing-major) are one contiguous run; the slots of A (tile-major) are the
+// smaller stream and take the strided side.  The arithmetic of each node is the same.
+template <bool ROWS = false>
 __device__ __forceinline__ void assemble2_value(const ClothDev& C, const AsmOut& O, double h, int tile) {
     const int nl = threadIdx.x & (TILE_N - 1), rc = threadIdx.x / TILE_N;
-    const int k = tile_node(C, tile, nl);
+    int k;
+    if constexpr (ROWS) {
+        const int rpb = (C.W + TILE_N - 1) / TILE_N, row = tile / rpb, uu = (tile - row * rpb) * TILE_N + nl;
+        k = uu < C.W ? uu + row * C.W : -1;
+    } else {
+        k = tile_node(C, tile, nl);
+    }
     if (k < 0) return;
     const int u = k % C.W, v = k / C.W;
     const bool inter = interior(C, u, v);
@@ -376,7 +387,7 @@ __device__ __forceinline__ void assemble2_tile(const ClothDev& C, const AsmOut& 
 }
 
 __global__ void __launch_bounds__(ASM2_THREADS) k_assemble2(ClothDev C, AsmOut O, double h) {
-    assemble2_value(C, O, h, blockIdx.x);
+    assemble2_value<true>(C, O, h, blockIdx.x);
 }
 
 // The second kernel's tangent run for every direction in one launch (blockIdx.y); REF as
@@ -1965,7 +1976,7 @@ int do_assemble(yc_ctx* c, double h, bool async = false, bool side = false) {
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
                            c->shear_rec, 4 * c->Nc);
-    hipLaunchKernelGGL(k_assemble2, dim3(c->nT), dim3(ASM2_THREADS), 0, c->stream, c->cd, make_out(c), h);
+    hipLaunchKernelGGL(k_assemble2, dim3(c->Lg * ((c->W + TILE_N - 1) / TILE_N)), dim3(ASM2_THREADS), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     c->bj_of = nullptr;   // A is new
     if (c->precond == YC_PRECOND_BLOCK2) {
@@ -2603,7 +2614,7 @@ int assemble_kind(yc_ctx* c, int kind) {
     if (c->Nc > 0)
         hipLaunchKernelGGL(k_shear_elem, dim3((4 * c->Nc + 255) / 256), dim3(256), 0, c->stream, c->cd, (const double*)c->comp,
                            c->shear_rec, 4 * c->Nc);
-    hipLaunchKernelGGL(k_assemble2, dim3(c->nT), dim3(ASM2_THREADS), 0, c->stream, c->cd, O, c->last_h);
+    hipLaunchKernelGGL(k_assemble2, dim3(c->Lg * ((c->W + TILE_N - 1) / TILE_N)), dim3(ASM2_THREADS), 0, c->stream, c->cd, O, c->last_h);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
