@@ -438,7 +438,7 @@ def main():
     spmv_bytes = cloth.spmv_bytes()
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     # the dominant kernel: one persistent-CG solve per step (k_cgp), timed with an event
-    # pair around each cooperative launch; its algorithmic bytes are the solve's
+    # pair around each solve's launch; its algorithmic bytes are the solve's
     # iterations times one PCG iteration streamed from memory
     dominant = None
     if args.cg == "persistent":

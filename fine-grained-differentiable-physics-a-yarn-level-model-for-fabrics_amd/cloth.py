@@ -414,7 +414,7 @@ class Cloth:
 
     # CG variants: "reference" (the reference solver's three-kernel order), "fused" (the
     # same recurrences with one kernel and one reduction per iteration) or "persistent"
-    # (the fused recurrences in one cooperative launch per solve)
+    # (the fused recurrences in one launch per solve)
     CG_VARIANTS = {"reference": 0, "fused": 1, "persistent": 2}
 
     def set_cg_variant(self, name):

@@ -46,6 +46,7 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr int BLK = 256;
+constexpr int MAX_DEVICES = 64;   // device indices of the per-device tables
 
 // Mean device time per launch of `reps` back-to-back launches issued by `launch` on
 // `stream` (after one untimed launch), from one event pair around the whole run.  An
@@ -894,8 +895,8 @@ __global__ void __launch_bounds__(BLK) k_cg1(Cg1Args a, int init, double tol, in
 }
 
 // ---------------------------------------------------------------------------
-// Persistent form of the single-reduction CG: one cooperative launch runs the whole
-// solve.  Each block owns one 8 x 8 tile for all iterations and keeps the tile's rows
+// Persistent form of the single-reduction CG: one launch runs the whole solve, its
+// blocks all resident at once (launch_cgp).  Each block owns one 8 x 8 tile for all iterations and keeps the tile's rows
 // of A (the 20 MB that every per-iteration kernel would otherwise stream from memory
 // again), D^-1, x and p in registers.  Blocks exchange only r, w, s on the tile
 // halos and their six partial sums per iteration.
@@ -1830,11 +1831,12 @@ Cg1Args make_cg1(yc_ctx* c) {
     return a;
 }
 
-// The persistent CG needs every tile's block resident at once (cooperative launch).
+// The persistent CG needs every tile's block resident at once: the grid must fit the
+// device's CUs at the kernel's occupancy.
 template <int KP, bool BJ>
 bool cgp_fits_kp(const yc_ctx* c) {
-    int dev = c->device, coop = 0, ncu = 0, per_cu = 0;
-    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return false;
+    int dev = c->device, ncu = 0, per_cu = 0;
+    if (dev < 0 || dev >= MAX_DEVICES) return false;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cgp<KP, BJ>, BLK, 0) != hipSuccess) return false;
     return (long long)per_cu * ncu >= c->nT;
@@ -1879,6 +1881,40 @@ int bj_factor(yc_ctx* c, const double* Aptr, hipStream_t st = nullptr) {
     return YC_OK;
 }
 
+// Persistent solves launch as ordinary kernels.  Their blocks wait on one another, so all
+// of a solve's blocks must be resident at once: the grid fits the device at the kernel's
+// occupancy (cgp_fits), and nothing else of this library that waits across blocks may hold
+// CUs meanwhile.  (A cooperative launch would also guarantee residency, but the runtime
+// runs it on a separate hardware queue behind a cross-queue wait on each side, about 30 us
+// per step at 128 x 128 crossings.)  Two persistent solves in flight together could each
+// hold part of the device and wait for the other until the spin limit, so the solves of all
+// contexts of the process run one at a time per device: a solve queued on a stream other
+// than the previous solve's first waits for everything queued on that stream so far.
+// Kernels of other libraries or processes do not wait on these blocks and finish by
+// themselves; they can only delay a solve.
+struct PersistOrder {
+    std::mutex mu;
+    hipStream_t last[MAX_DEVICES] = {};
+    hipEvent_t ev[MAX_DEVICES] = {};
+};
+PersistOrder& persist_order() {
+    static PersistOrder p;
+    return p;
+}
+
+// Orders a persistent solve about to be queued on c->stream after the previous one of the
+// device; call with the launch under the same lock.
+int persist_order_locked(yc_ctx* c, PersistOrder& po) {
+    const int dev = c->device;
+    if (po.last[dev] && po.last[dev] != c->stream) {
+        if (!po.ev[dev]) HIPCHK(hipEventCreateWithFlags(&po.ev[dev], hipEventDisableTiming));
+        HIPCHK(hipEventRecord(po.ev[dev], po.last[dev]));
+        HIPCHK(hipStreamWaitEvent(c->stream, po.ev[dev], 0));
+    }
+    po.last[dev] = c->stream;
+    return YC_OK;
+}
+
 // With the block-Jacobi preconditioner the rows of the matrix solved are made current first.
 int launch_cgp(yc_ctx* c, double tol, int maxit, const double* Aptr = nullptr) {
     CgpArgs a;
@@ -1899,7 +1935,11 @@ int launch_cgp(yc_ctx* c, double tol, int maxit, const double* Aptr = nullptr) {
     a.W = c->W; a.Lg = c->Lg; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx; a.nT = c->nT;
     void* args[] = {&a};
     const void* fn = bj ? cgp_kernel<true>(c->cg1_kp) : cgp_kernel<false>(c->cg1_kp);
-    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(c->nT), dim3(BLK), args, 0, c->stream));
+    PersistOrder& po = persist_order();
+    std::lock_guard<std::mutex> lock(po.mu);
+    int rc = persist_order_locked(c, po);
+    if (rc) return rc;
+    HIPCHK(hipLaunchKernel(fn, dim3(c->nT), dim3(BLK), args, 0, c->stream));
     return YC_OK;
 }
 
@@ -2391,6 +2431,11 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
 void yc_destroy(yc_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->device >= 0 && c->device < MAX_DEVICES) {   // the stream stops being the persistent solves' last
+        PersistOrder& po = persist_order();
+        std::lock_guard<std::mutex> lock(po.mu);
+        if (po.last[c->device] == c->stream) po.last[c->device] = nullptr;
+    }
     adjoint_free(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
@@ -2843,7 +2888,7 @@ int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, 
     // A) and the network's guess (from A, b and the state) are independent: the rows go to a
     // second stream of the same (default) priority, and the solve's launch waits for them.
     // The next step's rows wait for its assembly, so nothing else runs beside the
-    // cooperative solve.  (The join costs about as much as a cheap guess saves, so the
+    // persistent solve.  (The join costs about as much as a cheap guess saves, so the
     // other guesses keep the rows in order.)
     const bool side = m != nullptr && c->precond == YC_PRECOND_BLOCK2;
     if (side && !c->side) {
@@ -3394,7 +3439,6 @@ static __device__ __align__(256) float scratch[MLP_NOALLOC_FLOATS];
 
 namespace {
 
-constexpr int MAX_DEVICES = 64;
 std::mutex g_scratch_mu;
 float* g_scratch_dev[MAX_DEVICES];
 

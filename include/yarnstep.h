@@ -172,9 +172,15 @@ int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_s
  *   agree with the reference form to the solve tolerance; counts may differ by rounding. */
 #define YC_CG_REFERENCE 0
 #define YC_CG_FUSED 1
-/* YC_CG_PERSISTENT: the fused recurrences in one cooperative launch per solve; each
- * 8 x 8 tile's block keeps its rows of A in registers for the whole solve.  Needs every
- * tile resident at once (YC_ERR_ARG otherwise, e.g. above about 128 x 128 crossings). */
+/* YC_CG_PERSISTENT: the fused recurrences in one launch per solve; each 8 x 8 tile's
+ * block keeps its rows of A in registers for the whole solve.  Needs every tile resident
+ * at once (YC_ERR_ARG otherwise, e.g. above about 128 x 128 crossings).  The blocks wait
+ * on one another, so the persistent solves of all contexts of a process run one at a time
+ * per device (a solve queued on another context's stream waits for the work queued there
+ * so far).  A batch's solves (yc_batch_*) are not part of that order: do not queue them
+ * beside a single-cloth persistent solve on the same device from another thread.  Kernels
+ * of other libraries or processes can only delay a solve; one delayed past the spin limit
+ * (about a quarter second) fails with YC_ERR_HIP. */
 #define YC_CG_PERSISTENT 2
 int yc_set_cg_variant(yc_ctx* ctx, int variant);
 
@@ -225,7 +231,7 @@ int yc_time_spmv(yc_ctx* ctx, int reps, double* avg_ms);
  * plus 16 bytes per DOF (p read, A p written). */
 double yc_spmv_bytes(const yc_ctx* ctx);
 /* Times `reps` persistent-CG solves of the assembled system (YC_CG_PERSISTENT only) with
- * an event pair around each cooperative launch; avg_ms the mean launch duration, iters
+ * an event pair around each solve's launch; avg_ms the mean launch duration, iters
  * the iteration count of the solve.  Overwrites the CG workspace and the solution. */
 int yc_time_solve(yc_ctx* ctx, double tol, int maxit, int reps, double* avg_ms, int* iters);
 /* Algorithmic bytes of one Jacobi-PCG iteration streamed from memory: 8 per structural

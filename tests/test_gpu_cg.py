@@ -155,3 +155,52 @@ def test_fused_heter64_steps_match_reference_variant(ys, cs):
         say("  iters ref %s %s %s |dstate| %.2e" % (ia, v, ib, e))
         assert max(abs(a - b) for a, b in zip(ia, ib)) <= 1
         assert e < 1e-7
+
+
+def test_persistent_solves_from_two_threads(ys, cs):
+    """Two contexts stepping the 128 x 128 twill from two host threads queue persistent
+    solves at the same time.  Each solve's 256 tile blocks take the whole device, and the
+    blocks of a solve wait on one another, so the library runs the solves of different
+    contexts one at a time.  Both threaded runs must return OK with the trajectory and
+    iteration count of the same run alone, bit for bit."""
+    import threading
+    spec, state, ext, _ = cs.workload("twill128")
+    h, steps = spec["h"], 12
+
+    def make():
+        c = ys.Cloth(spec)
+        c.set_cg_variant("persistent")
+        c.set_preconditioner("block2")
+        c.set_ClothState(state)
+        return c
+
+    runs = {"zero": lambda c: c.run(h, steps, tol=1e-8), "quad": lambda c: c.run_guess("quad", h, steps, tol=1e-8)}
+    alone = {}
+    for k, fn in runs.items():
+        c = make()
+        alone[k] = (fn(c), c.get_ClothState())
+        c.close()
+    ctx = {k: make() for k in runs}
+    got = {}
+
+    def go(k):
+        try:
+            got[k] = runs[k](ctx[k])
+        except Exception as e:   # reported below, in the main thread
+            got[k] = e
+
+    threads = [threading.Thread(target=go, args=(k,)) for k in runs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in threads)
+    for k in runs:
+        assert not isinstance(got[k], Exception), got[k]
+        (it, rc), s = got[k], ctx[k].get_ClothState()
+        (it0, rc0), s0 = alone[k]
+        say("  %s: threaded iters %d rc %d, alone iters %d rc %d" % (k, it, rc, it0, rc0))
+        assert rc == 0 and rc0 == 0 and it == it0
+        for f in s0:
+            assert np.array_equal(s[f], s0[f]), (k, f)
+        ctx[k].close()
