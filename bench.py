@@ -274,7 +274,7 @@ def gradient_measure(ys, steps, precond):
     ok = sum(r[2] for r in res)
     grads["batched_adjoint"] = np.array([r[1] for r in res])
     out["batched_adjoint"] = {"form": "reverse mode of all 64 cloths together: one launch per phase over all cloths, the "
-                                      "solves (forward and backward) one cooperative launch, Jacobi preconditioner",
+                                      "solves (forward and backward) one launch, Jacobi preconditioner",
                               "cloth_steps": ok, "cloth_gradient_steps_per_s": ok / dt, "ms_per_cloth_step": 1e3 * dt / max(ok, 1),
                               "s_per_epoch": dt}
     out["batched_adjoint_over_tangents"] = out["batched_adjoint"]["cloth_gradient_steps_per_s"] / \

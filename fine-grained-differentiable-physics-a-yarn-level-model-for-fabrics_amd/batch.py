@@ -1,7 +1,7 @@
 """Many cloths of uneven size stepped together, and the launch-shape autotuner.
 
 ``ClothBatch`` wraps yc_batch_*: each phase of a step is one launch over every cloth
-(the persistent CG one cooperative launch per group of cloths that fit on the device at
+(the persistent CG one launch whose blocks walk groups of cloths that fit on the device at
 once), replayed from a hipGraph.  Each cloth's result is bit-identical to stepping it
 alone with ``Cloth.run`` and the persistent CG.  The reference has no batching: its
 training drivers loop over cloths one SimStep at a time (src/TrainHeter.h:81-146).
@@ -145,7 +145,7 @@ def _store_cache(path, key, value):
 def autotune_batch(batch, h, steps=3, tol=1e-8, maxit=None, cache=None, candidates=None):
     """Times every launch option of `batch` on its own cloths and keeps the fastest.
     Options: graph replay on/off, layout order (given / by decreasing size: decides
-    which cloths share a cooperative solve launch) and the per-node block size.  Each
+    which cloths share a resident group of the solve launch) and the per-node block size.  Each
     candidate runs one untimed step (graph capture) and `steps` timed steps from the
     same saved state; the state is restored at the end.  Returns (best options,
     {option tuple: seconds per step})."""

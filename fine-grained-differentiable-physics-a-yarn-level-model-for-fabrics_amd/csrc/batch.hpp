@@ -6,11 +6,11 @@
 // as the single-cloth kernel on that cloth's buffers, so every cloth's arithmetic is the
 // single-cloth path's, bit for bit.  A step is
 //   k_b_epoch, k_b_assemble1, k_b_shear, k_b_assemble2, k_b_cg_init,
-//   k_b_cgp (one cooperative launch; its blocks walk groups of cloths that fit on the
-//   device at once, block b taking tile b of each group in turn),
+//   k_b_cgp (one launch whose blocks are all resident; they walk groups of cloths that
+//   fit on the device at once, block b taking tile b of each group in turn),
 //   k_b_update, k_b_normals
 // with no host synchronisation; a run replays it from a hipGraph where the runtime
-// captures the cooperative launch, and launches it directly otherwise.  The reference
+// captures the step, and launches it directly otherwise.  The reference
 // steps one Cloth per PhysicsCloth (src/TrainHelper.cpp:15-97) and its training drivers
 // loop over cloths (src/TrainHeter.h:81-146); batching has no counterpart there.
 
@@ -84,7 +84,9 @@ __global__ void __launch_bounds__(BLK) k_b_cg_init(const BatchItem* __restrict__
     cg_init_block(b.va, b.part, 0, blockIdx.x - b.v0);
 }
 
-// The solves of all cloths in one cooperative launch.  Group g holds tiles
+// The solves of all cloths in one launch, an ordinary one whose grid (the largest group)
+// fits the device at the kernel's occupancy, so all of its blocks are resident (as for
+// k_cgp, a cooperative launch would add a cross-queue wait on each side).  Group g holds tiles
 // [grp[2g], grp[2g] + grp[2g+1]) of the tile numbering; block b solves tile b of group 0,
 // then tile b of group 1, and so on.  A cloth's tiles are all in one group, so its
 // blocks are distinct and resident; a block that reaches a cloth early waits (bounded)
@@ -172,7 +174,7 @@ int batch_free_layout(yc_batch* b) {
     return YC_OK;
 }
 
-// Block offsets, block -> cloth maps and cooperative groups for the current options.
+// Block offsets, block -> cloth maps and resident groups for the current options.
 int batch_layout(yc_batch* b) {
     batch_free_layout(b);
     const int n = (int)b->m.size();
@@ -183,11 +185,9 @@ int batch_layout(yc_batch* b) {
         std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return b->m[x]->nT > b->m[y]->nT; });
     b->kp = 1;
     for (yc_ctx* c : b->m) b->kp = std::max(b->kp, c->cg1_kp);
-    int per_cu = 0, ncu = 0, coop = 0;
-    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, b->device));
+    int per_cu = 0, ncu = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b_cgp_fn(b->kp, b->bj), BLK, 0));
-    if (!coop) return fail(YC_ERR_HIP, "batch: the device has no cooperative launch");
     b->capacity = per_cu * ncu;
     std::vector<int> mt, ms, mv, mn;
     b->items.assign(n, BatchItem());
@@ -314,7 +314,15 @@ int batch_step(yc_batch* b, double h) {
         const int* gp = b->d_groups;
         int ng = (int)b->groups.size();
         void* args[] = {(void*)&it, (void*)&mp, (void*)&gp, (void*)&ng};
-        HIPCHK(hipLaunchCooperativeKernel(b_cgp_fn(b->kp, b->bj), dim3(b->max_group), dim3(BLK), args, 0, b->stream));
+        auto launch = [&]() -> int {
+            HIPCHK(hipLaunchKernel(b_cgp_fn(b->kp, b->bj), dim3(b->max_group), dim3(BLK), args, 0, b->stream));
+            return YC_OK;
+        };
+        // while a step is captured into a graph the order is kept when the graph is launched
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(b->stream, &cap));
+        int rc2 = cap == hipStreamCaptureStatusNone ? persist_launch(b->device, b->stream, launch) : launch();
+        if (rc2) return rc2;
     }
     return batch_node_phase(b, h, 1);
 }
@@ -388,6 +396,7 @@ int yc_batch_create(yc_ctx* const* ctxs, int n, yc_batch** out) {
 void yc_batch_destroy(yc_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    persist_forget(b->device, b->stream);
     batch_free_layout(b);
     if (b->d_status) (void)hipFree(b->d_status);
     if (b->h_status) (void)hipHostFree(b->h_status);
@@ -455,7 +464,13 @@ int yc_batch_run(yc_batch* b, double h, int steps, double tol, int maxit, long l
         b->graph_used = b->graph != nullptr;
     }
     for (int s = 0; s < steps; ++s) {
-        if (b->graph_used) HIPCHK(hipGraphLaunch(b->graph, b->stream));
+        if (b->graph_used) {
+            rc = persist_launch(b->device, b->stream, [&]() -> int {
+                HIPCHK(hipGraphLaunch(b->graph, b->stream));
+                return YC_OK;
+            });
+            if (rc) return rc;
+        }
         else if ((rc = batch_step(b, h))) return rc;
     }
     hipLaunchKernelGGL(k_b_status, dim3((n + 63) / 64), dim3(64), 0, b->stream, (const BatchItem*)b->d_items, n, b->d_status);

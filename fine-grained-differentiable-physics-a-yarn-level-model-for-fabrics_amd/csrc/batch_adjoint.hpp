@@ -6,7 +6,7 @@
 // backward step (normals' adjoint, state reload, the system's recomputation, the update's
 // adjoint, the backward solve, the assembly's vector-Jacobian product, the loss gradient)
 // is one launch over all cloths, through the batch's block -> cloth maps, and the solves
-// are the batch's one cooperative launch.  Each cloth's arithmetic is its single-cloth
+// are the batch's one solve launch (k_b_cgp).  Each cloth's arithmetic is its single-cloth
 // backward pass's (the same device bodies on its own buffers).  The reference trains one
 // cloth per forward/backward pass in its drivers (src/TrainHeter.h:161-231).
 
@@ -366,10 +366,12 @@ int yc_batch_adjoint_backward(yc_batch* b, int mode, double h, double tol, int m
                 const int* gp = b->d_groups;
                 int ng = (int)b->groups.size();
                 void* args[] = {(void*)&it2, (void*)&mt, (void*)&gp, (void*)&ng};
-                if (hipLaunchCooperativeKernel(b_cgp_fn(b->kp, b->bj), dim3(b->max_group), dim3(BLK), args, 0, b->stream) != hipSuccess) {
-                    rc = fail(YC_ERR_HIP, "batch adjoint: cooperative launch");
-                    break;
-                }
+                rc = persist_launch(b->device, b->stream, [&]() -> int {
+                    if (hipLaunchKernel(b_cgp_fn(b->kp, b->bj), dim3(b->max_group), dim3(BLK), args, 0, b->stream) != hipSuccess)
+                        return fail(YC_ERR_HIP, "batch adjoint: solve launch");
+                    return YC_OK;
+                });
+                if (rc) break;
             }
             nodes(k_ba_keep_lam);
             if (exact) {

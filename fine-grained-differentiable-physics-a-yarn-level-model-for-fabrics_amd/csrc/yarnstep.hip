@@ -1881,15 +1881,16 @@ int bj_factor(yc_ctx* c, const double* Aptr, hipStream_t st = nullptr) {
     return YC_OK;
 }
 
-// Persistent solves launch as ordinary kernels.  Their blocks wait on one another, so all
-// of a solve's blocks must be resident at once: the grid fits the device at the kernel's
-// occupancy (cgp_fits), and nothing else of this library that waits across blocks may hold
-// CUs meanwhile.  (A cooperative launch would also guarantee residency, but the runtime
+// Persistent solves (k_cgp here, k_b_cgp in batches) launch as ordinary kernels.  Their
+// blocks wait on one another, so all of a solve's blocks must be resident at once: the grid
+// fits the device at the kernel's occupancy (cgp_fits), and nothing else of this library
+// that waits across blocks may hold CUs meanwhile.  (A cooperative launch would also guarantee residency, but the runtime
 // runs it on a separate hardware queue behind a cross-queue wait on each side, about 30 us
 // per step at 128 x 128 crossings.)  Two persistent solves in flight together could each
 // hold part of the device and wait for the other until the spin limit, so the solves of all
-// contexts of the process run one at a time per device: a solve queued on a stream other
-// than the previous solve's first waits for everything queued on that stream so far.
+// contexts and batches of the process run one at a time per device: a solve queued on a
+// stream other than the previous solve's first waits for everything queued on that stream
+// so far.
 // Kernels of other libraries or processes do not wait on these blocks and finish by
 // themselves; they can only delay a solve.
 struct PersistOrder {
@@ -1902,17 +1903,29 @@ PersistOrder& persist_order() {
     return p;
 }
 
-// Orders a persistent solve about to be queued on c->stream after the previous one of the
-// device; call with the launch under the same lock.
-int persist_order_locked(yc_ctx* c, PersistOrder& po) {
-    const int dev = c->device;
-    if (po.last[dev] && po.last[dev] != c->stream) {
+// Queues `launch` (a persistent solve, or a graph that holds one) on stream st of device
+// dev after the device's previous persistent solve.  The launch runs under the lock, so
+// the order of the records is the order of the launches.  Not for a capturing stream.
+template <class F>
+int persist_launch(int dev, hipStream_t st, F&& launch) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(YC_ERR_ARG, "device index beyond the persistent-solve table");
+    PersistOrder& po = persist_order();
+    std::lock_guard<std::mutex> lock(po.mu);
+    if (po.last[dev] && po.last[dev] != st) {
         if (!po.ev[dev]) HIPCHK(hipEventCreateWithFlags(&po.ev[dev], hipEventDisableTiming));
         HIPCHK(hipEventRecord(po.ev[dev], po.last[dev]));
-        HIPCHK(hipStreamWaitEvent(c->stream, po.ev[dev], 0));
+        HIPCHK(hipStreamWaitEvent(st, po.ev[dev], 0));
     }
-    po.last[dev] = c->stream;
-    return YC_OK;
+    po.last[dev] = st;
+    return launch();
+}
+
+// A stream about to be destroyed stops being its device's last persistent-solve stream.
+void persist_forget(int dev, hipStream_t st) {
+    if (dev < 0 || dev >= MAX_DEVICES || !st) return;
+    PersistOrder& po = persist_order();
+    std::lock_guard<std::mutex> lock(po.mu);
+    if (po.last[dev] == st) po.last[dev] = nullptr;
 }
 
 // With the block-Jacobi preconditioner the rows of the matrix solved are made current first.
@@ -1935,12 +1948,10 @@ int launch_cgp(yc_ctx* c, double tol, int maxit, const double* Aptr = nullptr) {
     a.W = c->W; a.Lg = c->Lg; a.Wp = c->Wp; a.P = c->P; a.ntx = c->ntx; a.nT = c->nT;
     void* args[] = {&a};
     const void* fn = bj ? cgp_kernel<true>(c->cg1_kp) : cgp_kernel<false>(c->cg1_kp);
-    PersistOrder& po = persist_order();
-    std::lock_guard<std::mutex> lock(po.mu);
-    int rc = persist_order_locked(c, po);
-    if (rc) return rc;
-    HIPCHK(hipLaunchKernel(fn, dim3(c->nT), dim3(BLK), args, 0, c->stream));
-    return YC_OK;
+    return persist_launch(c->device, c->stream, [&]() -> int {
+        HIPCHK(hipLaunchKernel(fn, dim3(c->nT), dim3(BLK), args, 0, c->stream));
+        return YC_OK;
+    });
 }
 
 template <int Q>
@@ -2431,11 +2442,7 @@ int yc_create(const yc_desc* desc, yc_ctx** out) {
 void yc_destroy(yc_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->device >= 0 && c->device < MAX_DEVICES) {   // the stream stops being the persistent solves' last
-        PersistOrder& po = persist_order();
-        std::lock_guard<std::mutex> lock(po.mu);
-        if (po.last[c->device] == c->stream) po.last[c->device] = nullptr;
-    }
+    persist_forget(c->device, c->stream);
     adjoint_free(c);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);

@@ -162,7 +162,7 @@ def batch_loss_terms_adjoint(specs, frames_list, names, mode="ref", tol=1e-10, p
     """``sample_loss_terms_adjoint`` for many cloths at once: the rollouts are recorded
     together and their backward passes run together (``ClothBatch.adjoint_run`` /
     ``adjoint_backward``: one launch per phase over all cloths, the solves in one
-    cooperative launch).  Returns a list of (loss_sum, grad_sum, steps_ok) per cloth.  If
+    launch).  Returns a list of (loss_sum, grad_sum, steps_ok) per cloth.  If
     a step of the batch fails (NaN/Inf, no convergence), every cloth is redone alone, so
     each stops at its own first failed step as the reference's rollouts do."""
     from .batch import ClothBatch

@@ -175,12 +175,10 @@ int yc_step(yc_ctx* ctx, double h, const double* x0, double tol, int maxit, yc_s
 /* YC_CG_PERSISTENT: the fused recurrences in one launch per solve; each 8 x 8 tile's
  * block keeps its rows of A in registers for the whole solve.  Needs every tile resident
  * at once (YC_ERR_ARG otherwise, e.g. above about 128 x 128 crossings).  The blocks wait
- * on one another, so the persistent solves of all contexts of a process run one at a time
- * per device (a solve queued on another context's stream waits for the work queued there
- * so far).  A batch's solves (yc_batch_*) are not part of that order: do not queue them
- * beside a single-cloth persistent solve on the same device from another thread.  Kernels
- * of other libraries or processes can only delay a solve; one delayed past the spin limit
- * (about a quarter second) fails with YC_ERR_HIP. */
+ * on one another, so the persistent solves of all contexts and batches of a process run
+ * one at a time per device (a solve queued on another stream waits for the work queued
+ * there so far).  Kernels of other libraries or processes can only delay a solve; one
+ * delayed past the spin limit (about a quarter second) fails with YC_ERR_HIP. */
 #define YC_CG_PERSISTENT 2
 int yc_set_cg_variant(yc_ctx* ctx, int variant);
 
@@ -396,9 +394,9 @@ int yc_adjoint_end(yc_ctx* ctx);
 
 /* ---------------------------------------------------------------------------------
  * Batches: many cloths of uneven size stepped together, each phase of a step one launch
- * over all of them (the persistent CG one cooperative launch per group of cloths that
+ * over all of them (the persistent CG one launch whose blocks walk groups of cloths that
  * fit on the device at once), replayed from a hipGraph where the runtime can capture
- * the cooperative launch.  No counterpart in the reference, which steps one Cloth per
+ * the step.  No counterpart in the reference, which steps one Cloth per
  * PhysicsCloth (src/TrainHelper.cpp:15-97) and loops over cloths in its training
  * drivers (src/TrainHeter.h:81-146).  Each cloth's arithmetic is yc_run's with the
  * persistent CG, bit for bit.
@@ -422,13 +420,13 @@ int yc_batch_get_opts(const yc_batch* b, yc_batch_opts* opts);
  * over the steps; status[i] (may be NULL): its worst status.  Returns the worst status
  * over all cloths; yc_last_error names the cloth. */
 int yc_batch_run(yc_batch* b, double h, int steps, double tol, int maxit, long long* iters, int* status);
-/* Layout facts: cooperative groups, whether the last run replayed a graph, tiles of all
+/* Layout facts: resident groups, whether the last run replayed a graph, tiles of all
  * cloths, tiles resident at once. */
 int yc_batch_info(const yc_batch* b, int* n_groups, int* graph_used, int* total_tiles, int* capacity);
 /* Reverse mode of a batch: the members' adjoint recordings (yc_adjoint_begin on each, with
  * checkpoint interval 1 and the same number of steps recorded) advanced together and
  * their backward passes run together, each phase one launch over all cloths and the
- * backward solves one cooperative launch; each cloth's gradient is its single-cloth
+ * backward solves one launch; each cloth's gradient is its single-cloth
  * backward pass's.  No counterpart in the reference, whose training drivers differentiate
  * one cloth per pass (src/TrainHeter.h:161-231).
  * yc_batch_adjoint_run: `steps` recorded SimSteps of every member (as yc_batch_run); a

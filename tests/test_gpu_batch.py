@@ -92,7 +92,7 @@ def test_batch_equals_cloths_alone(ys, cs, graph, order, nb, precond):
 @pytest.mark.parametrize("precond", ["jacobi", "block2"])
 def test_batch_uneven_64_cloths_several_groups(ys, cs, precond):
     """The 64-cloth workload (sides 10..64, 1,743 tiles) needs more tiles than the
-    device holds at once, so its solves run as several cooperative groups; two runs
+    device holds at once, so its solves run as several resident groups; two runs
     of one step each equal the cloths stepped alone."""
     items = cs.batch_workload()
     h, tol = items[0][0]["h"], 1e-8
