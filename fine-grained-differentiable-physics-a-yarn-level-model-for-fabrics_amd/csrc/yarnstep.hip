@@ -1369,12 +1369,22 @@ __device__ __forceinline__ void bj_invert_store(int W, int Lg, int Wp, int P, do
     }
 }
 
-// From A (or A^T) in HBM: three groups per 64-thread block.
+// Block b of a grid of nb as the dispatcher places it: on XCD b % 8 of the eight, in turn.
+// Returns the block index that gives each XCD a contiguous range of [0, nb) (a bijection),
+// so that blocks whose data share cache lines run on the XCD whose L2 holds them.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// From A (or A^T) in HBM: three groups per 64-thread block.  The groups of a tile read
+// the same 64-byte lines of its slot planes (neighbouring nodes of one slot are adjacent),
+// so consecutive groups go to one XCD.
 __global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx,
                                                    int ngx, int ngroups) {
     __shared__ double colk[BJF_GROUPS][20];
     const int gl = threadIdx.x / 20, j = threadIdx.x - 20 * gl;
-    const int grp = blockIdx.x * BJF_GROUPS + gl;
+    const int grp = xcd_block(blockIdx.x, gridDim.x) * BJF_GROUPS + gl;
     const bool live = gl < BJF_GROUPS && grp < ngroups;
     const int gu = live ? (grp % ngx) * 2 : 0, gv = live ? (grp / ngx) * 2 : 0;
     bj_invert_store(W, Lg, Wp, P, brow, live, gu, gv, gl, j, colk, [&](int qi, int, int s) {
