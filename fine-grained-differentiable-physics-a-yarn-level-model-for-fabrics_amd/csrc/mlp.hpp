@@ -417,15 +417,16 @@ __device__ __forceinline__ void stage_features(const FeatArgs& a, int k, int r, 
         if (in_group(G, j)) xs[r * MLP_LDX + j] = f2bf(k < a.N ? (f[j] - in_mean[j]) * in_istd[j] : 0.0f);
 }
 
-// The learned warm start in one launch per 64 nodes: features (wavefront G gathers group
-// G of every row), standardisation, the network (the same tile forward as k_mlp_fwd, so
-// the same bits), and x0 += out_std * y on the node's DOFs (x0 holds x_q on entry).
-__global__ void __launch_bounds__(256) k_mlp_warm(FeatArgs a, MlpW w, const float* in_mean, const float* in_istd,
-                                                   const float* out_std, double* x0) {
+// The learned warm start of nodes [64 tile, 64 tile + 64) by a 256-thread block: features
+// (wavefront G gathers group G of every row), standardisation, the network (the same tile
+// forward as k_mlp_fwd, so the same bits), and x0 += out_std * y on the node's DOFs (x0
+// holds x_q on entry).
+__device__ __forceinline__ void mlp_warm_tile(const FeatArgs& a, const MlpW& w, const float* in_mean, const float* in_istd,
+                                              const float* out_std, double* x0, int tile) {
     __shared__ unsigned short xs[MLP_TILE * MLP_LDX];
     __shared__ unsigned short h0[MLP_TILE * MLP_LDH];
     __shared__ unsigned short h1[MLP_TILE * MLP_LDH];
-    const int base = blockIdx.x * MLP_TILE;
+    const int base = tile * MLP_TILE;
     {
         const int r = threadIdx.x & (MLP_TILE - 1), k = base + r;
         switch (threadIdx.x / MLP_TILE) {
@@ -444,6 +445,12 @@ __global__ void __launch_bounds__(256) k_mlp_warm(FeatArgs a, MlpW w, const floa
                  const bool inter = u > 0 && u < a.W - 1 && v > 0 && v < a.Lg - 1;
                  if (c < 3 || inter) x0[(size_t)c * a.P + padidx(a.Wp, u, v)] += (double)y * (double)out_std[c];
              });
+}
+
+// The learned warm start in one launch per 64 nodes.
+__global__ void __launch_bounds__(256) k_mlp_warm(FeatArgs a, MlpW w, const float* in_mean, const float* in_istd,
+                                                   const float* out_std, double* x0) {
+    mlp_warm_tile(a, w, in_mean, in_istd, out_std, x0, blockIdx.x);
 }
 
 // padded v_old (x, y, z, u, v; boundary u, v zero)

@@ -1377,19 +1377,46 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-// From A (or A^T) in HBM: three groups per 64-thread block.  The groups of a tile read
-// the same 64-byte lines of its slot planes (neighbouring nodes of one slot are adjacent),
-// so consecutive groups go to one XCD.
-__global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx,
-                                                   int ngx, int ngroups) {
-    __shared__ double colk[BJF_GROUPS][20];
-    const int gl = threadIdx.x / 20, j = threadIdx.x - 20 * gl;
-    const int grp = xcd_block(blockIdx.x, gridDim.x) * BJF_GROUPS + gl;
+// From A (or A^T) in HBM: three groups per wavefront; `unit` numbers the wavefront's
+// three groups, `lane` is the thread's lane, colk the wavefront's own pivot columns.  Every
+// wavefront of the block calls it (it passes the block's barriers).
+__device__ __forceinline__ void bj_factor_unit(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx, int ngx,
+                                               int ngroups, int unit, int lane, double (*colk)[20]) {
+    const int gl = lane / 20, j = lane - 20 * gl;
+    const int grp = unit * BJF_GROUPS + gl;
     const bool live = gl < BJF_GROUPS && grp < ngroups;
     const int gu = live ? (grp % ngx) * 2 : 0, gv = live ? (grp / ngx) * 2 : 0;
     bj_invert_store(W, Lg, Wp, P, brow, live, gu, gv, gl, j, colk, [&](int qi, int, int s) {
         return A[a_base(ntx, gu + (qi & 1), gv + (qi >> 1)) + (size_t)s * TILE_N];
     });
+}
+
+// One wavefront per block.  The groups of a tile read the same 64-byte lines of its slot
+// planes (neighbouring nodes of one slot are adjacent), so consecutive groups go to one XCD.
+__global__ void __launch_bounds__(64) k_bj_factor(const double* A, double* brow, int W, int Lg, int Wp, int P, int ntx,
+                                                   int ngx, int ngroups) {
+    __shared__ double colk[BJF_GROUPS][20];
+    bj_factor_unit(A, brow, W, Lg, Wp, P, ntx, ngx, ngroups, xcd_block(blockIdx.x, gridDim.x), threadIdx.x, colk);
+}
+
+// The block-Jacobi rows of A and the learned warm start in one launch, so that they run side
+// by side without a second stream: blocks [0, nbj) factor twelve groups each (four
+// wavefronts of three, consecutive blocks of an XCD on consecutive groups), the others run
+// the network on 64 nodes each.  Both write what their own kernels write, bit for bit.
+struct BjArgs {
+    const double* A; double* brow;
+    int W, Lg, Wp, P, ntx, ngx, ngroups, nbj;
+};
+__global__ void __launch_bounds__(256) k_mlp_warm_bj(FeatArgs a, MlpW w, const float* in_mean, const float* in_istd,
+                                                      const float* out_std, double* x0, BjArgs bj) {
+    if ((int)blockIdx.x < bj.nbj) {
+        __shared__ double colk[4][BJF_GROUPS][20];
+        const int wv = threadIdx.x >> 6;
+        bj_factor_unit(bj.A, bj.brow, bj.W, bj.Lg, bj.Wp, bj.P, bj.ntx, bj.ngx, bj.ngroups,
+                       4 * xcd_block(blockIdx.x, bj.nbj) + wv, threadIdx.x & 63, colk[wv]);
+    } else {
+        mlp_warm_tile(a, w, in_mean, in_istd, out_std, x0, blockIdx.x - bj.nbj);
+    }
 }
 
 
@@ -1678,9 +1705,6 @@ struct yc_ctx {
     double* bj;                             // block-Jacobi rows, 5 P x 20 (when selected)
     const double* bj_of;                    // the matrix (A or At) bj was computed from, or null
     hipEvent_t io_event;                    // orders device state transfers with a caller's stream
-    hipStream_t side;                       // queued steps: the block-Jacobi rows beside the guess
-    hipEvent_t ev_side[2];                  // ... after the assembly / rows done
-    bool side_pending;                      // rows launched on `side`, not yet joined into `stream`
     hipGraphExec_t graph1;
     // forward-mode tangents of the step (directional derivatives)
     int ndir;
@@ -1874,17 +1898,12 @@ const void* cgp_kernel(int kp) {
 
 // The block-Jacobi rows of the matrix Aptr (k_bj_factor), unless they are already those
 // of Aptr: c->bj_of names the matrix the rows were computed from (the assembly computes
-// them for the new A; everything else that rewrites A or At clears it).
-int bj_factor(yc_ctx* c, const double* Aptr, hipStream_t st = nullptr) {
-    if (c->bj_of == Aptr) {
-        if (c->side_pending) {   // rows computed on the side stream: the context's stream waits for them
-            HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side[1], 0));
-            c->side_pending = false;
-        }
-        return YC_OK;
-    }
+// them for the new A, or the warm guess beside its network; everything else that rewrites
+// A or At clears it).
+int bj_factor(yc_ctx* c, const double* Aptr) {
+    if (c->bj_of == Aptr) return YC_OK;
     const int ngx = (c->W + 1) / 2, ngy = (c->Lg + 1) / 2, ng = ngx * ngy;
-    hipLaunchKernelGGL(k_bj_factor, dim3((ng + BJF_GROUPS - 1) / BJF_GROUPS), dim3(64), 0, st ? st : c->stream, Aptr, c->bj, c->W,
+    hipLaunchKernelGGL(k_bj_factor, dim3((ng + BJF_GROUPS - 1) / BJF_GROUPS), dim3(64), 0, c->stream, Aptr, c->bj, c->W,
                        c->Lg, c->Wp, c->P, c->ntx, ngx, ng);
     HIPCHK(hipGetLastError());
     c->bj_of = Aptr;
@@ -2020,11 +2039,10 @@ void from_padded(const yc_ctx* c, const std::vector<double>& pad, double* glob) 
         }
 }
 
-// async: no host synchronisation; the flags are sticky and read by the caller later
-// side: with the block-Jacobi preconditioner, the rows are computed on c->side after the
-// assembly (c->ev_side[1] marks them done) instead of in order on the context's stream;
-// the first bj_factor() call for them (the solve's) joins them into the context's stream.
-int do_assemble(yc_ctx* c, double h, bool async = false, bool side = false) {
+// async: no host synchronisation; the flags are sticky and read by the caller later.
+// With the block-Jacobi preconditioner the rows of the new A follow the assembly, unless
+// defer_bj: then their first user computes them (the warm guess, beside its network).
+int do_assemble(yc_ctx* c, double h, bool async = false, bool defer_bj = false) {
     if (!async) HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     c->cd = make_dev(c);
     c->last_h = h;
@@ -2040,17 +2058,9 @@ int do_assemble(yc_ctx* c, double h, bool async = false, bool side = false) {
     hipLaunchKernelGGL(k_assemble2, dim3(c->Lg * ((c->W + TILE_N - 1) / TILE_N)), dim3(ASM2_THREADS), 0, c->stream, c->cd, make_out(c), h);
     HIPCHK(hipGetLastError());
     c->bj_of = nullptr;   // A is new
-    if (c->precond == YC_PRECOND_BLOCK2) {
-        if (side) {
-            HIPCHK(hipEventRecord(c->ev_side[0], c->stream));
-            HIPCHK(hipStreamWaitEvent(c->side, c->ev_side[0], 0));
-        }
-        int rc = bj_factor(c, c->A, side ? c->side : nullptr);
+    if (c->precond == YC_PRECOND_BLOCK2 && !defer_bj) {
+        int rc = bj_factor(c, c->A);
         if (rc) return rc;
-        if (side) {
-            HIPCHK(hipEventRecord(c->ev_side[1], c->side));
-            c->side_pending = true;
-        }
     }
     if (async) {
         HIPCHK(hipGetLastError());
@@ -2467,10 +2477,6 @@ void yc_destroy(yc_ctx* c) {
     if (c->tdev) (void)hipFree(c->tdev);
     if (c->tout) (void)hipFree(c->tout);
     if (c->io_event) (void)hipEventDestroy(c->io_event);
-    if (c->side) (void)hipStreamSynchronize(c->side);
-    for (hipEvent_t e : c->ev_side)
-        if (e) (void)hipEventDestroy(e);
-    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2901,19 +2907,11 @@ int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, 
     HIPCHK(hipMemsetAsync(c->flag, 0, 2 * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync((char*)c->sc + offsetof(CGScal, total_iters), 0, sizeof(long long) + 2 * sizeof(int), c->stream));
     const bool x0 = m != nullptr || guess != YC_GUESS_ZERO;
-    // with the network and the block-Jacobi preconditioner, the preconditioner's rows (from
-    // A) and the network's guess (from A, b and the state) are independent: the rows go to a
-    // second stream of the same (default) priority, and the solve's launch waits for them.
-    // The next step's rows wait for its assembly, so nothing else runs beside the
-    // persistent solve.  (The join costs about as much as a cheap guess saves, so the
-    // other guesses keep the rows in order.)
-    const bool side = m != nullptr && c->precond == YC_PRECOND_BLOCK2;
-    if (side && !c->side) {
-        HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-        for (hipEvent_t& e : c->ev_side) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
+    // with the network and the block-Jacobi preconditioner, the rows (from A) and the
+    // network's guess (from A, b and the state) are independent, and warm_guess computes
+    // them in one launch side by side
     for (int s = 0; s < steps; ++s) {
-        int rc = do_assemble(c, h, true, side);
+        int rc = do_assemble(c, h, true, m != nullptr);
         if (!rc && m) rc = warm_guess(c, m);
         if (!rc && !m && guess != YC_GUESS_ZERO) {
             hipLaunchKernelGGL(k_guess_x0, dim3(grid_for(5LL * c->P)), dim3(BLK), 0, c->stream, (const double*)c->vpad,
@@ -2922,11 +2920,7 @@ int run_async(yc_ctx* c, yc_mlp* m, double h, int steps, double tol, int maxit, 
         }
         if (!rc) rc = do_solve(c, nullptr, tol, maxit, nullptr, nullptr, x0, true);
         if (!rc) rc = do_update(c, h);
-        if (rc) {   // no side work outlives the run
-            if (c->side) (void)hipStreamSynchronize(c->side);
-            c->side_pending = false;
-            return rc;
-        }
+        if (rc) return rc;
     }
     HIPCHK(hipMemcpyAsync(c->h_sc, c->sc, sizeof(CGScal), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(c->h_flag, c->flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -3325,10 +3319,26 @@ int extrapolate_and_apply(yc_ctx* c, double* xe) {
 
 // The network's initial guess in c->cx: x_q, its stencil product, then one launch of
 // features + network + correction.
+// With the block-Jacobi preconditioner and rows not yet those of A, the rows are computed
+// in the network's launch (k_mlp_warm_bj), beside it.
 int warm_guess(yc_ctx* c, yc_mlp* m) {
     int rc = extrapolate_and_apply(c, c->cx);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_mlp_warm, dim3((c->N + MLP_TILE - 1) / MLP_TILE), dim3(256), 0, c->stream, feat_args(c), mlp_w(m),
+    const int nmlp = (c->N + MLP_TILE - 1) / MLP_TILE;
+    if (c->precond == YC_PRECOND_BLOCK2 && c->bj_of != c->A) {
+        BjArgs bj;
+        bj.A = c->A; bj.brow = c->bj;
+        bj.W = c->W; bj.Lg = c->Lg; bj.Wp = c->Wp; bj.P = c->P; bj.ntx = c->ntx;
+        bj.ngx = (c->W + 1) / 2;
+        bj.ngroups = bj.ngx * ((c->Lg + 1) / 2);
+        bj.nbj = (bj.ngroups + 4 * BJF_GROUPS - 1) / (4 * BJF_GROUPS);
+        hipLaunchKernelGGL(k_mlp_warm_bj, dim3(bj.nbj + nmlp), dim3(256), 0, c->stream, feat_args(c), mlp_w(m),
+                           (const float*)m->in_mean, (const float*)m->in_istd, (const float*)m->out_std, c->cx, bj);
+        HIPCHK(hipGetLastError());
+        c->bj_of = c->A;
+        return YC_OK;
+    }
+    hipLaunchKernelGGL(k_mlp_warm, dim3(nmlp), dim3(256), 0, c->stream, feat_args(c), mlp_w(m),
                        (const float*)m->in_mean, (const float*)m->in_istd, (const float*)m->out_std, c->cx);
     HIPCHK(hipGetLastError());
     return YC_OK;
@@ -3618,7 +3628,7 @@ int yc_step_warm(yc_ctx* c, yc_mlp* m, double h, double tol, int maxit, yc_step_
     g_err.clear();
     int it = 0;
     double rr = 0;
-    int rc = do_assemble(c, h);
+    int rc = do_assemble(c, h, false, true);
     if (!rc) rc = warm_guess(c, m);
     if (!rc) rc = do_solve(c, nullptr, tol, maxit, &it, &rr, true);
     if (info) { info->cg_iters = it; info->cg_relres = rr; info->status = rc; }
