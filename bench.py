@@ -334,6 +334,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="time only the zero-guess CG path (profiling runs)")
+    ap.add_argument("--warm-only", action="store_true",
+                    help="time only the learned warm-start line (`value`'s path; profiling runs), print its steps/s")
     ap.add_argument("--batch-steps", type=int, default=20, help="steps of the 64-cloth batch line (0: skip it)")
     ap.add_argument("--large-steps", type=int, default=10, help="steps of the 256x256 line (0: skip it)")
     ap.add_argument("--tangent-steps", type=int, default=10,
@@ -419,6 +421,17 @@ def main():
         if rc != 0:
             print("warning: a CG solve stopped at maxit above tolerance", file=sys.stderr)
         return cloth, dt, iters
+
+    if args.warm_only and net is not None:
+        cloth, dt_w, it_w = timed(True)
+        cloth.close()
+        net.close()
+        import torch
+        torch.cuda.synchronize()
+        print(json.dumps({"warm_only": {"workload": args.workload, "steps": args.steps, "warmup": args.warmup,
+                                        "steps_per_s": args.steps / dt_w, "cg_iters_per_step": it_w / args.steps}}),
+              flush=True)
+        return
 
     ref_form = None
     if args.cg != "reference" and not args.plain_only:
