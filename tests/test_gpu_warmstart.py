@@ -181,3 +181,45 @@ def test_warm_start_on_cloth(ys, cs, ws):
     net.close()
     for c in cl.values():
         c.close()
+
+
+def test_warm_step_in_one_launch_equals_separate_kernels(ys, cs, ws):
+    """With the block-Jacobi preconditioner a warm step factors the rows in the network's
+    launch (k_mlp_warm_bj: four groups per block, blocks grouped by XCD).  The step equals,
+    bit for bit, the same step made of the separate kernels: the assembly with its rows from
+    k_bj_factor, the guess of k_mlp_warm, the solve from that guess, the update.  The CG
+    iterates depend on every bit of the rows, so this pins them; a cloth with partial tiles,
+    odd sides and wind."""
+    spec = cs.heter_spec(21, "satin", 5)
+    spec["length"] = 19
+    spec["wind_vel"] = [0.5, 3.0, 0.0]
+    c0 = ys.Cloth(spec)
+    c0.set_ClothState(cs.initial_state(spec))
+    c0.run(spec["h"], 3, tol=1e-10)
+    st = c0.get_ClothState()
+    c0.close()
+    h = spec["h"]
+    rng = np.random.default_rng(4)
+    net = ws.WarmStartMLP(params=ws.init_params(3) + rng.normal(0, 0.05, 37893).astype(np.float32))
+    net.set_norm(np.zeros(32, np.float32), np.ones(32, np.float32), np.full(5, 1e-4, np.float32))
+    a, b = ys.Cloth(spec), ys.Cloth(spec)
+    for c in (a, b):
+        c.set_cg_variant("persistent")
+        c.set_preconditioner("block2")
+        c.set_ClothState(st)
+    ia, ra, rc = a.step_warm(net, h, tol=1e-10)
+    assert rc == 0
+    b.assemble(h)
+    rows = b.precond_rows()
+    b.warm_guess(net)
+    guess = b.vec("x")
+    ib, rb, ok = b.solve(tol=1e-10, x0=guess)
+    b.update(h)
+    say("one launch: %d iterations, separate kernels: %d; rows nonzero %d" % (ia, ib, np.count_nonzero(rows)))
+    assert ok and ia == ib and np.count_nonzero(rows) > 0
+    sa, sb = a.get_ClothState(), b.get_ClothState()
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    for c in (a, b):
+        c.close()
+    net.close()
