@@ -1452,21 +1452,43 @@ __device__ __forceinline__ void update_node(const StateDev& s, const double* sol
     }
 }
 
-// With dprev set, also the node's velocity change of this solve relative to the velocity it
-// started from (dprev = sol - vpad, padded layout); the previous change moves to dprev2.
-// Padding entries are never written: sol and vpad are zero there, so are dprev and dprev2.
+// Component c of node k's update, in update_node's arithmetic.
+__device__ __forceinline__ void update_comp(const StateDev& s, const double* sol, const double* is_slide, int Wp, int P,
+                                            double h, int k, int c) {
+    const int u = k % s.W, v = k / s.W;
+    const int pk = padidx(Wp, u, v);
+    if (c < 3) {
+        const bool fixed = (s.flags[k] & 2) != 0;
+        double nv = sol[c * P + pk];
+        s.v[c * s.N + k] = nv;
+        double nx = s.x[c * s.N + k] + nv * h;
+        s.x[c * s.N + k] = nx;
+        if (!fixed) s.xfix[c * s.N + k] = nx;
+    } else if (u > 0 && u < s.W - 1 && v > 0 && v < s.Lg - 1) {
+        const int e = c - 3;
+        double nv = sol[(3 + e) * P + pk];
+        s.uvd[e * s.N + k] = nv;
+        s.uv[e * s.N + k] = s.uv[e * s.N + k] + nv * h;
+        s.uvbar[e * s.N + k] = s.uvbar[e * s.N + k] + is_slide[e * s.N + k] * nv * h;
+    }
+}
+
+// One thread per (component, node), component-major (5 N threads, so that the loads of a
+// wavefront are consecutive entries of one plane).  With dprev set, also the entry's
+// velocity change of this solve relative to the velocity it started from (dprev = sol -
+// vpad, padded layout); the previous change moves to dprev2.  Padding entries are never
+// written: sol and vpad are zero there, so are dprev and dprev2.
 __global__ void __launch_bounds__(NBLK) k_update(StateDev s, const double* sol, const double* is_slide, int Wp, int P, double h,
                                                  const double* vpad = nullptr, double* dprev = nullptr, double* dprev2 = nullptr) {
-    const int k = blockIdx.x * NBLK + threadIdx.x;
-    if (dprev && k < s.N) {
-        const int pk = padidx(Wp, k % s.W, k / s.W);
-        for (int c = 0; c < 5; ++c) {
-            const size_t i = (size_t)c * P + pk;
-            dprev2[i] = dprev[i];
-            dprev[i] = sol[i] - vpad[i];
-        }
+    const int t = blockIdx.x * NBLK + threadIdx.x;
+    if (t >= 5 * s.N) return;
+    const int c = t / s.N, k = t - c * s.N;
+    if (dprev) {
+        const size_t i = (size_t)c * P + padidx(Wp, k % s.W, k / s.W);
+        dprev2[i] = dprev[i];
+        dprev[i] = sol[i] - vpad[i];
     }
-    update_node(s, sol, is_slide, Wp, P, h, k);
+    update_comp(s, sol, is_slide, Wp, P, h, k, c);
 }
 
 // Cloth::NodeNorm (src/Cloth.cpp:440-463): smallest-eigenvalue eigenvector of the 3x3
@@ -2150,7 +2172,7 @@ int do_solve(yc_ctx* c, const double* x0_global, double tol, int maxit, int* ite
 
 int do_update(yc_ctx* c, double h) {
     if (!c->solved) return fail(YC_ERR_ARG, "yc_update before yc_solve");
-    hipLaunchKernelGGL(k_update, dim3((c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), (const double*)c->cx,
+    hipLaunchKernelGGL(k_update, dim3((5 * c->N + NBLK - 1) / NBLK), dim3(NBLK), 0, c->stream, make_state(c), (const double*)c->cx,
                        (const double*)c->is_slide, c->Wp, c->P, h, (const double*)c->vpad, c->dprev, c->dprev2);
     HIPCHK(hipGetLastError());
     int rc = refresh_geometry(c);
